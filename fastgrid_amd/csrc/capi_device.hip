@@ -1,0 +1,710 @@
+// Device half of the libfgmaps C ABI (include/fgmaps.h): contexts, device-
+// resident jobs, launches and timing.  The host data layouts the kernels need
+// (atom records, spatially sorted electrostatic tiles, coarse cells, guard
+// bands) are built here from an fg_job_desc.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "capi_common.hpp"
+#include "fgmaps.h"
+#include "kernels.hip"
+
+using fgk::CutAtom;
+
+namespace {
+
+constexpr int kPZ = 4;  // points per lane along z in k_elec
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+#define FG_HIP(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fgc::fail(FG_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+}  // namespace
+
+struct fg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    fg_timing last{0, 0, 0};
+    bool haveTiming = false;
+    bool cutoffRan = false, elecRan = false;
+};
+
+struct fg_job {
+    fg_ctx *ctx = nullptr;
+    fgk::GridGeom g{};
+    int64_t P = 0;
+    int N = 0, M = 0, T = 0;
+    int distDepDiel = 1, hydrogen = -1;
+    bool hasGeom = false;
+    std::vector<fgk::MapParam> mp;
+    // cutoff inputs
+    CutAtom *dAtoms = nullptr;
+    double *dPair = nullptr, *dDesolv = nullptr, *dEps = nullptr, *dCovalent = nullptr;
+    double *dXYZ = nullptr;
+    int32_t *dHbond = nullptr, *dCellStart = nullptr, *dCellAtoms = nullptr;
+    fgk::CellGrid cg{};
+    int nbx = 0, nby = 0, nbz = 0, nb = 0;
+    int32_t *dCounts = nullptr, *dStart = nullptr, *dHStart = nullptr, *dList = nullptr, *dHList = nullptr;
+    // electrostatic tiles
+    float4 *dTiles = nullptr;
+    double *dTiles64 = nullptr;
+    float *dTileBox = nullptr, *dEps32 = nullptr;
+    int nTiles = 0, tableLen = fgk::kTableLen;
+    bool clamp = true;
+    float farCut2 = 0, r2Lo = 0, r2Hi = 0, spacing100 = 0;
+    // outputs: [M affinity][e][d]
+    double *dMaps = nullptr;
+    std::vector<void *> owned;
+};
+
+namespace {
+
+template <class T>
+int devAlloc(fg_job *j, T *&ptr, size_t count)
+{
+    ptr = nullptr;
+    if (count == 0) count = 1;
+    void *p = nullptr;
+    FG_HIP(hipMalloc(&p, count * sizeof(T)));
+    j->owned.push_back(p);
+    ptr = static_cast<T *>(p);
+    return FG_OK;
+}
+
+template <class T>
+int devUpload(fg_job *j, T *&ptr, const T *host, size_t count)
+{
+    if (int rc = devAlloc(j, ptr, count)) return rc;
+    if (count) FG_HIP(hipMemcpy(ptr, host, count * sizeof(T), hipMemcpyHostToDevice));
+    return FG_OK;
+}
+
+uint32_t spread3(uint32_t v)  // 10 bits -> every third bit
+{
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// Launches the candidate-list count pass and the two scans.
+int runListCount(fg_job *j, const fgk::ListArgs &la)
+{
+    hipLaunchKernelGGL(fgk::k_brick_lists<false>, dim3(j->nb), dim3(fgk::kListThreads), 0, j->ctx->stream, la);
+    hipLaunchKernelGGL(fgk::k_scan_counts, dim3(1), dim3(1024), 0, j->ctx->stream, j->dCounts, j->nb, 0, j->dStart);
+    hipLaunchKernelGGL(fgk::k_scan_counts, dim3(1), dim3(1024), 0, j->ctx->stream, j->dCounts, j->nb, 1, j->dHStart);
+    FG_HIP(hipGetLastError());
+    return FG_OK;
+}
+
+fgk::ListArgs listArgs(const fg_job *j)
+{
+    fgk::ListArgs la{};
+    la.g = j->g;
+    la.cg = j->cg;
+    la.nbx = j->nbx;
+    la.nby = j->nby;
+    la.nbz = j->nbz;
+    la.N = j->N;
+    la.xyz = j->dXYZ;
+    la.hbond = j->dHbond;
+    la.cellStart = j->dCellStart;
+    la.cellAtoms = j->dCellAtoms;
+    la.counts = j->dCounts;
+    la.start = j->dStart;
+    la.hStart = j->dHStart;
+    la.list = j->dList;
+    la.hList = j->dHList;
+    return la;
+}
+
+fgk::CutoffArgs cutoffArgs(const fg_job *j, int doElecNear)
+{
+    fgk::CutoffArgs a{};
+    a.g = j->g;
+    a.nbx = j->nbx;
+    a.nby = j->nby;
+    a.nbz = j->nbz;
+    a.M = j->M;
+    a.hydrogen = j->hydrogen;
+    a.doElecNear = doElecNear;
+    a.P = j->P;
+    a.atoms = j->dAtoms;
+    a.brickStart = j->dStart;
+    a.brickList = j->dList;
+    a.hStart = j->dHStart;
+    a.hList = j->dHList;
+    a.pair = j->dPair;
+    a.desolv = j->dDesolv;
+    a.eps = j->dEps;
+    a.maps = j->dMaps;
+    a.covalent = j->dCovalent;
+    for (int m = 0; m < j->M && m < 16; ++m) a.mp[m] = j->mp[size_t(m)];
+    return a;
+}
+
+int launchCutoff(fg_job *j, bool affinity, int doElecNear)
+{
+    fgk::CutoffArgs a = cutoffArgs(j, doElecNear);
+    dim3 grid(unsigned(j->nb)), block(fgk::kCutThreads);
+    hipStream_t s = j->ctx->stream;
+    if (!affinity) hipLaunchKernelGGL((fgk::k_cutoff<1, false>), grid, block, 0, s, a);
+    else if (j->M <= 2) hipLaunchKernelGGL((fgk::k_cutoff<2, true>), grid, block, 0, s, a);
+    else if (j->M <= 4) hipLaunchKernelGGL((fgk::k_cutoff<4, true>), grid, block, 0, s, a);
+    else if (j->M <= 8) hipLaunchKernelGGL((fgk::k_cutoff<8, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((fgk::k_cutoff<16, true>), grid, block, 0, s, a);
+    FG_HIP(hipGetLastError());
+    return FG_OK;
+}
+
+int launchElec(fg_job *j, int precision)
+{
+    hipStream_t s = j->ctx->stream;
+    double *e = j->dMaps + int64_t(j->M) * j->P;
+    if (precision == FG_PREC_FP64) {
+        dim3 grid(unsigned((j->g.nx + 15) / 16), unsigned((j->g.ny + 15) / 16), unsigned(j->g.nz));
+        hipLaunchKernelGGL(fgk::k_elec_fp64, grid, dim3(256), 0, s, j->g, j->N, j->dAtoms, j->dEps, j->distDepDiel, e);
+        FG_HIP(hipGetLastError());
+        return FG_OK;
+    }
+    fgk::ElecArgs a{};
+    a.g = j->g;
+    a.nTiles = j->nTiles;
+    a.tableLen = j->tableLen;
+    a.mode = precision;
+    a.P = j->P;
+    a.tiles = j->dTiles;
+    a.tiles64 = j->dTiles64;
+    a.tileBox = j->dTileBox;
+    a.eps32 = j->dEps32;
+    a.e = e;
+    a.farCut2 = j->farCut2;
+    a.r2Lo = j->r2Lo;
+    a.r2Hi = j->r2Hi;
+    a.spacing100 = j->spacing100;
+    dim3 grid(unsigned((j->g.nx + fgk::kElecBX - 1) / fgk::kElecBX), unsigned((j->g.ny + fgk::kElecBY - 1) / fgk::kElecBY),
+              unsigned((j->g.nz + kPZ - 1) / kPZ));
+    size_t lds = size_t(j->tableLen) * sizeof(float);
+    const bool comp = precision == FG_PREC_COMPENSATED;
+    if (comp && j->clamp) hipLaunchKernelGGL((fgk::k_elec<kPZ, true, true>), grid, dim3(fgk::kElecThreads), lds, s, a);
+    else if (comp) hipLaunchKernelGGL((fgk::k_elec<kPZ, true, false>), grid, dim3(fgk::kElecThreads), lds, s, a);
+    else if (j->clamp) hipLaunchKernelGGL((fgk::k_elec<kPZ, false, true>), grid, dim3(fgk::kElecThreads), lds, s, a);
+    else hipLaunchKernelGGL((fgk::k_elec<kPZ, false, false>), grid, dim3(fgk::kElecThreads), lds, s, a);
+    FG_HIP(hipGetLastError());
+    return FG_OK;
+}
+
+// Electrostatic far-field tiles: atoms in Morton order of 2 A cells, kTile
+// per tile, coordinates and w scaled by 100 (FP32), plus FP64 coordinates
+// for the exact near/far predicate and outward-rounded tile boxes.  Also the
+// guard band of the near/far predicate (see ElecArgs).
+int buildElecTiles(fg_job *j, const fg_job_desc &d, double gridMaxAbs)
+{
+    const int N = j->N, TT = fgk::kTile;
+    const fg_atoms &A = d.atoms;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < N; ++i) {
+        const double c[3] = {A.x[i], A.y[i], A.z[i]};
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], c[k]);
+            hi[k] = std::max(hi[k], c[k]);
+        }
+    }
+    std::vector<std::pair<uint32_t, int>> order(static_cast<size_t>(N));
+    for (int i = 0; i < N; ++i) {
+        uint32_t c[3];
+        const double v[3] = {A.x[i], A.y[i], A.z[i]};
+        for (int k = 0; k < 3; ++k) c[k] = uint32_t(std::min(1023.0, std::floor((v[k] - lo[k]) / 2.0)));
+        order[size_t(i)] = {spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2), i};
+    }
+    std::stable_sort(order.begin(), order.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+    j->nTiles = (N + TT - 1) / TT;
+    const size_t NT = size_t(j->nTiles) * TT;
+    std::vector<float4> tiles(NT);
+    std::vector<double> t64(NT * 3);
+    std::vector<float> box(size_t(j->nTiles) * 6);
+    const double s = j->g.spacing;
+    for (size_t q = 0; q < NT; ++q) {
+        double c[3], w;
+        if (q < size_t(N)) {
+            int i = order[q].second;
+            c[0] = A.x[i];
+            c[1] = A.y[i];
+            c[2] = A.z[i];
+            w = A.w[i];
+        } else {
+            // padding: zero charge at the centre of a grid cell next to the
+            // last atom, never on a grid point, inside the table's range
+            int i = order[size_t(N) - 1].second;
+            const double v[3] = {A.x[i], A.y[i], A.z[i]};
+            for (int k = 0; k < 3; ++k) c[k] = (std::floor(v[k] / s) + 0.5) * s;
+            w = 0;
+        }
+        tiles[q] = make_float4(float(c[0] * 100.0), float(c[1] * 100.0), float(c[2] * 100.0), float(w * 100.0));
+        for (int k = 0; k < 3; ++k) t64[3 * q + k] = c[k];
+    }
+    for (int t = 0; t < j->nTiles; ++t) {
+        float bl[3] = {1e30f, 1e30f, 1e30f}, bh[3] = {-1e30f, -1e30f, -1e30f};
+        for (int q = t * TT; q < (t + 1) * TT; ++q) {
+            const float v[3] = {tiles[size_t(q)].x, tiles[size_t(q)].y, tiles[size_t(q)].z};
+            for (int k = 0; k < 3; ++k) {
+                bl[k] = std::min(bl[k], v[k]);
+                bh[k] = std::max(bh[k], v[k]);
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            box[size_t(6 * t + k)] = std::nextafter(bl[k], -1e30f);
+            box[size_t(6 * t + 3 + k)] = std::nextafter(bh[k], 1e30f);
+        }
+    }
+    // Bound on |r2' - 1e4 * r2_fp64| for r near 8 A, in scaled units.
+    const double u = std::ldexp(1.0, -24);
+    double C = gridMaxAbs * 100.0;
+    for (int k = 0; k < 3; ++k) C = std::max(C, std::max(std::fabs(lo[k]), std::fabs(hi[k])) * 100.0);
+    const double dmax = 900.0;  // |d'| near the boundary (<= 9 A)
+    const double Ed = u * (2 * C + dmax) + 1e-9 * C;
+    const double B2 = 3 * u * 6.5e5 + 2 * dmax * std::sqrt(3.0) * Ed + 3 * Ed * Ed;
+    j->r2Lo = float(6.4e5 - 4 * B2 - 1.0);
+    j->r2Hi = float(6.4e5 + 4 * B2 + 1.0);
+    j->farCut2 = float(900.0 * 900.0);  // tile boxes farther than 9 A hold only far pairs
+    j->spacing100 = float(s * 100.0);
+    // dielectric table for k_elec: extended past 8191 (no clamp) when it fits
+    double span2 = 0;
+    for (int k = 0; k < 3; ++k) {
+        double gl = -gridMaxAbs, gh = gridMaxAbs;
+        double ext = std::max(std::fabs(hi[k] - gl), std::fabs(gh - lo[k]));
+        span2 += ext * ext;
+    }
+    const double tmax = std::sqrt(span2) * 100.0 * (1 + 1e-5) + 4;
+    std::vector<float> eps32;
+    const double *eps = d.tables.eps;
+    if (tmax + 2 <= 16384) {
+        j->tableLen = std::max(fgk::kTableLen, int(std::ceil(tmax)) + 2);
+        j->clamp = false;
+    } else {
+        j->tableLen = fgk::kTableLen;
+        j->clamp = true;
+    }
+    eps32.resize(size_t(j->tableLen));
+    for (int k = 0; k < j->tableLen; ++k) {
+        double v = d.dist_dep_diel ? eps[std::min(k, fgk::kTableLen - 1)] : 1.0;
+        eps32[size_t(k)] = float(v);
+    }
+    if (int rc = devUpload(j, j->dTiles, tiles.data(), tiles.size())) return rc;
+    if (int rc = devUpload(j, j->dTiles64, t64.data(), t64.size())) return rc;
+    if (int rc = devUpload(j, j->dTileBox, box.data(), box.size())) return rc;
+    if (int rc = devUpload(j, j->dEps32, eps32.data(), eps32.size())) return rc;
+    return FG_OK;
+}
+
+int buildCutoff(fg_job *j, const fg_job_desc &d)
+{
+    const int N = j->N;
+    const fg_atoms &A = d.atoms;
+    std::vector<CutAtom> atoms(size_t(std::max(N, 1)));
+    std::vector<double> xyz(size_t(3) * size_t(std::max(N, 1)));
+    std::vector<int32_t> hb(size_t(std::max(N, 1)), 0);
+    for (int i = 0; i < N; ++i) {
+        CutAtom c{};
+        c.x = A.x[i];
+        c.y = A.y[i];
+        c.z = A.z[i];
+        c.w = A.w ? A.w[i] : 0.0;
+        if (j->hasGeom) {
+            c.vol = A.vol[i];
+            c.solQ = A.solpar[i] + d.solpar_q * std::fabs(A.charge[i]);
+            c.qvol = d.solpar_q * A.vol[i];
+            for (int k = 0; k < 3; ++k) {
+                c.rv[k] = d.geom.rvector[3 * i + k];
+                c.rv2[k] = d.geom.rvector2[3 * i + k];
+            }
+            c.type = A.type[i];
+            c.hbond = A.hbond[i];
+            c.rexp = d.geom.rexp[i];
+            c.disorder = d.geom.disorder[i];
+        }
+        atoms[size_t(i)] = c;
+        xyz[size_t(3 * i)] = c.x;
+        xyz[size_t(3 * i + 1)] = c.y;
+        xyz[size_t(3 * i + 2)] = c.z;
+        hb[size_t(i)] = c.hbond;
+    }
+    if (int rc = devUpload(j, j->dAtoms, atoms.data(), atoms.size())) return rc;
+    if (int rc = devUpload(j, j->dXYZ, xyz.data(), xyz.size())) return rc;
+    if (int rc = devUpload(j, j->dHbond, hb.data(), hb.size())) return rc;
+    // coarse cells: edge >= 4 A and >= the brick edge, so a brick's 8 A
+    // neighbourhood spans at most 8 cells per axis
+    const double s = j->g.spacing;
+    double size = std::max({4.0, fgk::kCutBX * s, fgk::kCutBY * s, fgk::kCutBZ * s});
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    if (N > 0) {
+        lo[0] = hi[0] = A.x[0];
+        lo[1] = hi[1] = A.y[0];
+        lo[2] = hi[2] = A.z[0];
+    }
+    for (int i = 0; i < N; ++i) {
+        const double c[3] = {A.x[i], A.y[i], A.z[i]};
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], c[k]);
+            hi[k] = std::max(hi[k], c[k]);
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        j->cg.origin[k] = lo[k] - 1e-3;
+        j->cg.nc[k] = int(std::floor((hi[k] - j->cg.origin[k]) / size)) + 1;
+    }
+    j->cg.size = size;
+    const int nc = j->cg.nc[0] * j->cg.nc[1] * j->cg.nc[2];
+    std::vector<int32_t> cstart(size_t(nc) + 1, 0), catoms(size_t(std::max(N, 1)));
+    std::vector<int32_t> cell(size_t(std::max(N, 1)));
+    for (int i = 0; i < N; ++i) {
+        const double c[3] = {A.x[i], A.y[i], A.z[i]};
+        int ci[3];
+        for (int k = 0; k < 3; ++k)
+            ci[k] = std::min(j->cg.nc[k] - 1, std::max(0, int(std::floor((c[k] - j->cg.origin[k]) / size))));
+        cell[size_t(i)] = (ci[2] * j->cg.nc[1] + ci[1]) * j->cg.nc[0] + ci[0];
+        cstart[size_t(cell[size_t(i)]) + 1]++;
+    }
+    for (int c = 0; c < nc; ++c) cstart[size_t(c) + 1] += cstart[size_t(c)];
+    std::vector<int32_t> fill(cstart.begin(), cstart.end() - 1);
+    for (int i = 0; i < N; ++i) catoms[size_t(fill[size_t(cell[size_t(i)])]++)] = i;
+    if (int rc = devUpload(j, j->dCellStart, cstart.data(), cstart.size())) return rc;
+    if (int rc = devUpload(j, j->dCellAtoms, catoms.data(), catoms.size())) return rc;
+
+    j->nbx = (j->g.nx + fgk::kCutBX - 1) / fgk::kCutBX;
+    j->nby = (j->g.ny + fgk::kCutBY - 1) / fgk::kCutBY;
+    j->nbz = (j->g.nz + fgk::kCutBZ - 1) / fgk::kCutBZ;
+    j->nb = j->nbx * j->nby * j->nbz;
+    if (int rc = devAlloc(j, j->dCounts, size_t(2) * size_t(j->nb))) return rc;
+    if (int rc = devAlloc(j, j->dStart, size_t(j->nb) + 1)) return rc;
+    if (int rc = devAlloc(j, j->dHStart, size_t(j->nb) + 1)) return rc;
+    // size the candidate lists with one count pass
+    fgk::ListArgs la = listArgs(j);
+    if (int rc = runListCount(j, la)) return rc;
+    int32_t tot = 0, totH = 0;
+    FG_HIP(hipMemcpyAsync(&tot, j->dStart + j->nb, sizeof tot, hipMemcpyDeviceToHost, j->ctx->stream));
+    FG_HIP(hipMemcpyAsync(&totH, j->dHStart + j->nb, sizeof totH, hipMemcpyDeviceToHost, j->ctx->stream));
+    FG_HIP(hipStreamSynchronize(j->ctx->stream));
+    if (int rc = devAlloc(j, j->dList, size_t(tot))) return rc;
+    if (int rc = devAlloc(j, j->dHList, size_t(totH))) return rc;
+    // tables
+    const size_t pairN = size_t(std::max(d.tables.n_types, 1)) * fgk::kTableLen * size_t(std::max(j->M, 1));
+    std::vector<double> zeros;
+    if (j->hasGeom && d.tables.pair) {
+        if (int rc = devUpload(j, j->dPair, d.tables.pair, pairN)) return rc;
+    } else {
+        zeros.assign(pairN, 0.0);
+        if (int rc = devUpload(j, j->dPair, zeros.data(), zeros.size())) return rc;
+    }
+    std::vector<double> zt(fgk::kTableLen, 0.0);
+    if (int rc = devUpload(j, j->dDesolv, (j->hasGeom && d.tables.desolv) ? d.tables.desolv : zt.data(), zt.size()))
+        return rc;
+    std::vector<double> eps(fgk::kTableLen, 1.0);
+    if (d.dist_dep_diel) std::copy(d.tables.eps, d.tables.eps + fgk::kTableLen, eps.begin());
+    if (int rc = devUpload(j, j->dEps, eps.data(), eps.size())) return rc;
+    if (d.covalent && j->hasGeom) {
+        if (int rc = devUpload(j, j->dCovalent, d.covalent, size_t(j->M) * size_t(j->P))) return rc;
+    }
+    return FG_OK;
+}
+
+void destroyJob(fg_job *j)
+{
+    if (!j) return;
+    for (void *p : j->owned) (void)hipFree(p);
+    delete j;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fg_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int fg_create(int device, fg_ctx **out)
+{
+    if (!out) return fgc::fail(FG_E_ARG, "fg_create: out is NULL");
+    *out = nullptr;
+    int n = fg_device_count();
+    if (n <= 0) return fgc::fail(FG_E_DEVICE, "fg_create: no GPU is visible (libfgmaps has no CPU fallback)");
+    if (device < 0 || device >= n) return fgc::fail(FG_E_ARG, "fg_create: device index out of range");
+    FG_HIP(hipSetDevice(device));
+    fg_ctx *c = new fg_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fgc::fail(FG_E_DEVICE, "fg_create: hipStreamCreate failed");
+    }
+    for (hipEvent_t &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete c;
+            return fgc::fail(FG_E_DEVICE, "fg_create: hipEventCreate failed");
+        }
+    *out = c;
+    return FG_OK;
+}
+
+void fg_destroy(fg_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (hipEvent_t e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int fg_job_create(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
+{
+    if (!ctx || !d || !out) return fgc::fail(FG_E_ARG, "fg_job_create: NULL argument");
+    *out = nullptr;
+    const fg_grid &g = d->grid;
+    if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0 || g.nx > 1025 || g.ny > 1025 || g.nz > 1025 || !(g.spacing > 0))
+        return fgc::fail(FG_E_ARG, "fg_job_create: invalid grid");
+    if (d->atoms.n < 0 || (d->atoms.n > 0 && (!d->atoms.x || !d->atoms.y || !d->atoms.z)))
+        return fgc::fail(FG_E_ARG, "fg_job_create: invalid atoms");
+    if (d->n_maps < 0 || d->n_maps > 14) return fgc::fail(FG_E_ARG, "fg_job_create: n_maps must be 0..14");
+    FG_HIP(hipSetDevice(ctx->device));
+    fg_job *j = new fg_job();
+    j->ctx = ctx;
+    j->g = fgk::GridGeom{g.nx, g.ny, g.nz, g.div2x, g.div2y, g.div2z, g.spacing};
+    j->P = int64_t(g.nx) * g.ny * g.nz;
+    j->N = d->atoms.n;
+    j->M = d->n_maps;
+    j->T = d->tables.n_types;
+    j->distDepDiel = d->dist_dep_diel;
+    j->hydrogen = d->hydrogen;
+    const fg_atoms &A = d->atoms;
+    j->hasGeom = A.charge && A.vol && A.solpar && A.type && A.hbond && d->geom.rvector && d->geom.rvector2 &&
+                 d->geom.rexp && d->geom.disorder && d->tables.pair && d->tables.desolv && (j->M == 0 || d->maps);
+    if (j->M > 0 && !j->hasGeom) {
+        destroyJob(j);
+        return fgc::fail(FG_E_ARG, "fg_job_create: affinity maps need charge/vol/solpar/type/hbond, bond geometry and tables");
+    }
+    if (d->dist_dep_diel && !d->tables.eps) {
+        destroyJob(j);
+        return fgc::fail(FG_E_ARG, "fg_job_create: distance-dependent dielectric needs tables.eps");
+    }
+    if (j->hasGeom) {
+        for (int i = 0; i < j->N; ++i)
+            if (A.type[i] < 0 || A.type[i] >= std::max(j->T, 1) || A.hbond[i] < 0 || A.hbond[i] > 5) {
+                destroyJob(j);
+                return fgc::fail(FG_E_ARG, "fg_job_create: atom type/hbond out of range of the tables");
+            }
+        if (j->hydrogen >= std::max(j->T, 1)) {
+            destroyJob(j);
+            return fgc::fail(FG_E_ARG, "fg_job_create: hydrogen type index out of range");
+        }
+    }
+    j->mp.resize(size_t(j->M));
+    for (int m = 0; m < j->M; ++m) {
+        const fg_map_params &p = d->maps[m];
+        j->mp[size_t(m)] = fgk::MapParam{p.hbond, p.is_hbonder, p.is_covalent, 0, p.solpar_probe, p.vol_probe};
+    }
+    double gridMaxAbs = 0;
+    const int n[3] = {g.nx, g.ny, g.nz}, dv[3] = {g.div2x, g.div2y, g.div2z};
+    for (int k = 0; k < 3; ++k)
+        gridMaxAbs = std::max({gridMaxAbs, std::fabs(double(0 - dv[k]) * g.spacing), std::fabs(double(n[k] - 1 - dv[k]) * g.spacing)});
+    int rc = buildCutoff(j, *d);
+    if (!rc) rc = buildElecTiles(j, *d, gridMaxAbs);
+    if (!rc) rc = devAlloc(j, j->dMaps, size_t(j->M + 2) * size_t(j->P));
+    if (!rc) {
+        hipError_t e = hipMemsetAsync(j->dMaps, 0, size_t(j->M + 2) * size_t(j->P) * sizeof(double), ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = fgc::fail(FG_E_DEVICE, std::string("fg_job_create: ") + hipGetErrorString(e));
+    }
+    if (rc) {
+        destroyJob(j);
+        return rc;
+    }
+    *out = j;
+    return FG_OK;
+}
+
+void fg_job_destroy(fg_job *job)
+{
+    if (job && job->ctx) (void)hipStreamSynchronize(job->ctx->stream);
+    destroyJob(job);
+}
+
+int fg_job_run(fg_ctx *ctx, fg_job *j, unsigned what, int precision)
+{
+    if (!ctx || !j || j->ctx != ctx) return fgc::fail(FG_E_ARG, "fg_job_run: job does not belong to this context");
+    if (precision < FG_PREC_PLAIN || precision > FG_PREC_FP64) return fgc::fail(FG_E_ARG, "fg_job_run: bad precision");
+    if ((what & FG_RUN_AFFINITY) && j->M > 0 && !j->hasGeom)
+        return fgc::fail(FG_E_ARG, "fg_job_run: job has no affinity inputs");
+    FG_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const bool elec = what & FG_RUN_ELEC, aff = (what & FG_RUN_AFFINITY) != 0;
+    const bool near = elec && precision == FG_PREC_COMPENSATED;
+    FG_HIP(hipEventRecord(ctx->ev[0], s));
+    if (aff || near) {
+        fgk::ListArgs la = listArgs(j);
+        if (int rc = runListCount(j, la)) return rc;
+        hipLaunchKernelGGL(fgk::k_brick_lists<true>, dim3(j->nb), dim3(fgk::kListThreads), 0, s, la);
+        FG_HIP(hipGetLastError());
+        if (int rc = launchCutoff(j, aff && j->hasGeom, near ? 1 : 0)) return rc;
+    }
+    FG_HIP(hipEventRecord(ctx->ev[1], s));
+    if (elec)
+        if (int rc = launchElec(j, precision)) return rc;
+    FG_HIP(hipEventRecord(ctx->ev[2], s));
+    ctx->haveTiming = true;
+    return FG_OK;
+}
+
+int fg_sync(fg_ctx *ctx)
+{
+    if (!ctx) return fgc::fail(FG_E_ARG, "fg_sync: NULL context");
+    FG_HIP(hipStreamSynchronize(ctx->stream));
+    return FG_OK;
+}
+
+int fg_last_timing(fg_ctx *ctx, fg_timing *t)
+{
+    if (!ctx || !t) return fgc::fail(FG_E_ARG, "fg_last_timing: NULL argument");
+    if (!ctx->haveTiming) return fgc::fail(FG_E_ARG, "fg_last_timing: nothing has run");
+    FG_HIP(hipEventSynchronize(ctx->ev[2]));
+    float a = 0, b = 0, c = 0;
+    FG_HIP(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]));
+    FG_HIP(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
+    FG_HIP(hipEventElapsedTime(&c, ctx->ev[0], ctx->ev[2]));
+    t->affinity_ms = a;
+    t->elec_ms = b;
+    t->total_ms = c;
+    return FG_OK;
+}
+
+int fg_job_fetch(fg_ctx *ctx, fg_job *j, double *const *out)
+{
+    if (!ctx || !j || !out) return fgc::fail(FG_E_ARG, "fg_job_fetch: NULL argument");
+    FG_HIP(hipSetDevice(ctx->device));
+    for (int m = 0; m < j->M + 2; ++m)
+        if (out[m])
+            FG_HIP(hipMemcpyAsync(out[m], j->dMaps + int64_t(m) * j->P, size_t(j->P) * sizeof(double),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+    FG_HIP(hipStreamSynchronize(ctx->stream));
+    return FG_OK;
+}
+
+int fg_job_cutoff_pairs(fg_ctx *ctx, fg_job *j, int64_t *pairs)
+{
+    if (!ctx || !j || !pairs) return fgc::fail(FG_E_ARG, "fg_job_cutoff_pairs: NULL argument");
+    FG_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    fgk::ListArgs la = listArgs(j);
+    if (int rc = runListCount(j, la)) return rc;
+    hipLaunchKernelGGL(fgk::k_brick_lists<true>, dim3(j->nb), dim3(fgk::kListThreads), 0, s, la);
+    unsigned long long *d = nullptr;
+    FG_HIP(hipMalloc(&d, sizeof *d));
+    unsigned long long h = 0;
+    hipError_t e = hipMemsetAsync(d, 0, sizeof *d, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(fgk::k_count_pairs, dim3(j->nb), dim3(fgk::kCutThreads), 0, s, cutoffArgs(j, 0), d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fgc::fail(FG_E_DEVICE, std::string("fg_job_cutoff_pairs: ") + hipGetErrorString(e));
+    *pairs = int64_t(h);
+    return FG_OK;
+}
+
+int fg_batch_run(fg_ctx *ctx, fg_job *const *jobs, int n, unsigned what, int precision)
+{
+    if (!ctx || (!jobs && n > 0)) return fgc::fail(FG_E_ARG, "fg_batch_run: NULL argument");
+    for (int i = 0; i < n; ++i)
+        if (int rc = fg_job_run(ctx, jobs[i], what, precision)) return rc;
+    return FG_OK;
+}
+
+int fg_elec_map(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, const double *eps, int ddd, int precision,
+                double *e_out)
+{
+    if (!ctx || !grid || !atoms || !e_out) return fgc::fail(FG_E_ARG, "fg_elec_map: NULL argument");
+    fg_job_desc d{};
+    d.grid = *grid;
+    d.atoms.n = atoms->n;
+    d.atoms.x = atoms->x;
+    d.atoms.y = atoms->y;
+    d.atoms.z = atoms->z;
+    d.atoms.w = atoms->w;
+    d.tables.eps = eps;
+    d.dist_dep_diel = ddd;
+    d.hydrogen = -1;
+    fg_job *j = nullptr;
+    if (int rc = fg_job_create(ctx, &d, &j)) return rc;
+    std::vector<double *> outs(2, nullptr);
+    outs[0] = e_out;
+    int rc = fg_job_run(ctx, j, FG_RUN_ELEC, precision);
+    if (!rc) rc = fg_job_fetch(ctx, j, outs.data());
+    fg_job_destroy(j);
+    return rc;
+}
+
+int fg_affinity_d_maps(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, const fg_hbond_geom *geom,
+                       const fg_tables *tables, const fg_map_params *maps, int n_maps, double solpar_q, int hydrogen,
+                       double *const *maps_out, double *d_out)
+{
+    if (!ctx || !grid || !atoms || !geom || !tables || (!maps && n_maps > 0) || !maps_out || !d_out)
+        return fgc::fail(FG_E_ARG, "fg_affinity_d_maps: NULL argument");
+    fg_job_desc d{};
+    d.grid = *grid;
+    d.atoms = *atoms;
+    d.geom = *geom;
+    d.tables = *tables;
+    d.n_maps = n_maps;
+    d.maps = maps;
+    d.solpar_q = solpar_q;
+    d.hydrogen = hydrogen;
+    d.dist_dep_diel = 0;
+    // covalent maps arrive as initial values in maps_out
+    const int64_t P = int64_t(grid->nx) * grid->ny * grid->nz;
+    std::vector<double> cov;
+    bool anyCov = false;
+    for (int m = 0; m < n_maps; ++m) anyCov = anyCov || maps[m].is_covalent;
+    if (anyCov) {
+        cov.assign(size_t(n_maps) * size_t(P), 0.0);
+        for (int m = 0; m < n_maps; ++m)
+            if (maps[m].is_covalent && maps_out[m]) std::copy(maps_out[m], maps_out[m] + P, cov.begin() + int64_t(m) * P);
+        d.covalent = cov.data();
+    }
+    fg_job *j = nullptr;
+    if (int rc = fg_job_create(ctx, &d, &j)) return rc;
+    std::vector<double *> outs(size_t(n_maps) + 2, nullptr);
+    for (int m = 0; m < n_maps; ++m) outs[size_t(m)] = maps_out[m];
+    outs[size_t(n_maps) + 1] = d_out;
+    int rc = fg_job_run(ctx, j, FG_RUN_AFFINITY, FG_PREC_FP64);
+    if (!rc) rc = fg_job_fetch(ctx, j, outs.data());
+    fg_job_destroy(j);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,111 @@
+// C ABI of the host front end (include/fgmaps.h): errors, prepared inputs,
+// writers and the synthetic generator.  No device code.
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "capi_common.hpp"
+#include "host.hpp"
+
+struct fg_prep {
+    fgh::Prepared p;
+};
+
+namespace {
+thread_local std::string g_error;
+thread_local int g_exitCode = 0;
+}  // namespace
+
+namespace fgc {
+int fail(int code, const std::string &msg)
+{
+    g_error = msg;
+    return code;
+}
+void clearError() { g_error.clear(); }
+}  // namespace fgc
+
+extern "C" {
+
+const char *fg_last_error(void) { return g_error.c_str(); }
+
+double fg_round3(double a) { return fgh::round3(a); }
+
+int fg_prep_exit_code(void) { return g_exitCode; }
+
+int fg_prep_load(const char *gpf, const fg_load_opts *opts, fg_prep **out)
+{
+    if (!out) return fgc::fail(FG_E_ARG, "fg_prep_load: out is NULL");
+    *out = nullptr;
+    fg_load_opts o{0, 0};
+    if (opts) o = *opts;
+    fg_prep *prep = new (std::nothrow) fg_prep();
+    if (!prep) return fgc::fail(FG_E_NOMEM, "fg_prep_load: out of memory");
+    std::string err;
+    int rc = 0;
+    try {
+        rc = fgh::loadGpf(gpf ? gpf : "", o, prep->p, err);
+    } catch (const std::bad_alloc &) {
+        rc = int(0xBADA110C);
+        err = "Not enough memory!";
+    }
+    if (rc != 0) {
+        g_exitCode = rc;
+        delete prep;
+        return fgc::fail(rc == 911 ? FG_E_IO : FG_E_INPUT, err);
+    }
+    g_exitCode = 0;
+    *out = prep;
+    return FG_OK;
+}
+
+void fg_prep_free(fg_prep *prep) { delete prep; }
+
+int fg_prep_desc(const fg_prep *prep, fg_job_desc *out)
+{
+    if (!prep || !out) return fgc::fail(FG_E_ARG, "fg_prep_desc: NULL argument");
+    *out = prep->p.desc();
+    return FG_OK;
+}
+
+int fg_prep_num_maps(const fg_prep *prep) { return prep ? prep->p.numAtomMaps : 0; }
+
+const char *fg_prep_map_type(const fg_prep *prep, int m)
+{
+    if (!prep || m < 0 || m >= int(prep->p.maps.size())) return "";
+    return prep->p.maps[size_t(m)].type.c_str();
+}
+
+int fg_prep_num_atoms(const fg_prep *prep) { return prep ? prep->p.numAtoms() : 0; }
+
+int64_t fg_prep_num_points(const fg_prep *prep) { return prep ? prep->p.P : 0; }
+
+const char *fg_prep_gpf_echo(const fg_prep *prep) { return prep ? prep->p.log.c_str() : ""; }
+
+int fg_prep_summary(const fg_prep *prep, int m, double *mn, double *mx)
+{
+    if (!prep || !mn || !mx) return fgc::fail(FG_E_ARG, "fg_prep_summary: NULL argument");
+    if (m < 0 || size_t(m) >= prep->p.mapMin.size()) return fgc::fail(FG_E_ARG, "fg_prep_summary: no such map");
+    *mn = prep->p.mapMin[size_t(m)];
+    *mx = prep->p.mapMax[size_t(m)];
+    return FG_OK;
+}
+
+int fg_write_outputs(fg_prep *prep, double *const *maps, const char *program_name, int threads)
+{
+    if (!prep || !maps) return fgc::fail(FG_E_ARG, "fg_write_outputs: NULL argument");
+    std::string err;
+    int rc = fgh::writeOutputs(prep->p, maps, program_name ? program_name : "fastgrid4", threads, err);
+    return rc ? fgc::fail(FG_E_IO, err) : FG_OK;
+}
+
+int fg_synth_write(const fg_synth_spec *spec, const char *dir)
+{
+    if (!spec) return fgc::fail(FG_E_ARG, "fg_synth_write: NULL spec");
+    std::string err;
+    int rc = fgh::synthWrite(*spec, dir ? dir : ".", err);
+    if (rc == -3) return fgc::fail(FG_E_IO, err);
+    return rc ? fgc::fail(FG_E_ARG, err) : FG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,559 @@
+// CDNA4 (gfx950) map kernels.  See kernels.hpp for the decomposition.
+// FG = fastgrid_master/fastgrid in the reference.
+//
+// FP64 code in this file keeps the reference's operation order: the file is
+// compiled without floating-point contraction, and FP32 fused multiply-adds
+// are written explicitly with __builtin_fmaf.
+#pragma clang fp contract(off)
+
+#include "kernels.hpp"
+
+namespace fgk {
+
+// ------------------------------------------------------------------------
+// helpers
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double dmin_(double a, double b) { return a < b ? a : b; }
+__device__ __forceinline__ double dmax_(double a, double b) { return a > b ? a : b; }
+// Math::Acos with the reference's clamp (FG/math/Math.h:51)
+__device__ __forceinline__ double acosClamped(double f)
+{
+    return f > 1 ? 0.0 : (f < -1 ? 3.14159265358979323846 : acos(f));
+}
+// angstromToIndex (FG/autogrid.h:120-125)
+__device__ __forceinline__ int binOf(double r)
+{
+    int i = int(r * 100.0);
+    return i < kTableLen - 1 ? i : kTableLen - 1;
+}
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// ------------------------------------------------------------------------
+// Candidate lists: for each brick of the cutoff kernel, every atom whose
+// distance to the brick's box is <= 8 A (+1e-4 A).  Order: coarse cells in
+// z, y, x order, ascending atom index within a cell; the same on every run.
+// ------------------------------------------------------------------------
+constexpr int kListThreads = 256;
+constexpr int kMaxCellsPerBrick = 1024;
+
+struct ListArgs {
+    GridGeom g;
+    CellGrid cg;
+    int32_t nbx, nby, nbz, N;
+    const double *xyz;        // [N][3]
+    const int32_t *hbond;     // [N]
+    const int32_t *cellStart; // [ncells+1]
+    const int32_t *cellAtoms; // atoms sorted by cell, ascending within a cell
+    int32_t *counts;          // [nb][2] all, H (count pass)
+    const int32_t *start;     // [nb+1] (fill pass)
+    const int32_t *hStart;    // [nb+1] (fill pass)
+    int32_t *list, *hList;
+};
+
+template <bool kFill>
+__global__ __launch_bounds__(kListThreads) void k_brick_lists(ListArgs a)
+{
+    __shared__ int32_t cellOff[kMaxCellsPerBrick + 1];
+    __shared__ int32_t cellId[kMaxCellsPerBrick];
+    __shared__ int32_t scanBuf[kListThreads], scanH[kListThreads];
+    __shared__ int32_t base, baseH;
+    const int tid = threadIdx.x;
+    const int b = blockIdx.x;
+    const int bx = b % a.nbx, by = (b / a.nbx) % a.nby, bz = b / (a.nbx * a.nby);
+    const int ib[3] = {bx * kCutBX, by * kCutBY, bz * kCutBZ};
+    const int ext[3] = {kCutBX, kCutBY, kCutBZ};
+    const int n[3] = {a.g.nx, a.g.ny, a.g.nz}, d2[3] = {a.g.dx2, a.g.dy2, a.g.dz2};
+    double lo[3], hi[3];
+    int c0[3], c1[3];
+    const double reach = 8.0 + 1e-4;
+    for (int k = 0; k < 3; ++k) {
+        int i1 = ib[k] + ext[k] - 1;
+        if (i1 > n[k] - 1) i1 = n[k] - 1;
+        lo[k] = (ib[k] - d2[k]) * a.g.spacing;
+        hi[k] = (i1 - d2[k]) * a.g.spacing;
+        c0[k] = int(floor((lo[k] - reach - a.cg.origin[k]) / a.cg.size));
+        c1[k] = int(floor((hi[k] + reach - a.cg.origin[k]) / a.cg.size));
+        if (c0[k] < 0) c0[k] = 0;
+        if (c1[k] > a.cg.nc[k] - 1) c1[k] = a.cg.nc[k] - 1;
+    }
+    const int ncx = c1[0] - c0[0] + 1, ncy = c1[1] - c0[1] + 1, ncz = c1[2] - c0[2] + 1;
+    const int ncell = (ncx > 0 && ncy > 0 && ncz > 0) ? ncx * ncy * ncz : 0;
+    // cell table and prefix of atom counts (one thread; at most kMaxCellsPerBrick cells)
+    if (tid == 0) {
+        int off = 0, m = 0;
+        for (int cz = 0; cz < ncz; ++cz)
+            for (int cy = 0; cy < ncy; ++cy)
+                for (int cx = 0; cx < ncx && m < kMaxCellsPerBrick; ++cx) {
+                    int c = ((c0[2] + cz) * a.cg.nc[1] + (c0[1] + cy)) * a.cg.nc[0] + (c0[0] + cx);
+                    cellId[m] = c;
+                    cellOff[m] = off;
+                    off += a.cellStart[c + 1] - a.cellStart[c];
+                    ++m;
+                }
+        cellOff[m] = off;
+        base = kFill ? a.start[b] : 0;
+        baseH = kFill ? a.hStart[b] : 0;
+    }
+    __syncthreads();
+    const int m = ncell < kMaxCellsPerBrick ? ncell : kMaxCellsPerBrick;
+    const int total = cellOff[m];
+    const double reach2 = reach * reach;
+    int cnt = 0, cntH = 0;
+    for (int v0 = 0; v0 < total; v0 += kListThreads) {
+        const int v = v0 + tid;
+        int keep = 0, keepH = 0, ia = -1;
+        if (v < total) {
+            int lo_ = 0, hi_ = m - 1;  // last cell with cellOff <= v
+            while (lo_ < hi_) {
+                int mid = (lo_ + hi_ + 1) >> 1;
+                if (cellOff[mid] <= v) lo_ = mid;
+                else hi_ = mid - 1;
+            }
+            const int c = cellId[lo_];
+            ia = a.cellAtoms[a.cellStart[c] + (v - cellOff[lo_])];
+            double dd = 0;
+            for (int k = 0; k < 3; ++k) {
+                double p = a.xyz[3 * ia + k], t = 0;
+                if (p < lo[k]) t = lo[k] - p;
+                else if (p > hi[k]) t = p - hi[k];
+                dd += t * t;
+            }
+            keep = dd <= reach2;
+            keepH = keep && (a.hbond[ia] == HB_DS || a.hbond[ia] == HB_D1);
+        }
+        if (!kFill) {
+            cnt += keep;
+            cntH += keepH;
+            continue;
+        }
+        // block-wide exclusive scan of keep / keepH (Hillis-Steele in LDS)
+        scanBuf[tid] = keep;
+        scanH[tid] = keepH;
+        __syncthreads();
+        for (int s = 1; s < kListThreads; s <<= 1) {
+            int x = tid >= s ? scanBuf[tid - s] : 0, y = tid >= s ? scanH[tid - s] : 0;
+            __syncthreads();
+            scanBuf[tid] += x;
+            scanH[tid] += y;
+            __syncthreads();
+        }
+        const int posA = base + scanBuf[tid] - keep, posH = baseH + scanH[tid] - keepH;
+        if (keep) a.list[posA] = ia;
+        if (keepH) a.hList[posH] = ia;
+        __syncthreads();
+        if (tid == kListThreads - 1) {
+            base += scanBuf[tid];
+            baseH += scanH[tid];
+        }
+        __syncthreads();
+    }
+    if (!kFill) {
+        // block reduction of the two counts
+        scanBuf[tid] = cnt;
+        scanH[tid] = cntH;
+        __syncthreads();
+        for (int s = kListThreads / 2; s > 0; s >>= 1) {
+            if (tid < s) {
+                scanBuf[tid] += scanBuf[tid + s];
+                scanH[tid] += scanH[tid + s];
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            a.counts[2 * b] = scanBuf[0];
+            a.counts[2 * b + 1] = scanH[0];
+        }
+    }
+}
+
+// Exclusive scan of counts[i*2 + which] into out[0..n] (out[n] = total), one
+// workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_scan_counts(const int32_t *counts, int n, int which, int32_t *out)
+{
+    __shared__ int32_t part[1024];
+    const int tid = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b0 = tid * per, b1 = min(n, b0 + per);
+    int s = 0;
+    for (int i = b0; i < b1; ++i) s += counts[2 * i + which];
+    part[tid] = s;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        int x = tid >= d ? part[tid - d] : 0;
+        __syncthreads();
+        part[tid] += x;
+        __syncthreads();
+    }
+    int run = part[tid] - s;
+    for (int i = b0; i < b1; ++i) {
+        out[i] = run;
+        run += counts[2 * i + which];
+    }
+    if (tid == 1023) out[n] = part[1023];
+}
+
+// ------------------------------------------------------------------------
+// Cutoff kernel: affinity maps + desolvation map + near-field electrostatics
+// (FG/CalculateGridmaps.cpp:238-316, FG/electrostatics/Cpu.cpp:67-79 for the
+// pairs with r^2 <= 64).
+// ------------------------------------------------------------------------
+
+// racc / rdon / Hramp (FG/CalculateGridmaps.cpp:84-197).  hb, rexp and
+// disorder are wave-uniform.
+__device__ __forceinline__ void hbondAngular(int hb, int rexp, int disorder, bool isClosest,
+                                             const double rv[3], const double rv2[3], const double cHrv[3],
+                                             double ux, double uy, double uz,
+                                             double &racc, double &rdon, double &Hramp)
+{
+    racc = 1;
+    rdon = 1;
+    Hramp = 1;
+    const double cosT = -(ux * rv[0] + uy * rv[1] + uz * rv[2]);
+    if (hb == HB_D1) {
+        racc = 0;
+        if (cosT > 0) {
+            double r = cosT;
+            if (rexp == 4) r = r * r;
+            if (rexp == 4 || rexp == 2) r = r * r;
+            racc = r;
+            if (!isClosest) {
+                double th = acosClamped(cHrv[0] * rv[0] + cHrv[1] * rv[1] + cHrv[2] * rv[2]);
+                Hramp = 0.5 - 0.5 * cos(th * (120.0 / 90.0));
+            }
+        }
+    } else if (hb == HB_A1) {
+        rdon = 0;
+        if (cosT > 0) rdon = cosT * cosT;
+    } else if (hb == HB_A2) {
+        rdon = 0;
+        if (disorder) {
+            racc = 0;
+            double th = acosClamped(cosT);
+            if (th <= 1.24791 + (3.14159265358979323846 / 2)) {
+                double c = cos(th - 1.24791);
+                c = c * c;
+                rdon = c * c;
+                racc = rdon;
+            }
+        } else {
+            double r = 0;
+            if (cosT >= 0) {
+                double cx = uy * rv2[2] - uz * rv2[1];
+                double cy = uz * rv2[0] - ux * rv2[2];
+                double cz = ux * rv2[1] - uy * rv2[0];
+                double rd2 = cx * cx + cy * cy + cz * cz;
+                if (rd2 < 1.0E-6) rd2 = 1.0E-6;
+                double inv = 1 / sqrt(rd2);
+                double ti = fabs(acosClamped(inv * (cx * rv[0] + cy * rv[1] + cz * rv[2])) -
+                                 (3.14159265358979323846 / 2));
+                r = 0.9 + 0.1 * sin(ti + ti);
+            } else if (cosT >= -0.34202) {
+                double t = 0.116978 - cosT * cosT;
+                r = 562.25 * (t * t * t);
+            }
+            double t0 = (3.14159265358979323846 / 2) - acosClamped(ux * rv2[0] + uy * rv2[1] + uz * rv2[2]);
+            r *= cos(t0);
+            rdon = r;
+        }
+    }
+}
+
+template <int MM, bool kAffinity>
+__global__ __launch_bounds__(kCutThreads) void k_cutoff(CutoffArgs a)
+{
+    // XCD-aware brick order: consecutive workgroups go to different XCDs, so
+    // hand each XCD a contiguous run of bricks (neighbours share atoms in L2).
+    // (bijection on the first 8*floor(nb/8) workgroups, identity on the tail)
+    const int nb = a.nbx * a.nby * a.nbz;
+    const int full = (nb / 8) * 8, g = int(blockIdx.x);
+    const int b = g < full ? (g % 8) * (full / 8) + g / 8 : g;
+    const int bx = b % a.nbx, by = (b / a.nbx) % a.nby, bz = b / (a.nbx * a.nby);
+    // wave = 4x4x4 cube of points, workgroup = 2x2x1 waves
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const int ix = bx * kCutBX + (w & 1) * 4 + (l & 3);
+    const int iy = by * kCutBY + (w >> 1) * 4 + ((l >> 2) & 3);
+    const int iz = bz * kCutBZ + (l >> 4);
+    const bool valid = ix < a.g.nx && iy < a.g.ny && iz < a.g.nz;
+    const double gx = (ix - a.g.dx2) * a.g.spacing;
+    const double gy = (iy - a.g.dy2) * a.g.spacing;
+    const double gz = (iz - a.g.dz2) * a.g.spacing;
+    const int64_t p = valid ? (int64_t(iz) * a.g.ny + iy) * a.g.nx + ix : 0;
+    const int M = a.M;
+
+    // pass 1: nearest DS/D1 atom, lowest index on ties (FG/CalculateGridmaps.cpp:65-82)
+    int cH = 0;
+    double cHrv[3] = {0, 0, 0};
+    if (kAffinity) {
+        double rmin = 999999.0 * 999999.0;
+        const int h0 = a.hStart[b], h1 = a.hStart[b + 1];
+        for (int c = h0; c < h1; ++c) {
+            const int ia = uniform(a.hList[c]);
+            const CutAtom &A = a.atoms[ia];
+            double dx = A.x - gx, dy = A.y - gy, dz = A.z - gz;
+            double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < rmin || (r2 == rmin && ia < cH)) {
+                rmin = r2;
+                cH = ia;
+            }
+        }
+        cHrv[0] = a.atoms[cH].rv[0];
+        cHrv[1] = a.atoms[cH].rv[1];
+        cHrv[2] = a.atoms[cH].rv[2];
+    }
+
+    double e[MM], hmin[MM], hmax[MM];
+    unsigned flags = 0;
+#pragma unroll
+    for (int m = 0; m < MM; ++m) {
+        e[m] = (kAffinity && m < M && a.covalent && a.mp[m].isCovalent) ? a.covalent[int64_t(m) * a.P + p] : 0.0;
+        hmin[m] = 999999;
+        hmax[m] = -999999;
+    }
+    double dmap = 0, enear = 0;
+
+    const int c0 = a.brickStart[b], c1 = a.brickStart[b + 1];
+    for (int c = c0; c < c1; ++c) {
+        const int ia = uniform(a.brickList[c]);
+        const CutAtom &A = a.atoms[ia];
+        double dx = A.x - gx, dy = A.y - gy, dz = A.z - gz;
+        double rSq = dx * dx + dy * dy + dz * dz;
+        if (!valid || rSq > 64.0) continue;
+        if (a.doElecNear) {
+            // the electrostatic term exactly as FG/electrostatics/Cpu.cpp:44-75 forms it
+            double dydz = dy * dy + dz * dz;
+            double r = sqrt(dx * dx + dydz);
+            double inv = 1 / r;
+            enear += A.w * dmin_(inv, 2) * a.eps[binOf(r)];
+        }
+        if (!kAffinity) continue;
+        const int type = A.type, ahb = A.hbond, dis = A.disorder;
+        if (type == a.hydrogen && dis) continue;
+        if (rSq < 1.0E-24) rSq = 1.0E-24;
+        const double invR = 1 / sqrt(rSq);
+        const double ux = dx * invR, uy = dy * invR, uz = dz * invR;
+        const int idx = binOf(1 / invR);
+        double racc, rdon, Hramp;
+        if (ahb == HB_D1 || ahb == HB_A1 || ahb == HB_A2) {
+            const double rv[3] = {A.rv[0], A.rv[1], A.rv[2]};
+            const double rv2[3] = {A.rv2[0], A.rv2[1], A.rv2[2]};
+            hbondAngular(ahb, A.rexp, dis, ia == cH, rv, rv2, cHrv, ux, uy, uz, racc, rdon, Hramp);
+        } else {
+            racc = 1;
+            rdon = 1;
+            Hramp = 1;
+        }
+        const bool aDon = ahb == HB_DS || ahb == HB_D1;
+        const double S = a.desolv[idx];
+        const double *row = a.pair + (int64_t(type) * kTableLen + idx) * M;
+        const double *rowH = (dis && a.hydrogen >= 0)
+                                 ? a.pair + (int64_t(a.hydrogen) * kTableLen + (idx - 110 > 0 ? idx - 110 : 0)) * M
+                                 : row;
+        const double vol = A.vol, solQ = A.solQ;
+#pragma unroll
+        for (int m = 0; m < MM; ++m) {
+            if (m >= M || a.mp[m].isCovalent) continue;
+            const double E = row[m];
+            const int mhb = a.mp[m].hbond;
+            if (a.mp[m].isHBonder) {
+                if ((mhb == HB_AS || mhb == HB_A2) && aDon) {
+                    double rsph = dmin_(dmax_(E / 100, 0), 1);
+                    e[m] += rowH[m] * Hramp * (racc + (1 - racc) * rsph);
+                } else if (mhb == HB_A1 && aDon) {
+                    double rsph = dmin_(dmax_(E / 100, 0), 1);
+                    double v = E * (racc + (1 - racc) * rsph);
+                    hmin[m] = dmin_(hmin[m], v);
+                    hmax[m] = dmax_(hmax[m], v);
+                    flags |= 1u << m;
+                } else if ((mhb == HB_DS || mhb == HB_D1) && ahb >= HB_AS) {
+                    double rsph = dmin_(dmax_(E / 100, 0), 1);
+                    double v = E * (rdon + (1 - rdon) * rsph);
+                    hmin[m] = dmin_(hmin[m], v);
+                    hmax[m] = dmax_(hmax[m], v);
+                    flags |= 1u << m;
+                } else {
+                    e[m] += E;
+                }
+            } else {
+                e[m] += E;
+            }
+            e[m] += a.mp[m].solparProbe * vol * S + solQ * a.mp[m].volProbe * S;
+        }
+        dmap += A.qvol * S;
+    }
+    if (!valid) return;
+    if (kAffinity) {
+#pragma unroll
+        for (int m = 0; m < MM; ++m) {
+            if (m >= M) continue;
+            double v = e[m];
+            if (flags & (1u << m)) v += hmin[m] + hmax[m];
+            a.maps[int64_t(m) * a.P + p] = v;
+        }
+        a.maps[int64_t(M + 1) * a.P + p] = dmap;
+    }
+    if (a.doElecNear) a.maps[int64_t(M) * a.P + p] = enear;
+}
+
+// ------------------------------------------------------------------------
+// Far-field electrostatics in FP32 (compensated) or all-pairs FP32 (plain).
+// Coordinates are in units of 0.01 A, so 100*r = r2 * rsq(r2) indexes the
+// dielectric table directly and w is stored as 100*w.
+// ------------------------------------------------------------------------
+template <int PZ, bool kComp, bool kClamp>
+__global__ __launch_bounds__(kElecThreads) void k_elec(ElecArgs a)
+{
+    extern __shared__ float eps[];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < a.tableLen; i += kElecThreads) eps[i] = a.eps32[i];
+    __syncthreads();
+
+    const int lx = tid & 15, ly = tid >> 4;
+    const int ix = blockIdx.x * kElecBX + lx, iy = blockIdx.y * kElecBY + ly, iz0 = blockIdx.z * PZ;
+    const float s100 = a.spacing100;
+    const float gx = float(ix - a.g.dx2) * s100, gy = float(iy - a.g.dy2) * s100;
+    float gz[PZ];
+#pragma unroll
+    for (int k = 0; k < PZ; ++k) gz[k] = float(iz0 + k - a.g.dz2) * s100;
+    // brick box (scaled), clamped to the grid
+    const int bx1 = min(int(blockIdx.x) * kElecBX + kElecBX - 1, a.g.nx - 1);
+    const int by1 = min(int(blockIdx.y) * kElecBY + kElecBY - 1, a.g.ny - 1);
+    const int bz1 = min(iz0 + PZ - 1, a.g.nz - 1);
+    const float blo[3] = {float(int(blockIdx.x) * kElecBX - a.g.dx2) * s100,
+                          float(int(blockIdx.y) * kElecBY - a.g.dy2) * s100, float(iz0 - a.g.dz2) * s100};
+    const float bhi[3] = {float(bx1 - a.g.dx2) * s100, float(by1 - a.g.dy2) * s100, float(bz1 - a.g.dz2) * s100};
+
+    float hi[PZ], lo[PZ], part[PZ];
+#pragma unroll
+    for (int k = 0; k < PZ; ++k) hi[k] = lo[k] = part[k] = 0.f;
+
+    for (int t = 0; t < a.nTiles; ++t) {
+        bool far = true;
+        if (kComp) {
+            const float *tb = a.tileBox + 6 * t;
+            float dd = 0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                float d = fmaxf(fmaxf(tb[k] - bhi[k], blo[k] - tb[3 + k]), 0.f);
+                dd = __builtin_fmaf(d, d, dd);
+            }
+            far = uniform(dd > a.farCut2);
+        }
+        const float4 *T = a.tiles + int64_t(t) * kTile;
+        if (far) {
+#pragma unroll 4
+            for (int j = 0; j < kTile; ++j) {
+                const float4 at = T[j];
+                const float dx = at.x - gx, dy = at.y - gy;
+                const float dxy2 = __builtin_fmaf(dx, dx, dy * dy);
+#pragma unroll
+                for (int k = 0; k < PZ; ++k) {
+                    const float dz = at.z - gz[k];
+                    const float r2 = __builtin_fmaf(dz, dz, dxy2);
+                    float ri = __builtin_amdgcn_rsqf(r2);
+                    if (!kComp) ri = fminf(ri, 0.02f);  // min(1/r, 2) in scaled units
+                    unsigned bin = unsigned(r2 * ri);
+                    if (kClamp) bin = min(bin, unsigned(kTableLen - 1));
+                    part[k] = __builtin_fmaf(at.w * ri, eps[bin], part[k]);
+                }
+            }
+        } else {
+            const double *T64 = a.tiles64 + int64_t(t) * kTile * 3;
+            for (int j = 0; j < kTile; ++j) {
+                const float4 at = T[j];
+                const float dx = at.x - gx, dy = at.y - gy;
+                const float dxy2 = __builtin_fmaf(dx, dx, dy * dy);
+#pragma unroll
+                for (int k = 0; k < PZ; ++k) {
+                    const float dz = at.z - gz[k];
+                    const float r2 = __builtin_fmaf(dz, dz, dxy2);
+                    bool include = r2 > a.r2Hi;
+                    const bool amb = !include && r2 >= a.r2Lo;
+                    if (amb) {
+                        // exact FP64 predicate of the cutoff kernel: r^2 <= 64 is near
+                        const double ddx = T64[3 * j] - (ix - a.g.dx2) * a.g.spacing;
+                        const double ddy = T64[3 * j + 1] - (iy - a.g.dy2) * a.g.spacing;
+                        const double ddz = T64[3 * j + 2] - (iz0 + k - a.g.dz2) * a.g.spacing;
+                        include = !(ddx * ddx + ddy * ddy + ddz * ddz <= 64.0);
+                    }
+                    const float ri = __builtin_amdgcn_rsqf(r2);
+                    unsigned bin = unsigned(r2 * ri);
+                    bin = min(bin, unsigned(a.tableLen - 1));
+                    const float v = __builtin_fmaf(at.w * ri, eps[bin], part[k]);
+                    part[k] = include ? v : part[k];
+                }
+            }
+        }
+        if (kComp) {
+            // fold the tile's FP32 partial into the two-float accumulator (TwoSum)
+#pragma unroll
+            for (int k = 0; k < PZ; ++k) {
+                const float s = hi[k] + part[k];
+                const float bp = s - hi[k];
+                const float err = (hi[k] - (s - bp)) + (part[k] - bp);
+                hi[k] = s;
+                lo[k] += err;
+                part[k] = 0.f;
+            }
+        }
+    }
+    if (ix >= a.g.nx || iy >= a.g.ny) return;
+#pragma unroll
+    for (int k = 0; k < PZ; ++k) {
+        const int iz = iz0 + k;
+        if (iz >= a.g.nz) break;
+        const int64_t p = (int64_t(iz) * a.g.ny + iy) * a.g.nx + ix;
+        if (kComp) a.e[p] = a.e[p] + (double(hi[k]) + double(lo[k]));
+        else a.e[p] = double(part[k]);
+    }
+}
+
+// ------------------------------------------------------------------------
+// Electrostatic map in the reference's FP64 arithmetic, all atoms in index
+// order (FG/electrostatics/Cpu.cpp:27-88).  One point per lane.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_elec_fp64(GridGeom g, int N, const CutAtom *atoms, const double *epsG,
+                                                   int ddd, double *e)
+{
+    __shared__ double eps[kTableLen];
+    for (int i = threadIdx.x; i < kTableLen; i += 256) eps[i] = epsG[i];
+    __syncthreads();
+    const int ix = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int iy = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int iz = blockIdx.z;
+    const double gx = (ix - g.dx2) * g.spacing, gy = (iy - g.dy2) * g.spacing, gz = (iz - g.dz2) * g.spacing;
+    double en = 0;
+    for (int ia = 0; ia < N; ++ia) {
+        const CutAtom &A = atoms[ia];
+        const double tz = A.z - gz, ty = A.y - gy, tx = A.x - gx;
+        const double dzSq = tz * tz;
+        const double dydz = ty * ty + dzSq;
+        const double r = sqrt(tx * tx + dydz);
+        const double inv = 1 / r;
+        if (ddd) en += A.w * dmin_(inv, 2) * eps[binOf(r)];
+        else en += A.w * dmin_(inv, 2);
+    }
+    if (ix < g.nx && iy < g.ny) e[(int64_t(iz) * g.ny + iy) * g.nx + ix] = en;
+}
+
+// Number of (point, atom) pairs with r^2 <= 64, per brick candidate list.
+__global__ __launch_bounds__(kCutThreads) void k_count_pairs(CutoffArgs a, unsigned long long *out)
+{
+    const int b = blockIdx.x;
+    const int bx = b % a.nbx, by = (b / a.nbx) % a.nby, bz = b / (a.nbx * a.nby);
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const int ix = bx * kCutBX + (w & 1) * 4 + (l & 3);
+    const int iy = by * kCutBY + (w >> 1) * 4 + ((l >> 2) & 3);
+    const int iz = bz * kCutBZ + (l >> 4);
+    const bool valid = ix < a.g.nx && iy < a.g.ny && iz < a.g.nz;
+    const double gx = (ix - a.g.dx2) * a.g.spacing, gy = (iy - a.g.dy2) * a.g.spacing,
+                 gz = (iz - a.g.dz2) * a.g.spacing;
+    unsigned long long n = 0;
+    for (int c = a.brickStart[b]; c < a.brickStart[b + 1]; ++c) {
+        const CutAtom &A = a.atoms[uniform(a.brickList[c])];
+        double dx = A.x - gx, dy = A.y - gy, dz = A.z - gz;
+        n += (valid && dx * dx + dy * dy + dz * dz <= 64.0) ? 1 : 0;
+    }
+    atomicAdd(out, n);
+}
+
+}  // namespace fgk

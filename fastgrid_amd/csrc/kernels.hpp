@@ -1,0 +1,103 @@
+// CDNA4 (gfx950) kernels of libfgmaps.  Included by device.hip only.
+//
+// Three map kernels:
+//  * k_cutoff: affinity maps, desolvation map and the near-field part of the
+//    electrostatic map.  One workgroup per brick of 8x8x4 grid points, one
+//    point per lane.  Candidate atoms (those within 8 A of the brick) are
+//    streamed in a uniform order, so every receptor-atom branch (H-bond
+//    class, disorder, per-map probe class) is wave-uniform and atom records
+//    are fetched with scalar loads.  All arithmetic is FP64 in the
+//    reference's operation order (no contraction), so results match the
+//    reference CPU path (FG/CalculateGridmaps.cpp:84-316).
+//  * k_elec: far-field electrostatics in FP32.  One workgroup per brick of
+//    16x16xPZ points; each lane owns PZ points along z.  Receptor atoms are
+//    spatially sorted into tiles of kTile atoms; each tile's contribution is
+//    summed in FP32 and folded into a compensated two-float accumulator, and
+//    the final per-point value is formed in FP64.  The inverse dielectric
+//    table lives in LDS.  Pairs whose exact FP64 distance is <= 8 A belong
+//    to k_cutoff and are masked here; the mask is decided in FP32 with an
+//    FP64 replay in a narrow ambiguity band, so the two kernels partition the
+//    (point, atom) pairs exactly.
+//  * k_elec_fp64: the whole electrostatic sum in the reference's FP64
+//    arithmetic (FG/electrostatics/Cpu.cpp:27-88), used as the on-device
+//    FP64 oracle for large workloads.
+// plus the candidate-list builders k_brick_count / k_brick_fill and a scan.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fgk {
+
+constexpr int kTableLen = 8192;
+constexpr int kTile = 64;          // atoms per electrostatic tile
+constexpr int kCutBX = 8, kCutBY = 8, kCutBZ = 4;  // k_cutoff brick (points)
+constexpr int kCutThreads = kCutBX * kCutBY * kCutBZ;
+constexpr int kElecBX = 16, kElecBY = 16;          // k_elec brick in x, y
+constexpr int kElecThreads = kElecBX * kElecBY;
+
+enum { HB_NON = 0, HB_DS = 1, HB_D1 = 2, HB_AS = 3, HB_A1 = 4, HB_A2 = 5 };
+
+// One receptor atom as the cutoff kernel reads it (128 bytes, one scalar
+// cache line pair).  Derived terms are precomputed on the host in FP64 with
+// the reference's operation order.
+struct alignas(16) CutAtom {
+    double x, y, z, w;   // centred coordinates; w = q * coeff_estat [* 332/diel]
+    double vol;          // atomic fragmental volume
+    double solQ;         // solpar + solparQ * |q|  (FG/CalculateGridmaps.cpp:235)
+    double qvol;         // solparQ * vol            (FG/CalculateGridmaps.cpp:302)
+    double rv[3];        // bond vector
+    double rv2[3];       // second bond vector
+    int32_t type, hbond, rexp, disorder;
+};
+static_assert(sizeof(CutAtom) == 128, "CutAtom must stay 128 bytes");
+
+// Per affinity map, uniform across the kernel.
+struct MapParam {
+    int32_t hbond, isHBonder, isCovalent, pad;
+    double solparProbe, volProbe;
+};
+
+struct GridGeom {
+    int32_t nx, ny, nz, dx2, dy2, dz2;
+    double spacing;
+};
+
+// Coarse cell grid over the atoms, for building per-brick candidate lists.
+struct CellGrid {
+    int32_t nc[3];
+    double origin[3];  // lower corner of cell (0,0,0)
+    double size;       // cell edge (A)
+};
+
+struct CutoffArgs {
+    GridGeom g;
+    int32_t nbx, nby, nbz;  // bricks per axis
+    int32_t M, hydrogen, doElecNear, pad;
+    int64_t P;
+    const CutAtom *atoms;
+    const int32_t *brickStart, *brickList;    // all candidates
+    const int32_t *hStart, *hList;            // DS/D1 candidates
+    const double *pair;    // [T][8192][M]
+    const double *desolv;  // [8192]
+    const double *eps;     // [8192]
+    double *maps;          // [M+2][P]: affinity maps, e (near part), d
+    const double *covalent;  // [M][P] or null
+    MapParam mp[16];
+};
+
+struct ElecArgs {
+    GridGeom g;
+    int32_t nTiles, tableLen, mode, pad;  // mode: 0 plain, 1 compensated
+    int64_t P;
+    const float4 *tiles;     // [nTiles*kTile] (x, y, z, w) scaled by 100 (w: 100*w)
+    const double *tiles64;   // [nTiles*kTile][3] FP64 coordinates (for the exact mask)
+    const float *tileBox;    // [nTiles][6] lo xyz, hi xyz (scaled)
+    const float *eps32;      // [tableLen]
+    double *e;               // [P] electrostatic map (near part already there in mode 1)
+    float farCut2;           // (800 + margin)^2 for tile culling (scaled units)
+    float r2Lo, r2Hi;        // ambiguity band around 64e4 in scaled r^2
+    float spacing100;        // 100 * spacing
+};
+
+}  // namespace fgk

@@ -1,0 +1,240 @@
+/*
+ * fgmaps.h — C ABI of libfgmaps, the MI355X grid-map library.
+ *
+ * The reference (FastGrid 4.2.1) has no library API: its map backends are
+ * C++ functions inside one executable.  This header is the boundary a
+ * maintainer binds in their place.  Each entry point names the reference
+ * interface it replaces (paths relative to the reference root;
+ * FG = fastgrid_master/fastgrid).
+ *
+ * Conventions
+ *  - Plain C types only; every call returns FG_OK (0) or a negative FG_E*
+ *    code, and fg_last_error() returns the message of the calling thread's
+ *    last failure.  No exception crosses the ABI.
+ *  - Host buffers are caller-owned.  Device buffers, streams and graphs are
+ *    owned by an fg_ctx (one context per host thread).
+ *  - Map outputs are FP64 and NOT rounded: the caller applies the 3-decimal
+ *    rounding (fg_round3, FG/autogrid.h:128-131) so CPU and GPU share one
+ *    rounding step.  fg_write_outputs() rounds while writing.
+ *  - Grid indexing is x fastest: p = (z*ny + y)*nx + x (FG/autogrid.h:140-161).
+ *  - There is no CPU fallback: every compute entry point fails with
+ *    FG_E_DEVICE when no GPU is usable.
+ */
+#ifndef FGMAPS_H
+#define FGMAPS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FG_OK 0
+#define FG_E_ARG -1       /* invalid argument or size */
+#define FG_E_DEVICE -2    /* HIP/device failure */
+#define FG_E_IO -3        /* file could not be read/written */
+#define FG_E_INPUT -4     /* GPF/PDBQT semantic error (see fg_prep_exit_code) */
+#define FG_E_NOMEM -5
+
+#define FG_TABLE_LEN 8192 /* MAX_DIST, FG/autogrid.h:88 */
+#define FG_MAX_MAPS 16    /* MAX_MAPS, fastgrid_master/autodock/autocomm.h:103 */
+
+/* H-bond classes, FG/ParameterLibrary.h:30-38 */
+enum { FG_HB_NON = 0, FG_HB_DS = 1, FG_HB_D1 = 2, FG_HB_AS = 3, FG_HB_A1 = 4, FG_HB_A2 = 5 };
+
+/* Precision forms of the electrostatic map (the affinity/desolvation maps are
+ * always computed in the reference's FP64 arithmetic). */
+enum {
+    FG_PREC_PLAIN = 0,       /* FP32 pair math, FP32 accumulation, FP32 bin decision */
+    FG_PREC_COMPENSATED = 1, /* FP32 pair math, compensated two-float cross-tile sum,
+                                FP64 final sum, exact FP64 bin decision in a guard band */
+    FG_PREC_FP64 = 2         /* FP64 everywhere, the reference's operation order */
+};
+
+/* What a job run computes. */
+#define FG_RUN_ELEC 1u
+#define FG_RUN_AFFINITY 2u /* affinity maps + desolvation map */
+#define FG_RUN_ALL 3u
+
+typedef struct fg_ctx fg_ctx;
+typedef struct fg_job fg_job;
+typedef struct fg_prep fg_prep;
+
+/* Grid: numGridPoints, numGridPointsDiv2, gridSpacing (FG/InputData.h:38-43).
+ * Point (x,y,z) sits at ((x-div2x)*spacing, ...) relative to the grid centre. */
+typedef struct {
+    int32_t nx, ny, nz;
+    int32_t div2x, div2y, div2z;
+    double spacing;
+} fg_grid;
+
+/* Receptor atoms, structure of arrays (FG/InputData.h:50-58).  Coordinates
+ * are centred on the grid centre; w = charge * coeff_estat [* 332/diel]. */
+typedef struct {
+    int32_t n;
+    const double *x, *y, *z, *w;
+    const double *charge, *vol, *solpar;
+    const int32_t *type;  /* receptor type index (atomType) */
+    const int32_t *hbond; /* FG_HB_* */
+} fg_atoms;
+
+/* Bond geometry for directional H-bonds (FG/BondVectors.h:28-42). */
+typedef struct {
+    const double *rvector;    /* [n][3] */
+    const double *rvector2;   /* [n][3] */
+    const int32_t *rexp;      /* [n] */
+    const uint8_t *disorder;  /* [n] */
+} fg_hbond_geom;
+
+/* Lookup tables (FG/PairwiseInteractionEnergies.h:31-49, FG/DesolvExpFunc.h,
+ * FG/InputData.h:60).  pair is [n_types][FG_TABLE_LEN][n_maps]. */
+typedef struct {
+    int32_t n_types, n_maps;
+    const double *pair;
+    const double *desolv;  /* [FG_TABLE_LEN] */
+    const double *eps;     /* [FG_TABLE_LEN] inverse distance-dependent dielectric */
+} fg_tables;
+
+/* Per affinity map (FG/GridMap.h:27-55). */
+typedef struct {
+    int32_t hbond, is_hbonder, is_covalent;
+    double solpar_probe, vol_probe;
+} fg_map_params;
+
+/* Everything one map set needs: the reference's InputData + GridMapList +
+ * tables + bond vectors after precomputation (FG/main.cpp:150-193). */
+typedef struct {
+    fg_grid grid;
+    fg_atoms atoms;
+    fg_hbond_geom geom;
+    fg_tables tables;
+    int32_t n_maps;                     /* affinity maps */
+    const fg_map_params *maps;          /* [n_maps] */
+    const double *covalent;             /* [n_maps][P] initial values of covalent maps, or NULL */
+    double solpar_q;                    /* solparQ */
+    int32_t hydrogen;                   /* receptor type index of "HD", or -1 */
+    int32_t dist_dep_diel;              /* 1: binned eps table, 0: constant dielectric */
+} fg_job_desc;
+
+/* ---------------------------------------------------------------- */
+/* Errors                                                           */
+/* ---------------------------------------------------------------- */
+const char *fg_last_error(void);
+double fg_round3(double a); /* roundOutput, FG/autogrid.h:128-131 */
+
+/* ---------------------------------------------------------------- */
+/* Device context                                                   */
+/* ---------------------------------------------------------------- */
+/* Replaces the device set-up of FG/electrostatics/Cuda.cpp:130-136
+ * (cudaSetDevice + cudaStreamCreate) and the --cuda-dev choice. */
+int fg_create(int device, fg_ctx **out);
+void fg_destroy(fg_ctx *ctx);
+/* Number of visible GPUs (0 when none); never fails. */
+int fg_device_count(void);
+
+/* ---------------------------------------------------------------- */
+/* One-shot entry points on host buffers                            */
+/* ---------------------------------------------------------------- */
+/* Electrostatic map.  Replaces calculateElectrostaticMapAsync +
+ * synchronizeCalculation (FG/electrostatics/Electrostatics.h:32-36,
+ * FG/electrostatics/Cuda.cpp:75-199 and Cpu.cpp:27-88).  e_out[P] receives the
+ * unrounded sum.  eps may be NULL when dist_dep_diel == 0. */
+int fg_elec_map(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, const double *eps,
+                int dist_dep_diel, int precision, double *e_out);
+
+/* Affinity maps + desolvation map.  Replaces calculateGridmaps
+ * (FG/CalculateGridmaps.h:30-31, FG/CalculateGridmaps.cpp:433-485).
+ * maps_out[m] (P doubles each) are in/out: covalent maps must hold their
+ * initial values (FG/main.cpp:32-52), the others are overwritten.
+ * d_out[P] receives the desolvation map.  Outputs are unrounded. */
+int fg_affinity_d_maps(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms,
+                       const fg_hbond_geom *geom, const fg_tables *tables,
+                       const fg_map_params *maps, int n_maps, double solpar_q, int hydrogen,
+                       double *const *maps_out, double *d_out);
+
+/* ---------------------------------------------------------------- */
+/* Device-resident jobs (no reference counterpart: they split the   */
+/* one-shot calls into upload / compute / fetch so inputs and maps  */
+/* stay in HBM between runs, for benchmarking and batching)         */
+/* ---------------------------------------------------------------- */
+int fg_job_create(fg_ctx *ctx, const fg_job_desc *desc, fg_job **out);
+void fg_job_destroy(fg_job *job);
+/* Enqueue the computation on the context's stream; returns immediately. */
+int fg_job_run(fg_ctx *ctx, fg_job *job, unsigned what, int precision);
+/* Wait for everything enqueued on the context. */
+int fg_sync(fg_ctx *ctx);
+/* Copy maps back: out[m] for the n_maps affinity maps, out[n_maps] = e,
+ * out[n_maps+1] = d (P doubles each; NULL entries are skipped). */
+int fg_job_fetch(fg_ctx *ctx, fg_job *job, double *const *out);
+/* Number of (point, atom) pairs with r <= 8 A (computed on the device). */
+int fg_job_cutoff_pairs(fg_ctx *ctx, fg_job *job, int64_t *pairs);
+
+/* Many jobs in one persistent launch per map kind, captured once in a
+ * hipGraph and replayed.  Jobs must have been created on ctx. */
+int fg_batch_run(fg_ctx *ctx, fg_job *const *jobs, int n_jobs, unsigned what, int precision);
+
+/* Device time of the kernels of the last fg_job_run / fg_batch_run on ctx,
+ * measured with HIP events on the context's stream (ms). */
+typedef struct {
+    double elec_ms;      /* electrostatic kernel(s) */
+    double affinity_ms;  /* cutoff kernel(s), incl. the per-brick atom lists */
+    double total_ms;     /* first to last event */
+} fg_timing;
+int fg_last_timing(fg_ctx *ctx, fg_timing *t);
+
+/* ---------------------------------------------------------------- */
+/* Host front end: the reference's parsing, precompute and writers  */
+/* ---------------------------------------------------------------- */
+typedef struct {
+    int32_t v4;                /* --v4: AutoDock 4.0 default parameters (FG/main.cpp:145-147) */
+    int32_t ag421_param_order; /* read parameter_file where the GPF names it (AutoGrid 4.2.1) */
+} fg_load_opts;
+
+/* InputDataLoader::load + parameter file + tables + bond vectors
+ * (FG/main.cpp:145-193, FG/InputDataLoader.cpp:100-797).  Writes the .xyz file
+ * as the reference does.  On FG_E_INPUT, fg_prep_exit_code() gives the
+ * reference's exit code for the failure. */
+int fg_prep_load(const char *gpf_path, const fg_load_opts *opts, fg_prep **out);
+int fg_prep_exit_code(void);
+void fg_prep_free(fg_prep *prep);
+/* Views into the prepared arrays (valid until fg_prep_free). */
+int fg_prep_desc(const fg_prep *prep, fg_job_desc *out);
+/* Map bookkeeping for writers and callers. */
+int fg_prep_num_maps(const fg_prep *prep);                 /* affinity maps */
+const char *fg_prep_map_type(const fg_prep *prep, int m);  /* e.g. "OA" */
+int fg_prep_num_atoms(const fg_prep *prep);
+int64_t fg_prep_num_points(const fg_prep *prep);
+/* The GPF echo ("GPF> " lines) the reference writes into its .glg log. */
+const char *fg_prep_gpf_echo(const fg_prep *prep);
+/* Minimum / maximum of written map m (0..n_maps+1) after fg_write_outputs
+ * (GridMapList::logSummary, FG/GridMap.cpp:71-101). */
+int fg_prep_summary(const fg_prep *prep, int m, double *min, double *max);
+
+/* .fld + .map files (FG/GridMap.cpp:127-212, FG/Utils.cpp:40-84).  maps[m]
+ * for m < n_maps, maps[n_maps] = e, maps[n_maps+1] = d, unrounded.  Also
+ * writes the min/max summary into the prep for fg_write_log_summary. */
+int fg_write_outputs(fg_prep *prep, double *const *maps, const char *program_name, int threads);
+
+/* ---------------------------------------------------------------- */
+/* Synthetic receptors (seeded; deterministic across machines)      */
+/* ---------------------------------------------------------------- */
+typedef struct {
+    uint64_t seed;
+    int32_t n_atoms;
+    int32_t shape;              /* 0 sphere (radius a), 1 box (a x b x c, centred) */
+    double a, b, c;
+    double min_sep;             /* minimum pair separation (A), e.g. 1.2 */
+    int32_t npts[3];            /* points per axis (odd), e.g. 41 41 41 */
+    double spacing;             /* e.g. 0.375 */
+    const char *ligand_types;   /* e.g. "A C HD N OA" */
+    const char *stem;           /* file stem for gridfld/map names */
+} fg_synth_spec;
+/* Writes <dir>/<stem>.pdbqt and <dir>/<stem>.gpf (paths inside the GPF are
+ * relative to dir). */
+int fg_synth_write(const fg_synth_spec *spec, const char *dir);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,49 @@
+# Builds the reference FastGrid CPU path (OpenMP, no CUDA) from its own
+# sources where they lie under /root/reference, into oracle/_ref/.
+# Test infrastructure only: the binary is the checker for the oracle and an
+# optional CPU baseline, never part of the product.
+#
+#   make -f oracle/ref.mk -j8          (from the repository root)
+#
+# Sources: fastgrid_master/fastgrid/CMakeLists.txt:7-46 (the non-CUDA list).
+# AG_CUDA is left undefined so electrostatics/Cpu.cpp:114-127 provides the
+# e-map backend.  The only portability fix is a forced <stdint.h>/<cassert>
+# include (math/Int.h uses int32_t without including it); no source is edited
+# or copied.
+
+REF      ?= /root/reference
+FG       := $(REF)/fastgrid_master/fastgrid
+OUT      := oracle/_ref
+OBJ      := $(OUT)/obj
+
+CXX      ?= g++
+CXXFLAGS := -std=gnu++98 -O2 -DLinux -DFASTGRID -DAG_OPENMP -fopenmp \
+            -include stdint.h -include cassert -include unistd.h -w -I$(FG)
+
+SRCS := ann/src/perf.cpp ann/src/kd_dump.cpp ann/src/bd_tree.cpp ann/src/ANN.cpp \
+        ann/src/kd_search.cpp ann/src/brute.cpp ann/src/kd_tree.cpp ann/src/bd_search.cpp \
+        ann/src/bd_fix_rad_search.cpp ann/src/kd_util.cpp ann/src/kd_pr_search.cpp \
+        ann/src/kd_fix_rad_search.cpp ann/src/bd_pr_search.cpp ann/src/kd_split.cpp \
+        math/Geometric/Sphere3.cpp math/Geometric/AxisAlignedBox3.cpp \
+        math/Geometric/Vec4.cpp math/Geometric/Intersection3.cpp \
+        times.cpp DesolvExpFunc.cpp InputDataLoader.cpp ParameterLibrary.cpp \
+        PairwiseInteractionEnergies.cpp main.cpp BondVectors.cpp LogFile.cpp \
+        CalculateGridmaps.cpp ProgramParameters.cpp Utils.cpp GridMap.cpp \
+        electrostatics/Cpu.cpp
+
+OBJS := $(addprefix $(OBJ)/,$(SRCS:.cpp=.o))
+
+all: $(OUT)/fastgrid4_ref $(OUT)/testdiff
+
+$(OBJ)/%.o: $(FG)/%.cpp
+	@mkdir -p $(dir $@)
+	$(CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OUT)/fastgrid4_ref: $(OBJS)
+	$(CXX) -fopenmp -o $@ $(OBJS) -lpthread -lm
+
+$(OUT)/testdiff: $(REF)/testdiff_master/main.cpp
+	@mkdir -p $(OUT)
+	$(CXX) -O2 -w -include cstdlib -include cstring -include climits -o $@ $<
+
+.PHONY: all
