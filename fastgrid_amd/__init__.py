@@ -1,0 +1,12 @@
+"""fastgrid_amd — FastGrid's grid-map hot path on MI355X.
+
+The product is libfgmaps.so (C ABI in include/fgmaps.h: host front end plus
+gfx950 HIP kernels) and the fastgrid4 command line built beside it.  This
+package is its Python binding; see fastgrid_amd.api.
+"""
+from .api import (  # noqa: F401
+    LIB, LIB_PATH, PREC_COMPENSATED, PREC_FP64, PREC_PLAIN, RUN_ALL, RUN_AFFINITY, RUN_ELEC, SIGNATURES,
+    Context, FgError, Job, JobArrays, Prepared, batch_run, device_count, round3, synth_write,
+)
+
+BIN_DIR = __import__("os").path.join(__import__("os").path.dirname(__file__), "bin")

@@ -1,0 +1,406 @@
+"""ctypes binding of libfgmaps (include/fgmaps.h).
+
+The library is built in-tree by fastgrid_amd.build; importing this module
+loads it and raises if it is missing.  There is no Python or CPU fallback for
+any map computation: Context() fails when no GPU is visible.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfgmaps.so")
+
+FG_OK = 0
+PREC_PLAIN, PREC_COMPENSATED, PREC_FP64 = 0, 1, 2
+RUN_ELEC, RUN_AFFINITY, RUN_ALL = 1, 2, 3
+TABLE_LEN = 8192
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_u8p = C.POINTER(C.c_uint8)
+
+
+class FgGrid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("div2x", C.c_int32), ("div2y", C.c_int32), ("div2z", C.c_int32), ("spacing", C.c_double)]
+
+
+class FgAtoms(C.Structure):
+    _fields_ = [("n", C.c_int32), ("x", _dp), ("y", _dp), ("z", _dp), ("w", _dp),
+                ("charge", _dp), ("vol", _dp), ("solpar", _dp), ("type", _ip), ("hbond", _ip)]
+
+
+class FgHbondGeom(C.Structure):
+    _fields_ = [("rvector", _dp), ("rvector2", _dp), ("rexp", _ip), ("disorder", _u8p)]
+
+
+class FgTables(C.Structure):
+    _fields_ = [("n_types", C.c_int32), ("n_maps", C.c_int32), ("pair", _dp), ("desolv", _dp), ("eps", _dp)]
+
+
+class FgMapParams(C.Structure):
+    _fields_ = [("hbond", C.c_int32), ("is_hbonder", C.c_int32), ("is_covalent", C.c_int32),
+                ("solpar_probe", C.c_double), ("vol_probe", C.c_double)]
+
+
+class FgJobDesc(C.Structure):
+    _fields_ = [("grid", FgGrid), ("atoms", FgAtoms), ("geom", FgHbondGeom), ("tables", FgTables),
+                ("n_maps", C.c_int32), ("maps", C.POINTER(FgMapParams)), ("covalent", _dp),
+                ("solpar_q", C.c_double), ("hydrogen", C.c_int32), ("dist_dep_diel", C.c_int32)]
+
+
+class FgTiming(C.Structure):
+    _fields_ = [("elec_ms", C.c_double), ("affinity_ms", C.c_double), ("total_ms", C.c_double)]
+
+
+class FgLoadOpts(C.Structure):
+    _fields_ = [("v4", C.c_int32), ("ag421_param_order", C.c_int32)]
+
+
+class FgSynthSpec(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("n_atoms", C.c_int32), ("shape", C.c_int32),
+                ("a", C.c_double), ("b", C.c_double), ("c", C.c_double), ("min_sep", C.c_double),
+                ("npts", C.c_int32 * 3), ("spacing", C.c_double), ("ligand_types", C.c_char_p),
+                ("stem", C.c_char_p)]
+
+
+# Every symbol include/fgmaps.h declares, with its ctypes signature.
+SIGNATURES = {
+    "fg_last_error": (C.c_char_p, []),
+    "fg_round3": (C.c_double, [C.c_double]),
+    "fg_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "fg_destroy": (None, [C.c_void_p]),
+    "fg_device_count": (C.c_int, []),
+    "fg_elec_map": (C.c_int, [C.c_void_p, C.POINTER(FgGrid), C.POINTER(FgAtoms), _dp, C.c_int, C.c_int, _dp]),
+    "fg_affinity_d_maps": (C.c_int, [C.c_void_p, C.POINTER(FgGrid), C.POINTER(FgAtoms), C.POINTER(FgHbondGeom),
+                                     C.POINTER(FgTables), C.POINTER(FgMapParams), C.c_int, C.c_double, C.c_int,
+                                     C.POINTER(_dp), _dp]),
+    "fg_job_create": (C.c_int, [C.c_void_p, C.POINTER(FgJobDesc), C.POINTER(C.c_void_p)]),
+    "fg_job_destroy": (None, [C.c_void_p]),
+    "fg_job_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_int]),
+    "fg_sync": (C.c_int, [C.c_void_p]),
+    "fg_job_fetch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_dp)]),
+    "fg_job_cutoff_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]),
+    "fg_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, C.c_uint, C.c_int]),
+    "fg_last_timing": (C.c_int, [C.c_void_p, C.POINTER(FgTiming)]),
+    "fg_prep_load": (C.c_int, [C.c_char_p, C.POINTER(FgLoadOpts), C.POINTER(C.c_void_p)]),
+    "fg_prep_exit_code": (C.c_int, []),
+    "fg_prep_free": (None, [C.c_void_p]),
+    "fg_prep_desc": (C.c_int, [C.c_void_p, C.POINTER(FgJobDesc)]),
+    "fg_prep_num_maps": (C.c_int, [C.c_void_p]),
+    "fg_prep_map_type": (C.c_char_p, [C.c_void_p, C.c_int]),
+    "fg_prep_num_atoms": (C.c_int, [C.c_void_p]),
+    "fg_prep_num_points": (C.c_int64, [C.c_void_p]),
+    "fg_prep_gpf_echo": (C.c_char_p, [C.c_void_p]),
+    "fg_prep_summary": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
+    "fg_write_outputs": (C.c_int, [C.c_void_p, C.POINTER(_dp), C.c_char_p, C.c_int]),
+    "fg_synth_write": (C.c_int, [C.POINTER(FgSynthSpec), C.c_char_p]),
+}
+
+
+class FgError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libfgmaps error {code}: {msg}")
+        self.code = code
+
+
+def _load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `python -m fastgrid_amd.build` (no fallback exists)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+LIB = _load()
+
+
+def _check(rc: int) -> None:
+    if rc != FG_OK:
+        raise FgError(rc, (LIB.fg_last_error() or b"").decode())
+
+
+def round3(a: float) -> float:
+    return LIB.fg_round3(a)
+
+
+def _ptr(a: np.ndarray | None, ctype):
+    if a is None:
+        return C.cast(None, C.POINTER(ctype))
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _arr(ptr, n: int, dtype) -> np.ndarray:
+    if not ptr or n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+@dataclass
+class JobArrays:
+    """Prepared inputs of one map set as numpy arrays (host copies).  Field
+    meanings follow fg_job_desc in include/fgmaps.h."""
+    n: tuple[int, int, int]
+    div2: tuple[int, int, int]
+    spacing: float
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    w: np.ndarray
+    charge: np.ndarray
+    vol: np.ndarray
+    solpar: np.ndarray
+    type: np.ndarray
+    hbond: np.ndarray
+    rvector: np.ndarray
+    rvector2: np.ndarray
+    rexp: np.ndarray
+    disorder: np.ndarray
+    n_types: int
+    pair: np.ndarray
+    desolv: np.ndarray
+    eps: np.ndarray
+    map_hbond: np.ndarray
+    map_is_hbonder: np.ndarray
+    map_is_covalent: np.ndarray
+    map_solpar: np.ndarray
+    map_vol: np.ndarray
+    covalent: np.ndarray | None
+    solpar_q: float
+    hydrogen: int
+    dist_dep_diel: int
+
+    @property
+    def n_atoms(self) -> int:
+        return int(self.x.size)
+
+    @property
+    def n_maps(self) -> int:
+        return int(self.map_hbond.size)
+
+    @property
+    def P(self) -> int:
+        return int(self.n[0] * self.n[1] * self.n[2])
+
+    def desc(self) -> tuple[FgJobDesc, list]:
+        """fg_job_desc pointing into these arrays; keep the returned list alive."""
+        keep = []
+
+        def c(a, dtype):
+            a = np.ascontiguousarray(a, dtype=dtype)
+            keep.append(a)
+            return a
+
+        d = FgJobDesc()
+        d.grid = FgGrid(self.n[0], self.n[1], self.n[2], self.div2[0], self.div2[1], self.div2[2], self.spacing)
+        at = FgAtoms()
+        at.n = self.n_atoms
+        for f in ("x", "y", "z", "w", "charge", "vol", "solpar"):
+            setattr(at, f, _ptr(c(getattr(self, f), np.float64), C.c_double))
+        at.type = _ptr(c(self.type, np.int32), C.c_int32)
+        at.hbond = _ptr(c(self.hbond, np.int32), C.c_int32)
+        d.atoms = at
+        d.geom = FgHbondGeom(_ptr(c(self.rvector, np.float64), C.c_double), _ptr(c(self.rvector2, np.float64), C.c_double),
+                             _ptr(c(self.rexp, np.int32), C.c_int32), _ptr(c(self.disorder, np.uint8), C.c_uint8))
+        d.tables = FgTables(self.n_types, self.n_maps, _ptr(c(self.pair, np.float64), C.c_double),
+                            _ptr(c(self.desolv, np.float64), C.c_double), _ptr(c(self.eps, np.float64), C.c_double))
+        mp = (FgMapParams * max(self.n_maps, 1))()
+        for m in range(self.n_maps):
+            mp[m] = FgMapParams(int(self.map_hbond[m]), int(self.map_is_hbonder[m]), int(self.map_is_covalent[m]),
+                                float(self.map_solpar[m]), float(self.map_vol[m]))
+        keep.append(mp)
+        d.n_maps = self.n_maps
+        d.maps = C.cast(mp, C.POINTER(FgMapParams))
+        d.covalent = _ptr(c(self.covalent, np.float64), C.c_double) if self.covalent is not None else \
+            C.cast(None, _dp)
+        d.solpar_q = self.solpar_q
+        d.hydrogen = self.hydrogen
+        d.dist_dep_diel = self.dist_dep_diel
+        return d, keep
+
+
+class Prepared:
+    """A GPF loaded and precomputed by the host front end (fg_prep_load)."""
+
+    def __init__(self, gpf: str, v4: bool = False, ag421_param_order: bool = False):
+        self._h = C.c_void_p()
+        opts = FgLoadOpts(1 if v4 else 0, 1 if ag421_param_order else 0)
+        rc = LIB.fg_prep_load(gpf.encode(), C.byref(opts), C.byref(self._h))
+        if rc != FG_OK:
+            self.exit_code = LIB.fg_prep_exit_code()
+            _check(rc)
+
+    def close(self) -> None:
+        if self._h:
+            LIB.fg_prep_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def map_types(self) -> list[str]:
+        return [LIB.fg_prep_map_type(self._h, m).decode() for m in range(LIB.fg_prep_num_maps(self._h))]
+
+    def arrays(self) -> JobArrays:
+        d = FgJobDesc()
+        _check(LIB.fg_prep_desc(self._h, C.byref(d)))
+        g, a, geo, t = d.grid, d.atoms, d.geom, d.tables
+        N, M = a.n, d.n_maps
+        P = g.nx * g.ny * g.nz
+        mp = [d.maps[m] for m in range(M)]
+        return JobArrays(
+            n=(g.nx, g.ny, g.nz), div2=(g.div2x, g.div2y, g.div2z), spacing=g.spacing,
+            x=_arr(a.x, N, np.float64), y=_arr(a.y, N, np.float64), z=_arr(a.z, N, np.float64),
+            w=_arr(a.w, N, np.float64), charge=_arr(a.charge, N, np.float64), vol=_arr(a.vol, N, np.float64),
+            solpar=_arr(a.solpar, N, np.float64), type=_arr(a.type, N, np.int32), hbond=_arr(a.hbond, N, np.int32),
+            rvector=_arr(geo.rvector, 3 * N, np.float64), rvector2=_arr(geo.rvector2, 3 * N, np.float64),
+            rexp=_arr(geo.rexp, N, np.int32), disorder=_arr(geo.disorder, N, np.uint8),
+            n_types=t.n_types, pair=_arr(t.pair, t.n_types * TABLE_LEN * max(M, 1), np.float64),
+            desolv=_arr(t.desolv, TABLE_LEN, np.float64), eps=_arr(t.eps, TABLE_LEN, np.float64),
+            map_hbond=np.array([m.hbond for m in mp], np.int32),
+            map_is_hbonder=np.array([m.is_hbonder for m in mp], np.int32),
+            map_is_covalent=np.array([m.is_covalent for m in mp], np.int32),
+            map_solpar=np.array([m.solpar_probe for m in mp], np.float64),
+            map_vol=np.array([m.vol_probe for m in mp], np.float64),
+            covalent=_arr(d.covalent, M * P, np.float64) if d.covalent else None,
+            solpar_q=d.solpar_q, hydrogen=d.hydrogen, dist_dep_diel=d.dist_dep_diel)
+
+    def write_outputs(self, maps: list[np.ndarray], program_name: str = "fastgrid4", threads: int = 0) -> None:
+        keep = [np.ascontiguousarray(m, dtype=np.float64) for m in maps]
+        ptrs = (_dp * len(keep))(*[_ptr(m, C.c_double) for m in keep])
+        _check(LIB.fg_write_outputs(self._h, ptrs, program_name.encode(), threads))
+
+
+def device_count() -> int:
+    return LIB.fg_device_count()
+
+
+class Context:
+    """One GPU, one HIP stream (fg_create)."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _check(LIB.fg_create(device, C.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            LIB.fg_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def sync(self) -> None:
+        _check(LIB.fg_sync(self._h))
+
+    def timing(self) -> FgTiming:
+        t = FgTiming()
+        _check(LIB.fg_last_timing(self._h, C.byref(t)))
+        return t
+
+    def elec_map(self, arrays: JobArrays, precision: int = PREC_COMPENSATED) -> np.ndarray:
+        d, keep = arrays.desc()
+        out = np.zeros(arrays.P, np.float64)
+        _check(LIB.fg_elec_map(self._h, C.byref(d.grid), C.byref(d.atoms), d.tables.eps, d.dist_dep_diel, precision,
+                               _ptr(out, C.c_double)))
+        return out
+
+    def affinity_d_maps(self, arrays: JobArrays) -> tuple[list[np.ndarray], np.ndarray]:
+        d, keep = arrays.desc()
+        P, M = arrays.P, arrays.n_maps
+        outs = [arrays.covalent[m * P:(m + 1) * P].copy() if arrays.covalent is not None else np.zeros(P)
+                for m in range(M)]
+        dmap = np.zeros(P)
+        ptrs = (_dp * max(M, 1))(*[_ptr(o, C.c_double) for o in outs])
+        _check(LIB.fg_affinity_d_maps(self._h, C.byref(d.grid), C.byref(d.atoms), C.byref(d.geom), C.byref(d.tables),
+                                      d.maps, M, d.solpar_q, d.hydrogen, ptrs, _ptr(dmap, C.c_double)))
+        return outs, dmap
+
+
+class Job:
+    """Device-resident inputs and maps of one map set (fg_job_*)."""
+
+    def __init__(self, ctx: Context, arrays: JobArrays):
+        self.ctx = ctx
+        self.arrays = arrays
+        d, self._keep = arrays.desc()
+        self._h = C.c_void_p()
+        _check(LIB.fg_job_create(ctx.handle, C.byref(d), C.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            LIB.fg_job_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def run(self, what: int = RUN_ALL, precision: int = PREC_COMPENSATED) -> None:
+        _check(LIB.fg_job_run(self.ctx.handle, self._h, what, precision))
+
+    def fetch(self) -> list[np.ndarray]:
+        """[affinity maps..., e, d], unrounded FP64."""
+        P, M = self.arrays.P, self.arrays.n_maps
+        outs = [np.empty(P, np.float64) for _ in range(M + 2)]
+        ptrs = (_dp * (M + 2))(*[_ptr(o, C.c_double) for o in outs])
+        _check(LIB.fg_job_fetch(self.ctx.handle, self._h, ptrs))
+        return outs
+
+    def cutoff_pairs(self) -> int:
+        v = C.c_int64()
+        _check(LIB.fg_job_cutoff_pairs(self.ctx.handle, self._h, C.byref(v)))
+        return int(v.value)
+
+
+def batch_run(ctx: Context, jobs: list[Job], what: int = RUN_ALL, precision: int = PREC_COMPENSATED) -> None:
+    arr = (C.c_void_p * len(jobs))(*[j.handle for j in jobs])
+    _check(LIB.fg_batch_run(ctx.handle, arr, len(jobs), what, precision))
+
+
+def synth_write(directory: str, stem: str, seed: int, n_atoms: int, shape: str, dims, npts, ligand_types: str,
+                min_sep: float = 1.2, spacing: float = 0.375) -> tuple[str, str]:
+    """Writes <directory>/<stem>.pdbqt and .gpf; returns their paths."""
+    s = FgSynthSpec()
+    s.seed = seed
+    s.n_atoms = n_atoms
+    if shape == "sphere":
+        s.shape, s.a = 0, float(dims)
+    else:
+        s.shape = 1
+        s.a, s.b, s.c = (float(v) for v in dims)
+    s.min_sep = min_sep
+    s.npts = (C.c_int32 * 3)(*[int(v) for v in npts])
+    s.spacing = spacing
+    s.ligand_types = ligand_types.encode()
+    s.stem = stem.encode()
+    _check(LIB.fg_synth_write(C.byref(s), directory.encode()))
+    return os.path.join(directory, stem + ".pdbqt"), os.path.join(directory, stem + ".gpf")

@@ -295,16 +295,15 @@ int buildElecTiles(fg_job *j, const fg_job_desc &d, double gridMaxAbs)
         double ext = std::max(std::fabs(hi[k] - gl), std::fabs(gh - lo[k]));
         span2 += ext * ext;
     }
-    const double tmax = std::sqrt(span2) * 100.0 * (1 + 1e-5) + 4;
+    // padding atoms sit up to half a cell outside the atom box: add 2 cells
+    const double tmax = (std::sqrt(span2) + 2 * s) * 100.0 * (1 + 1e-5) + 4;
     std::vector<float> eps32;
     const double *eps = d.tables.eps;
-    if (tmax + 2 <= 16384) {
-        j->tableLen = std::max(fgk::kTableLen, int(std::ceil(tmax)) + 2);
-        j->clamp = false;
-    } else {
-        j->tableLen = fgk::kTableLen;
-        j->clamp = true;
-    }
+    // Entries past 8191 repeat entry 8191, so clamping the index to the end of
+    // the (possibly extended) table is the reference's min(k, 8191).
+    if (tmax + 2 <= 16384) j->tableLen = std::max(fgk::kTableLen, int(std::ceil(tmax)) + 2);
+    else j->tableLen = fgk::kTableLen;
+    j->clamp = true;
     eps32.resize(size_t(j->tableLen));
     for (int k = 0; k < j->tableLen; ++k) {
         double v = d.dist_dep_diel ? eps[std::min(k, fgk::kTableLen - 1)] : 1.0;

@@ -449,11 +449,13 @@ __global__ __launch_bounds__(kElecThreads) void k_elec(ElecArgs a)
                 for (int k = 0; k < PZ; ++k) {
                     const float dz = at.z - gz[k];
                     const float r2 = __builtin_fmaf(dz, dz, dxy2);
-                    float ri = __builtin_amdgcn_rsqf(r2);
-                    if (!kComp) ri = fminf(ri, 0.02f);  // min(1/r, 2) in scaled units
+                    const float ri = __builtin_amdgcn_rsqf(r2);
+                    // min(1/r, 2) in scaled units; the far-field pairs of the
+                    // compensated form are all beyond 8 A
+                    const float rc = kComp ? ri : fminf(ri, 0.02f);
                     unsigned bin = unsigned(r2 * ri);
-                    if (kClamp) bin = min(bin, unsigned(kTableLen - 1));
-                    part[k] = __builtin_fmaf(at.w * ri, eps[bin], part[k]);
+                    if (kClamp) bin = min(bin, unsigned(a.tableLen - 1));
+                    part[k] = __builtin_fmaf(at.w * rc, eps[bin], part[k]);
                 }
             }
         } else {

@@ -896,7 +896,10 @@ static void grid_point(const or_aff *A, const int *cand, int nc, double gx, doub
             if (A->mIsHB[m]) {
                 double rsph = sat(E / 100);
                 if ((mhb == OR_AS || mhb == OR_A2) && aDon) {
-                    double Ep = A->dis[ia] ? A->pair[((size_t)A->hydrogen * OR_MAX_DIST + (idx - 110 > 0 ? idx - 110 : 0)) * A->M + m] : E;
+                    /* a negative HD index is undefined in the reference; E is used */
+                    double Ep = (A->dis[ia] && A->hydrogen >= 0)
+                                    ? A->pair[((size_t)A->hydrogen * OR_MAX_DIST + (idx - 110 > 0 ? idx - 110 : 0)) * A->M + m]
+                                    : E;
                     *e += Ep * Hramp * (racc + (1 - racc) * rsph);
                 } else if (mhb == OR_A1 && aDon) {
                     double v = E * (racc + (1 - racc) * rsph);
@@ -1149,3 +1152,74 @@ void or_free(or_input *in)
     for (int i = 0; i < OR_MAX_MAPS + 2; i++) free(in->maps[i].energies);
     memset(in, 0, sizeof *in);
 }
+
+/* ------------------------------------------------------------------ */
+/* Accessors for the test harness (ctypes).                            */
+/* ------------------------------------------------------------------ */
+or_input *or_open(const char *gpf, int v4, int agParamOrder, int *rc)
+{
+    or_input *in = calloc(1, sizeof *in);
+    int r = or_load(in, gpf, v4, agParamOrder, NULL);
+    if (r == 0) r = or_precompute(in);
+    if (rc) *rc = r;
+    if (r) { or_free(in); free(in); return NULL; }
+    return in;
+}
+
+void or_close(or_input *in)
+{
+    if (!in) return;
+    or_free(in);
+    free(in);
+}
+
+/* N, numAtomMaps, tableTypes, n[3], div2[3], distDepDiel, hydrogen, P */
+void or_dims(const or_input *in, int out[12])
+{
+    int v[12] = {in->N, in->numAtomMaps, in->tableTypes, in->n[0], in->n[1], in->n[2],
+                 in->div2[0], in->div2[1], in->div2[2], in->distDepDiel, in->hydrogen, in->P};
+    memcpy(out, v, sizeof v);
+}
+
+/* spacing, solparQ, center xyz */
+void or_scalars(const or_input *in, double out[5])
+{
+    out[0] = in->spacing; out[1] = in->solparQ;
+    out[2] = in->center[0]; out[3] = in->center[1]; out[4] = in->center[2];
+}
+
+const double *or_darray(const or_input *in, const char *name)
+{
+    if (!strcmp(name, "x")) return in->x;
+    if (!strcmp(name, "y")) return in->y;
+    if (!strcmp(name, "z")) return in->z;
+    if (!strcmp(name, "w")) return in->w;
+    if (!strcmp(name, "charge")) return in->charge;
+    if (!strcmp(name, "vol")) return in->vol;
+    if (!strcmp(name, "solpar")) return in->solpar;
+    if (!strcmp(name, "rvector")) return in->rvec;
+    if (!strcmp(name, "rvector2")) return in->rvec2;
+    if (!strcmp(name, "pair")) return in->pair;
+    if (!strcmp(name, "desolv")) return in->desolv;
+    if (!strcmp(name, "eps")) return in->eps;
+    return NULL;
+}
+
+const int *or_iarray(const or_input *in, const char *name)
+{
+    if (!strcmp(name, "type")) return in->type;
+    if (!strcmp(name, "hbond")) return in->hbond;
+    if (!strcmp(name, "rexp")) return in->rexp;
+    return NULL;
+}
+
+const unsigned char *or_disorder(const or_input *in) { return in->disorder; }
+
+/* hbond, isHBonder, isCovalent; solparProbe, volProbe */
+void or_map_params(const or_input *in, int m, int iv[3], double dv[2])
+{
+    iv[0] = in->maps[m].hbond; iv[1] = in->maps[m].isHBonder; iv[2] = in->maps[m].isCovalent;
+    dv[0] = in->maps[m].solparProbe; dv[1] = in->maps[m].volProbe;
+}
+
+const char *or_error(const or_input *in) { return in ? in->err : ""; }

@@ -136,6 +136,17 @@ double or_round3(double a);
 long long or_count_cutoff_pairs(const int n[3], const int div2[3], double spacing,
                                 int N, const double *x, const double *y, const double *z, int threads);
 
+/* Accessors for the test harness: or_open() = or_load + or_precompute. */
+or_input *or_open(const char *gpf, int v4, int agParamOrder, int *rc);
+void or_close(or_input *in);
+void or_dims(const or_input *in, int out[12]);
+void or_scalars(const or_input *in, double out[5]);
+const double *or_darray(const or_input *in, const char *name);
+const int *or_iarray(const or_input *in, const char *name);
+const unsigned char *or_disorder(const or_input *in);
+void or_map_params(const or_input *in, int m, int iv[3], double dv[2]);
+const char *or_error(const or_input *in);
+
 #ifdef __cplusplus
 }
 #endif
