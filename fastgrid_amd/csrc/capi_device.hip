@@ -57,6 +57,7 @@ struct fg_job {
     std::vector<fgk::MapParam> mp;
     // cutoff inputs
     CutAtom *dAtoms = nullptr;
+    double *dAB = nullptr;
     double *dPair = nullptr, *dDesolv = nullptr, *dEps = nullptr, *dCovalent = nullptr;
     double *dXYZ = nullptr;
     int32_t *dHbond = nullptr, *dCellStart = nullptr, *dCellAtoms = nullptr;
@@ -150,6 +151,7 @@ fgk::CutoffArgs cutoffArgs(const fg_job *j, int doElecNear)
     a.doElecNear = doElecNear;
     a.P = j->P;
     a.atoms = j->dAtoms;
+    a.ab = j->dAB;
     a.brickStart = j->dStart;
     a.brickList = j->dList;
     a.hStart = j->dHStart;
@@ -330,8 +332,6 @@ int buildCutoff(fg_job *j, const fg_job_desc &d)
         c.z = A.z[i];
         c.w = A.w ? A.w[i] : 0.0;
         if (j->hasGeom) {
-            c.vol = A.vol[i];
-            c.solQ = A.solpar[i] + d.solpar_q * std::fabs(A.charge[i]);
             c.qvol = d.solpar_q * A.vol[i];
             for (int k = 0; k < 3; ++k) {
                 c.rv[k] = d.geom.rvector[3 * i + k];
@@ -349,6 +349,17 @@ int buildCutoff(fg_job *j, const fg_job_desc &d)
         hb[size_t(i)] = c.hbond;
     }
     if (int rc = devUpload(j, j->dAtoms, atoms.data(), atoms.size())) return rc;
+    // desolvation products per atom and map, formed as FG/CalculateGridmaps.cpp:234-235 forms them
+    std::vector<double> ab(size_t(std::max(N, 1)) * size_t(2 * std::max(j->M, 1)), 0.0);
+    if (j->hasGeom)
+        for (int i = 0; i < N; ++i) {
+            const double solQ = A.solpar[i] + d.solpar_q * std::fabs(A.charge[i]);
+            for (int m = 0; m < j->M; ++m) {
+                ab[size_t(i) * 2 * j->M + 2 * m] = d.maps[m].solpar_probe * A.vol[i];
+                ab[size_t(i) * 2 * j->M + 2 * m + 1] = solQ * d.maps[m].vol_probe;
+            }
+        }
+    if (int rc = devUpload(j, j->dAB, ab.data(), ab.size())) return rc;
     if (int rc = devUpload(j, j->dXYZ, xyz.data(), xyz.size())) return rc;
     if (int rc = devUpload(j, j->dHbond, hb.data(), hb.size())) return rc;
     // coarse cells: edge >= 4 A and >= the brick edge, so a brick's 8 A
@@ -522,9 +533,18 @@ int fg_job_create(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
         }
     }
     j->mp.resize(size_t(j->M));
+    int slots = 0;
     for (int m = 0; m < j->M; ++m) {
         const fg_map_params &p = d->maps[m];
-        j->mp[size_t(m)] = fgk::MapParam{p.hbond, p.is_hbonder, p.is_covalent, 0, p.solpar_probe, p.vol_probe};
+        // maps whose probe keeps an H-bond min/max (FG/CalculateGridmaps.cpp:217-223)
+        const bool keeps = p.is_hbonder && !p.is_covalent && (p.hbond == FG_HB_A1 || p.hbond == FG_HB_DS || p.hbond == FG_HB_D1);
+        j->mp[size_t(m)] = fgk::MapParam{p.hbond, p.is_hbonder, p.is_covalent, keeps ? slots : -1, p.solpar_probe,
+                                         p.vol_probe};
+        if (keeps) ++slots;
+    }
+    if (slots > fgk::kMaxHbSlots) {
+        destroyJob(j);
+        return fgc::fail(FG_E_ARG, "fg_job_create: at most 4 maps of donor (DS/D1) or A1 probes are supported");
     }
     double gridMaxAbs = 0;
     const int n[3] = {g.nx, g.ny, g.nz}, dv[3] = {g.div2x, g.div2y, g.div2z};

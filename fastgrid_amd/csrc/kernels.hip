@@ -258,11 +258,42 @@ __device__ __forceinline__ void hbondAngular(int hb, int rexp, int disorder, boo
     }
 }
 
+// FP64 1/sqrt(x) and x*that from the hardware estimate plus two Newton steps:
+// a few ulp from the correctly rounded values, which is all the continuous
+// uses need; table-bin decisions made from it are guarded (see binGuarded).
+__device__ __forceinline__ double rsqrtFast(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = y * __builtin_fma(-h * y, y, 1.5);
+    y = y * __builtin_fma(-h * y, y, 1.5);
+    return y;
+}
+
+// floor(100 r) from an approximation r_a within ~1e-15 relative of the exact
+// r: equal to the reference's angstromToIndex unless 100 r_a lies within 1e-7
+// of an integer, in which case `edge` is set and the caller recomputes r in
+// the reference's exact operation order.
+__device__ __forceinline__ int binGuarded(double r, bool &edge)
+{
+    const double t = r * 100.0;
+    const int k = int(t);
+    const double f = t - double(k);
+    edge = f < 1e-7 || f > 1.0 - 1e-7;
+    return k < kTableLen - 1 ? k : kTableLen - 1;
+}
+
+constexpr int kChunk = 32;  // candidate atoms staged in LDS per round
+
 template <int MM, bool kAffinity>
 __global__ __launch_bounds__(kCutThreads) void k_cutoff(CutoffArgs a)
 {
+    __shared__ CutAtom sAtom[kChunk];
+    __shared__ double sAB[kAffinity ? kChunk * 2 * MM : 1];
+    __shared__ int32_t sIdx[kChunk];
+
     // XCD-aware brick order: consecutive workgroups go to different XCDs, so
-    // hand each XCD a contiguous run of bricks (neighbours share atoms in L2).
+    // hand each XCD a contiguous run of bricks (neighbours share atoms in L2)
     // (bijection on the first 8*floor(nb/8) workgroups, identity on the tail)
     const int nb = a.nbx * a.nby * a.nbz;
     const int full = (nb / 8) * 8, g = int(blockIdx.x);
@@ -288,8 +319,8 @@ __global__ __launch_bounds__(kCutThreads) void k_cutoff(CutoffArgs a)
         const int h0 = a.hStart[b], h1 = a.hStart[b + 1];
         for (int c = h0; c < h1; ++c) {
             const int ia = uniform(a.hList[c]);
-            const CutAtom &A = a.atoms[ia];
-            double dx = A.x - gx, dy = A.y - gy, dz = A.z - gz;
+            const double ax = a.atoms[ia].x, ay = a.atoms[ia].y, az = a.atoms[ia].z;
+            double dx = ax - gx, dy = ay - gy, dz = az - gz;
             double r2 = dx * dx + dy * dy + dz * dz;
             if (r2 < rmin || (r2 == rmin && ia < cH)) {
                 rmin = r2;
@@ -301,84 +332,119 @@ __global__ __launch_bounds__(kCutThreads) void k_cutoff(CutoffArgs a)
         cHrv[2] = a.atoms[cH].rv[2];
     }
 
-    double e[MM], hmin[MM], hmax[MM];
+    double e[MM];
+    double hmin[kMaxHbSlots], hmax[kMaxHbSlots];
     unsigned flags = 0;
 #pragma unroll
-    for (int m = 0; m < MM; ++m) {
+    for (int m = 0; m < MM; ++m)
         e[m] = (kAffinity && m < M && a.covalent && a.mp[m].isCovalent) ? a.covalent[int64_t(m) * a.P + p] : 0.0;
-        hmin[m] = 999999;
-        hmax[m] = -999999;
+#pragma unroll
+    for (int s = 0; s < kMaxHbSlots; ++s) {
+        hmin[s] = 999999;
+        hmax[s] = -999999;
     }
     double dmap = 0, enear = 0;
 
     const int c0 = a.brickStart[b], c1 = a.brickStart[b + 1];
-    for (int c = c0; c < c1; ++c) {
-        const int ia = uniform(a.brickList[c]);
-        const CutAtom &A = a.atoms[ia];
-        double dx = A.x - gx, dy = A.y - gy, dz = A.z - gz;
-        double rSq = dx * dx + dy * dy + dz * dz;
-        if (!valid || rSq > 64.0) continue;
-        if (a.doElecNear) {
-            // the electrostatic term exactly as FG/electrostatics/Cpu.cpp:44-75 forms it
-            double dydz = dy * dy + dz * dz;
-            double r = sqrt(dx * dx + dydz);
-            double inv = 1 / r;
-            enear += A.w * dmin_(inv, 2) * a.eps[binOf(r)];
+    for (int cb = c0; cb < c1; cb += kChunk) {
+        const int nc = min(kChunk, c1 - cb);
+        __syncthreads();
+        // stage nc atom records (16-byte pieces) and their desolvation products
+        if (tid < nc) sIdx[tid] = a.brickList[cb + tid];
+        __syncthreads();
+        for (int q = tid; q < nc * 8; q += kCutThreads) {
+            const float4 *src = reinterpret_cast<const float4 *>(a.atoms + sIdx[q >> 3]);
+            reinterpret_cast<float4 *>(sAtom)[q] = src[q & 7];
         }
-        if (!kAffinity) continue;
-        const int type = A.type, ahb = A.hbond, dis = A.disorder;
-        if (type == a.hydrogen && dis) continue;
-        if (rSq < 1.0E-24) rSq = 1.0E-24;
-        const double invR = 1 / sqrt(rSq);
-        const double ux = dx * invR, uy = dy * invR, uz = dz * invR;
-        const int idx = binOf(1 / invR);
-        double racc, rdon, Hramp;
-        if (ahb == HB_D1 || ahb == HB_A1 || ahb == HB_A2) {
-            const double rv[3] = {A.rv[0], A.rv[1], A.rv[2]};
-            const double rv2[3] = {A.rv2[0], A.rv2[1], A.rv2[2]};
-            hbondAngular(ahb, A.rexp, dis, ia == cH, rv, rv2, cHrv, ux, uy, uz, racc, rdon, Hramp);
-        } else {
-            racc = 1;
-            rdon = 1;
-            Hramp = 1;
-        }
-        const bool aDon = ahb == HB_DS || ahb == HB_D1;
-        const double S = a.desolv[idx];
-        const double *row = a.pair + (int64_t(type) * kTableLen + idx) * M;
-        const double *rowH = (dis && a.hydrogen >= 0)
-                                 ? a.pair + (int64_t(a.hydrogen) * kTableLen + (idx - 110 > 0 ? idx - 110 : 0)) * M
-                                 : row;
-        const double vol = A.vol, solQ = A.solQ;
+        if (kAffinity)
+            for (int q = tid; q < nc * MM; q += kCutThreads) {
+                const int c = q / MM, m = q - c * MM;
+                if (m < M) {
+                    const double *src = a.ab + int64_t(sIdx[c]) * 2 * M + 2 * m;
+                    sAB[(c * MM + m) * 2] = src[0];
+                    sAB[(c * MM + m) * 2 + 1] = src[1];
+                }
+            }
+        __syncthreads();
+        for (int c = 0; c < nc; ++c) {
+            const CutAtom &A = sAtom[c];
+            const double dx = A.x - gx, dy = A.y - gy, dz = A.z - gz;
+            const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+            double rSq = dx2 + dy2 + dz2;
+            if (!valid || rSq > 64.0) continue;
+            if (rSq < 1.0E-24) rSq = 1.0E-24;
+            // guarded fast geometry (see binGuarded)
+            const double yinv = rsqrtFast(rSq);
+            bool edge;
+            int idx = binGuarded(rSq * yinv, edge);
+            int idxE = idx;
+            if (edge) {
+                // exact: the cutoff loop's 1/(1/sqrt(r^2)) and the e-map's
+                // sqrt(dx^2 + (dy^2 + dz^2))  (FG/CalculateGridmaps.cpp:286-289,
+                // FG/electrostatics/Cpu.cpp:70-75)
+                const double invR = 1 / sqrt(rSq);
+                idx = binOf(1 / invR);
+                idxE = binOf(sqrt(dx2 + (dy2 + dz2)));
+            }
+            if (a.doElecNear) enear += A.w * dmin_(yinv, 2) * a.eps[idxE];
+            if (!kAffinity) continue;
+            const int type = A.type, ahb = A.hbond, dis = A.disorder;
+            if (type == a.hydrogen && dis) continue;
+            double racc = 1, rdon = 1, Hramp = 1;
+            if (ahb == HB_D1 || ahb == HB_A1 || ahb == HB_A2) {
+                // directional terms use the unit vector exactly as the reference forms it
+                const double invR = 1 / sqrt(rSq);
+                const double rv[3] = {A.rv[0], A.rv[1], A.rv[2]};
+                const double rv2[3] = {A.rv2[0], A.rv2[1], A.rv2[2]};
+                hbondAngular(ahb, A.rexp, dis, uniform(sIdx[c]) == cH, rv, rv2, cHrv, dx * invR, dy * invR, dz * invR,
+                             racc, rdon, Hramp);
+            }
+            const bool aDon = ahb == HB_DS || ahb == HB_D1;
+            const double S = a.desolv[idx];
+            const double *row = a.pair + (int64_t(type) * kTableLen + idx) * M;
+            const double *rowH = (dis && a.hydrogen >= 0)
+                                     ? a.pair + (int64_t(a.hydrogen) * kTableLen + (idx - 110 > 0 ? idx - 110 : 0)) * M
+                                     : row;
+            const double *ab = &sAB[c * MM * 2];
 #pragma unroll
-        for (int m = 0; m < MM; ++m) {
-            if (m >= M || a.mp[m].isCovalent) continue;
-            const double E = row[m];
-            const int mhb = a.mp[m].hbond;
-            if (a.mp[m].isHBonder) {
-                if ((mhb == HB_AS || mhb == HB_A2) && aDon) {
-                    double rsph = dmin_(dmax_(E / 100, 0), 1);
-                    e[m] += rowH[m] * Hramp * (racc + (1 - racc) * rsph);
-                } else if (mhb == HB_A1 && aDon) {
-                    double rsph = dmin_(dmax_(E / 100, 0), 1);
-                    double v = E * (racc + (1 - racc) * rsph);
-                    hmin[m] = dmin_(hmin[m], v);
-                    hmax[m] = dmax_(hmax[m], v);
-                    flags |= 1u << m;
-                } else if ((mhb == HB_DS || mhb == HB_D1) && ahb >= HB_AS) {
-                    double rsph = dmin_(dmax_(E / 100, 0), 1);
-                    double v = E * (rdon + (1 - rdon) * rsph);
-                    hmin[m] = dmin_(hmin[m], v);
-                    hmax[m] = dmax_(hmax[m], v);
-                    flags |= 1u << m;
+            for (int m = 0; m < MM; ++m) {
+                if (m >= M || a.mp[m].isCovalent) continue;
+                const double E = row[m];
+                const int mhb = a.mp[m].hbond;
+                double ins = 0;
+                bool doIns = false;
+                if (a.mp[m].isHBonder) {
+                    if ((mhb == HB_AS || mhb == HB_A2) && aDon) {
+                        double rsph = dmin_(dmax_(E / 100, 0), 1);
+                        e[m] += rowH[m] * Hramp * (racc + (1 - racc) * rsph);
+                    } else if (mhb == HB_A1 && aDon) {
+                        double rsph = dmin_(dmax_(E / 100, 0), 1);
+                        ins = E * (racc + (1 - racc) * rsph);
+                        doIns = true;
+                    } else if ((mhb == HB_DS || mhb == HB_D1) && ahb >= HB_AS) {
+                        double rsph = dmin_(dmax_(E / 100, 0), 1);
+                        ins = E * (rdon + (1 - rdon) * rsph);
+                        doIns = true;
+                    } else {
+                        e[m] += E;
+                    }
                 } else {
                     e[m] += E;
                 }
-            } else {
-                e[m] += E;
+                if (doIns) {
+                    const int s = a.mp[m].hbSlot;  // wave-uniform
+#pragma unroll
+                    for (int k = 0; k < kMaxHbSlots; ++k)
+                        if (s == k) {
+                            hmin[k] = dmin_(hmin[k], ins);
+                            hmax[k] = dmax_(hmax[k], ins);
+                        }
+                    flags |= 1u << m;
+                }
+                e[m] += ab[2 * m] * S + ab[2 * m + 1] * S;
             }
-            e[m] += a.mp[m].solparProbe * vol * S + solQ * a.mp[m].volProbe * S;
+            dmap += A.qvol * S;
         }
-        dmap += A.qvol * S;
     }
     if (!valid) return;
     if (kAffinity) {
@@ -386,7 +452,17 @@ __global__ __launch_bounds__(kCutThreads) void k_cutoff(CutoffArgs a)
         for (int m = 0; m < MM; ++m) {
             if (m >= M) continue;
             double v = e[m];
-            if (flags & (1u << m)) v += hmin[m] + hmax[m];
+            if (flags & (1u << m)) {
+                const int s = a.mp[m].hbSlot;
+                double mn = 0, mx = 0;
+#pragma unroll
+                for (int k = 0; k < kMaxHbSlots; ++k)
+                    if (s == k) {
+                        mn = hmin[k];
+                        mx = hmax[k];
+                    }
+                v += mn + mx;
+            }
             a.maps[int64_t(m) * a.P + p] = v;
         }
         a.maps[int64_t(M + 1) * a.P + p] = dmap;
@@ -440,22 +516,29 @@ __global__ __launch_bounds__(kElecThreads) void k_elec(ElecArgs a)
         }
         const float4 *T = a.tiles + int64_t(t) * kTile;
         if (far) {
-#pragma unroll 4
+#pragma unroll 2
             for (int j = 0; j < kTile; ++j) {
                 const float4 at = T[j];
                 const float dx = at.x - gx, dy = at.y - gy;
                 const float dxy2 = __builtin_fmaf(dx, dx, dy * dy);
+                float ri[PZ], ev[PZ];
+                unsigned bin[PZ];
 #pragma unroll
                 for (int k = 0; k < PZ; ++k) {
                     const float dz = at.z - gz[k];
                     const float r2 = __builtin_fmaf(dz, dz, dxy2);
-                    const float ri = __builtin_amdgcn_rsqf(r2);
+                    ri[k] = __builtin_amdgcn_rsqf(r2);
+                    bin[k] = unsigned(r2 * ri[k]);
+                    if (kClamp) bin[k] = min(bin[k], unsigned(a.tableLen - 1));
+                }
+#pragma unroll
+                for (int k = 0; k < PZ; ++k) ev[k] = eps[bin[k]];
+#pragma unroll
+                for (int k = 0; k < PZ; ++k) {
                     // min(1/r, 2) in scaled units; the far-field pairs of the
                     // compensated form are all beyond 8 A
-                    const float rc = kComp ? ri : fminf(ri, 0.02f);
-                    unsigned bin = unsigned(r2 * ri);
-                    if (kClamp) bin = min(bin, unsigned(a.tableLen - 1));
-                    part[k] = __builtin_fmaf(at.w * rc, eps[bin], part[k]);
+                    const float rc = kComp ? ri[k] : fminf(ri[k], 0.02f);
+                    part[k] = __builtin_fmaf(at.w * rc, ev[k], part[k]);
                 }
             }
         } else {

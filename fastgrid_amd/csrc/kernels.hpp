@@ -38,25 +38,27 @@ constexpr int kElecThreads = kElecBX * kElecBY;
 
 enum { HB_NON = 0, HB_DS = 1, HB_D1 = 2, HB_AS = 3, HB_A1 = 4, HB_A2 = 5 };
 
-// One receptor atom as the cutoff kernel reads it (128 bytes, one scalar
-// cache line pair).  Derived terms are precomputed on the host in FP64 with
-// the reference's operation order.
+// One receptor atom as the cutoff kernel reads it (128 bytes).  Derived terms
+// are precomputed on the host in FP64 with the reference's operation order.
 struct alignas(16) CutAtom {
     double x, y, z, w;   // centred coordinates; w = q * coeff_estat [* 332/diel]
-    double vol;          // atomic fragmental volume
-    double solQ;         // solpar + solparQ * |q|  (FG/CalculateGridmaps.cpp:235)
     double qvol;         // solparQ * vol            (FG/CalculateGridmaps.cpp:302)
     double rv[3];        // bond vector
     double rv2[3];       // second bond vector
+    double pad[3];
     int32_t type, hbond, rexp, disorder;
 };
 static_assert(sizeof(CutAtom) == 128, "CutAtom must stay 128 bytes");
+// Per atom and map, the two desolvation products of
+// FG/CalculateGridmaps.cpp:234-235, formed exactly as the reference forms them:
+// ab[2m] = solparProbe_m * vol_a, ab[2m+1] = (solpar_a + solparQ*|q_a|) * volProbe_m.
 
 // Per affinity map, uniform across the kernel.
 struct MapParam {
-    int32_t hbond, isHBonder, isCovalent, pad;
+    int32_t hbond, isHBonder, isCovalent, hbSlot;  // hbSlot: min/max slot or -1
     double solparProbe, volProbe;
 };
+constexpr int kMaxHbSlots = 4;  // maps that keep an H-bond min/max per point
 
 struct GridGeom {
     int32_t nx, ny, nz, dx2, dy2, dz2;
@@ -76,6 +78,7 @@ struct CutoffArgs {
     int32_t M, hydrogen, doElecNear, pad;
     int64_t P;
     const CutAtom *atoms;
+    const double *ab;                         // [N][2M] desolvation products
     const int32_t *brickStart, *brickList;    // all candidates
     const int32_t *hStart, *hList;            // DS/D1 candidates
     const double *pair;    // [T][8192][M]

@@ -1,0 +1,82 @@
+"""Helpers that materialise the golden fixtures of tests/golden/ into a
+scratch directory, read map files, and apply the reference's 3-decimal
+rounding (FG/autogrid.h:128-131)."""
+from __future__ import annotations
+
+import gzip
+import hashlib
+import json
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def r3(a):
+    a = np.asarray(a, dtype=np.float64)
+    return np.where(np.abs(a) < 0.0005, 0.0, np.floor(a * 1000 + 0.5) * 0.001)
+
+
+def milli(a):
+    """Rounded map values as int thousandths."""
+    return np.rint(r3(a) * 1000).astype(np.int64)
+
+
+def read_map(path):
+    lines = open(path).read().split("\n")
+    return lines[:6], np.rint(np.array([float(v) for v in lines[6:] if v.strip()]) * 1000).astype(np.int64)
+
+
+def goldens(name, which):
+    with np.load(os.path.join(GOLDEN, name, which)) as z:
+        return {k: z[k].astype(np.int64) for k in z.files}
+
+
+def _copy(src, dst):
+    """Copies file content only (fixture files may be read-only)."""
+    with open(src, "rb") as f, open(dst, "wb") as g:
+        g.write(f.read())
+
+
+def colagen2(tmp):
+    """Returns (run_dir, gpf) with the receptor at ../ as the GPF expects."""
+    src = os.path.join(GOLDEN, "colagen2")
+    _copy(os.path.join(src, "colagen2.gpf"), os.path.join(tmp, "colagen2.gpf"))
+    with gzip.open(os.path.join(src, "macromol_1t60_half_prot.pdbqt.gz"), "rb") as g:
+        open(os.path.join(tmp, "macromol_1t60_half_prot.pdbqt"), "wb").write(g.read())
+    run = os.path.join(tmp, "run")
+    os.makedirs(run, exist_ok=True)
+    return run, "../colagen2.gpf"
+
+
+def pgp(tmp):
+    src = os.path.join(GOLDEN, "pgp")
+    for f in ("1pgp.gpf", "AD4_parameters.dat"):
+        _copy(os.path.join(src, f), os.path.join(tmp, f))
+    with gzip.open(os.path.join(src, "1pgp_rec.pdbqt.gz"), "rb") as g:
+        open(os.path.join(tmp, "1pgp_rec.pdbqt"), "wb").write(g.read())
+    return tmp, "1pgp.gpf"
+
+
+def synth(tmp, name):
+    """Regenerates a synthetic fixture input and checks it is the one the
+    goldens were made from."""
+    import fastgrid_amd as fg
+    meta = json.load(open(os.path.join(GOLDEN, "synth", "synth.json")))[name]
+    fg.synth_write(tmp, name, meta["seed"], meta["n_atoms"], meta["shape"], meta["dims"], meta["npts"],
+                   meta["ligand_types"])
+    for ext in ("pdbqt", "gpf"):
+        h = hashlib.sha256(open(os.path.join(tmp, f"{name}.{ext}"), "rb").read()).hexdigest()
+        assert h == meta[f"{ext}_sha256"], f"generator output for {name}.{ext} changed"
+    return tmp, f"{name}.gpf", meta["ligand_types"].split() + ["e", "d"]
+
+
+def load_arrays(run_dir, gpf, **kw):
+    import fastgrid_amd as fg
+    cwd = os.getcwd()
+    os.chdir(run_dir)
+    try:
+        return fg.Prepared(gpf, **kw).arrays()
+    finally:
+        os.chdir(cwd)

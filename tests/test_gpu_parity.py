@@ -1,0 +1,194 @@
+"""GPU parity: libfgmaps on an MI355X against the reference goldens and the
+FP64 oracle.
+
+Tolerances: affinity and desolvation maps are computed in the reference's
+FP64 arithmetic and must print identically (rounded maps bit-exact); the
+electrostatic map must stay within 1e-3 kcal/mol of the FP64 result (the
+compensated form; the FP64 form is bit-exact)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests import fixtures as fx
+
+pytestmark = pytest.mark.gpu
+E_TOL = 1e-3 + 1e-9
+
+
+def fastgrid4(run, gpf, *args):
+    import fastgrid_amd as fg
+    exe = os.path.join(fg.BIN_DIR, "fastgrid4")
+    p = subprocess.run([exe, "-p", gpf, "-l", "fg.glg", *args], cwd=run, stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    assert "Successful Completion" in open(os.path.join(run, "fg.glg")).read()
+    return p
+
+
+def compare_files(run, stem, gold, exact_e=False):
+    for n, want in gold.items():
+        _, got = fx.read_map(os.path.join(run, f"{stem}.{n}.map"))
+        if n == "e" and not exact_e:
+            assert np.abs(got - want).max() <= 1, (n, np.abs(got - want).max())
+        else:
+            assert np.array_equal(got, want), (n, int((got != want).sum()))
+
+
+def test_cli_colagen2_goldens(tmp_path):
+    run, gpf = fx.colagen2(str(tmp_path))
+    fastgrid4(run, gpf, "--v4")
+    compare_files(run, "colagen2", fx.goldens("colagen2", "goldens_autogrid400.npz"))
+    assert open(os.path.join(run, "colagen2.maps.xyz")).read() == \
+        open(os.path.join(fx.GOLDEN, "colagen2", "colagen2.maps.xyz")).read()
+
+
+def test_cli_colagen2_fp64_form_is_exact(tmp_path):
+    run, gpf = fx.colagen2(str(tmp_path))
+    fastgrid4(run, gpf, "--v4", "--precision=fp64")
+    compare_files(run, "colagen2", fx.goldens("colagen2", "goldens_autogrid400.npz"), exact_e=True)
+
+
+def test_cli_pgp_fastgrid_and_autogrid_order(tmp_path):
+    run, gpf = fx.pgp(str(tmp_path))
+    fast = fx.goldens("pgp", "goldens_fastgrid_v4.npz")
+    fastgrid4(run, gpf, "--v4")
+    compare_files(run, "1pgp_rec", fast)
+    delta = fx.goldens("pgp", "goldens_autogrid421_minus_fastgrid.npz")
+    fastgrid4(run, gpf, "--v4", "--ag421-param-order")
+    for n in ("e", "d"):
+        _, got = fx.read_map(os.path.join(run, f"1pgp_rec.{n}.map"))
+        assert np.abs(got - (fast[n] + delta[n])).max() <= (1 if n == "e" else 0), n
+
+
+@pytest.mark.parametrize("name", ["micro", "S"])
+def test_cli_synthetic_goldens(tmp_path, name):
+    run, gpf, names = fx.synth(str(tmp_path), name)
+    fastgrid4(run, gpf)
+    compare_files(run, name, fx.goldens("synth", f"{name}_goldens_fastgrid.npz"))
+
+
+def _kernel_check(ctx, a, comp_tol=3e-4):
+    import fastgrid_amd as fg
+    from oracle import oracle as orc
+    job = fg.Job(ctx, a)
+    oe = orc.elec_map(a)
+    om, od = orc.affinity_maps(a)
+    M = a.n_maps
+    res = {}
+    for prec in (fg.PREC_FP64, fg.PREC_COMPENSATED, fg.PREC_PLAIN):
+        job.run(fg.RUN_ALL, prec)
+        out = job.fetch()
+        for m in range(M):
+            assert np.array_equal(fx.milli(out[m]), fx.milli(om[m])), (prec, m)
+            assert np.allclose(out[m], om[m], rtol=1e-12, atol=1e-9), (prec, m)
+        assert np.array_equal(fx.milli(out[M + 1]), fx.milli(od))
+        err = np.abs(out[M] - oe).max()
+        res[prec] = err
+        if prec == fg.PREC_FP64:
+            assert err == 0.0
+        elif prec == fg.PREC_COMPENSATED:
+            assert err < comp_tol, err
+            assert np.abs(fx.milli(out[M]) - fx.milli(oe)).max() <= 1
+    assert job.cutoff_pairs() == orc.cutoff_pairs(a)
+    job.close()
+    return res
+
+
+@pytest.mark.parametrize("name", ["micro", "S"])
+def test_kernels_vs_oracle_synthetic(ctx, tmp_path, name):
+    run, gpf, _ = fx.synth(str(tmp_path), name)
+    _kernel_check(ctx, fx.load_arrays(run, gpf))
+
+
+def test_kernels_vs_oracle_medium(ctx, tmp_path):
+    import fastgrid_amd as fg
+    fg.synth_write(str(tmp_path), "M", 2, 8000, "sphere", 40.0, (81, 81, 81), "A C HD N NA OA SA S")
+    _kernel_check(ctx, fx.load_arrays(str(tmp_path), "M.gpf"))
+
+
+def _edit_gpf(src_dir, gpf, out_dir, edits, extra=()):
+    text = open(os.path.join(src_dir, gpf)).read()
+    for a, b in edits:
+        assert a in text, a
+        text = text.replace(a, b)
+    text += "".join(l + "\n" for l in extra)
+    open(os.path.join(out_dir, gpf), "w").write(text)
+
+
+@pytest.mark.parametrize("variant", ["constant_dielectric", "disorder_h", "covalent_qasp", "odd_grid", "one_point"])
+def test_kernels_vs_oracle_gpf_variants(ctx, tmp_path, variant):
+    run, gpf = fx.colagen2(str(tmp_path))
+    d = os.path.dirname(run)
+    edits = {
+        "constant_dielectric": [("dielectric -0.1465", "dielectric 40"), ("npts 40 40 40", "npts 20 20 20")],
+        "disorder_h": [("npts 40 40 40", "npts 24 24 24")],
+        "covalent_qasp": [("npts 40 40 40", "npts 16 18 20"), ("spacing 1 ", "spacing 0.375 "),
+                          ("ligand_types  C HD OA", "ligand_types  C HD OA Z"),
+                          ("map colagen2.OA.map", "map colagen2.OA.map\nmap colagen2.Z.map\ncovalentmap 1.7 1000.0 64.5 -16.0 37.2")],
+        "odd_grid": [("npts 40 40 40", "npts 21 34 9"), ("spacing 1 ", "spacing 0.4 ")],
+        "one_point": [("npts 40 40 40", "npts 0 0 0")],
+    }[variant]
+    extra = {"disorder_h": ["disorder_h"], "covalent_qasp": ["qasp 0.01234"]}.get(variant, [])
+    _edit_gpf(d, "colagen2.gpf", d, edits, extra)
+    a = fx.load_arrays(run, gpf, v4=True)
+    _kernel_check(ctx, a)
+
+
+def test_atom_on_grid_point(ctx, tmp_path):
+    """r = 0 pairs: min(1/r, 2) and bin 0 (FG/electrostatics/Cpu.cpp:70-75),
+    rSq clamped to 1e-24 in the cutoff loop (FG/CalculateGridmaps.cpp:284-285)."""
+    lines = []
+    pos = [(0.0, 0.0, 0.0), (0.375, 0.0, 0.0), (1.2, 0.9, -0.3), (0.0, 0.96, 0.0), (-1.1, 0.2, 0.4)]
+    types = ["OA", "C", "N", "HD", "SA"]
+    q = [-0.4, 0.2, -0.1, 0.3, 0.0]
+    for i, ((x, y, z), t, c) in enumerate(zip(pos, types, q)):
+        lines.append(f"ATOM  {i + 1:5d} X{i:<3d} RES A   1    {x:8.3f}{y:8.3f}{z:8.3f}  1.00  0.00    {c:6.3f} {t:<2s}\n")
+    open(os.path.join(tmp_path, "r.pdbqt"), "w").write("".join(lines))
+    open(os.path.join(tmp_path, "r.gpf"), "w").write(
+        "npts 16 16 16\ngridfld r.maps.fld\nspacing 0.375\nreceptor_types A C HD N OA SA\nligand_types A C HD N OA\n"
+        "receptor r.pdbqt\ngridcenter 0.0 0.0 0.0\nsmooth 0.5\nmap r.A.map\nmap r.C.map\nmap r.HD.map\nmap r.N.map\n"
+        "map r.OA.map\nelecmap r.e.map\ndsolvmap r.d.map\ndielectric -0.1465\n")
+    a = fx.load_arrays(str(tmp_path), "r.gpf")
+    _kernel_check(ctx, a)
+
+
+def test_far_grid_has_no_cutoff_pairs(ctx, tmp_path):
+    """A grid more than 8 A from every atom: affinity maps are zero, the
+    electrostatic map is all far field."""
+    run, gpf = fx.colagen2(str(tmp_path))
+    d = os.path.dirname(run)
+    _edit_gpf(d, "colagen2.gpf", d, [("gridcenter 64.0 -16.0 37.0", "gridcenter 300.0 300.0 300.0"),
+                                      ("npts 40 40 40", "npts 10 10 10")])
+    a = fx.load_arrays(run, gpf, v4=True)
+    import fastgrid_amd as fg
+    job = fg.Job(ctx, a)
+    assert job.cutoff_pairs() == 0
+    res = _kernel_check(ctx, a)
+    assert res[fg.PREC_COMPENSATED] < 1e-6
+
+
+def test_sign_symmetry_at_full_size(ctx, tmp_path):
+    """Size-independent property at the 50k-atom / 127^3 size: negating every
+    charge negates the electrostatic map bit for bit (every pair term changes
+    sign exactly) and leaves the affinity and desolvation maps unchanged."""
+    import copy
+    import fastgrid_amd as fg
+    fg.synth_write(str(tmp_path), "L", 3, 50000, "box", (70.0, 70.0, 50.0), (127, 127, 127), "A C HD N NA OA SA S")
+    a = fx.load_arrays(str(tmp_path), "L.gpf")
+    b = copy.copy(a)
+    b.w = -a.w
+    b.charge = -a.charge
+    ja, jb = fg.Job(ctx, a), fg.Job(ctx, b)
+    for prec in (fg.PREC_COMPENSATED, fg.PREC_PLAIN):
+        ja.run(fg.RUN_ALL, prec)
+        oa = ja.fetch()
+        jb.run(fg.RUN_ALL, prec)
+        ob = jb.fetch()
+        M = a.n_maps
+        assert np.array_equal(oa[M], -ob[M]), prec
+        for m in list(range(M)) + [M + 1]:
+            assert np.array_equal(oa[m], ob[m]), (prec, m)
+    ja.close()
+    jb.close()
