@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <numeric>
@@ -22,7 +23,6 @@ using fgk::CutAtom;
 
 namespace {
 
-constexpr int kPZ = 4;  // points per lane along z in k_elec
 
 struct DevBuf {
     void *p = nullptr;
@@ -54,7 +54,10 @@ struct fg_job {
     int N = 0, M = 0, T = 0;
     int distDepDiel = 1, hydrogen = -1;
     bool hasGeom = false;
-    std::vector<fgk::MapParam> mp;
+    std::vector<fgk::MapParam> mp;  // kernel order
+    std::vector<int> order;         // kernel map k computes output map order[k]
+    int KH = 0;                     // maps keeping an H-bond min/max (kernel positions 0..KH-1)
+    bool pointAng = false;          // some atom's directional terms depend on the point
     // cutoff inputs
     CutAtom *dAtoms = nullptr;
     double *dAB = nullptr;
@@ -70,6 +73,7 @@ struct fg_job {
     float *dTileBox = nullptr, *dEps32 = nullptr;
     int nTiles = 0, tableLen = fgk::kTableLen;
     bool clamp = true;
+    int pz = 4;  // grid points per lane along z in k_elec
     float farCut2 = 0, r2Lo = 0, r2Hi = 0, spacing100 = 0;
     // outputs: [M affinity][e][d]
     double *dMaps = nullptr;
@@ -165,18 +169,45 @@ fgk::CutoffArgs cutoffArgs(const fg_job *j, int doElecNear)
     return a;
 }
 
+template <int MM, bool PA>
+void launchCutoffKH(fg_job *j, const fgk::CutoffArgs &a, dim3 grid, dim3 block)
+{
+    hipStream_t s = j->ctx->stream;
+    if (j->KH == 0) hipLaunchKernelGGL((fgk::k_cutoff<MM, 0, true, PA>), grid, block, 0, s, a);
+    else if (j->KH <= 2) hipLaunchKernelGGL((fgk::k_cutoff<MM, 2, true, PA>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((fgk::k_cutoff<MM, 4, true, PA>), grid, block, 0, s, a);
+}
+
+template <int MM>
+void launchCutoffM(fg_job *j, const fgk::CutoffArgs &a, dim3 grid, dim3 block)
+{
+    if (j->pointAng) launchCutoffKH<MM, true>(j, a, grid, block);
+    else launchCutoffKH<MM, false>(j, a, grid, block);
+}
+
 int launchCutoff(fg_job *j, bool affinity, int doElecNear)
 {
     fgk::CutoffArgs a = cutoffArgs(j, doElecNear);
     dim3 grid(unsigned(j->nb)), block(fgk::kCutThreads);
-    hipStream_t s = j->ctx->stream;
-    if (!affinity) hipLaunchKernelGGL((fgk::k_cutoff<1, false>), grid, block, 0, s, a);
-    else if (j->M <= 2) hipLaunchKernelGGL((fgk::k_cutoff<2, true>), grid, block, 0, s, a);
-    else if (j->M <= 4) hipLaunchKernelGGL((fgk::k_cutoff<4, true>), grid, block, 0, s, a);
-    else if (j->M <= 8) hipLaunchKernelGGL((fgk::k_cutoff<8, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((fgk::k_cutoff<16, true>), grid, block, 0, s, a);
+    if (!affinity) hipLaunchKernelGGL((fgk::k_cutoff<1, 0, false, false>), grid, block, 0, j->ctx->stream, a);
+    else if (j->M <= 4) launchCutoffM<4>(j, a, grid, block);
+    else if (j->M <= 8) launchCutoffM<8>(j, a, grid, block);
+    else launchCutoffM<16>(j, a, grid, block);
     FG_HIP(hipGetLastError());
     return FG_OK;
+}
+
+template <int PZ>
+void launchElecPZ(fg_job *j, const fgk::ElecArgs &a, bool comp, size_t lds)
+{
+    dim3 grid(unsigned((j->g.nx + fgk::kElecBX - 1) / fgk::kElecBX), unsigned((j->g.ny + fgk::kElecBY - 1) / fgk::kElecBY),
+              unsigned((j->g.nz + PZ - 1) / PZ));
+    dim3 block(fgk::kElecThreads);
+    hipStream_t s = j->ctx->stream;
+    if (comp && j->clamp) hipLaunchKernelGGL((fgk::k_elec<PZ, true, true>), grid, block, lds, s, a);
+    else if (comp) hipLaunchKernelGGL((fgk::k_elec<PZ, true, false>), grid, block, lds, s, a);
+    else if (j->clamp) hipLaunchKernelGGL((fgk::k_elec<PZ, false, true>), grid, block, lds, s, a);
+    else hipLaunchKernelGGL((fgk::k_elec<PZ, false, false>), grid, block, lds, s, a);
 }
 
 int launchElec(fg_job *j, int precision)
@@ -204,14 +235,10 @@ int launchElec(fg_job *j, int precision)
     a.r2Lo = j->r2Lo;
     a.r2Hi = j->r2Hi;
     a.spacing100 = j->spacing100;
-    dim3 grid(unsigned((j->g.nx + fgk::kElecBX - 1) / fgk::kElecBX), unsigned((j->g.ny + fgk::kElecBY - 1) / fgk::kElecBY),
-              unsigned((j->g.nz + kPZ - 1) / kPZ));
     size_t lds = size_t(j->tableLen) * sizeof(float);
     const bool comp = precision == FG_PREC_COMPENSATED;
-    if (comp && j->clamp) hipLaunchKernelGGL((fgk::k_elec<kPZ, true, true>), grid, dim3(fgk::kElecThreads), lds, s, a);
-    else if (comp) hipLaunchKernelGGL((fgk::k_elec<kPZ, true, false>), grid, dim3(fgk::kElecThreads), lds, s, a);
-    else if (j->clamp) hipLaunchKernelGGL((fgk::k_elec<kPZ, false, true>), grid, dim3(fgk::kElecThreads), lds, s, a);
-    else hipLaunchKernelGGL((fgk::k_elec<kPZ, false, false>), grid, dim3(fgk::kElecThreads), lds, s, a);
+    if (j->pz == 8) launchElecPZ<8>(j, a, comp, lds);
+    else launchElecPZ<4>(j, a, comp, lds);
     FG_HIP(hipGetLastError());
     return FG_OK;
 }
@@ -303,9 +330,16 @@ int buildElecTiles(fg_job *j, const fg_job_desc &d, double gridMaxAbs)
     const double *eps = d.tables.eps;
     // Entries past 8191 repeat entry 8191, so clamping the index to the end of
     // the (possibly extended) table is the reference's min(k, 8191).
-    if (tmax + 2 <= 16384) j->tableLen = std::max(fgk::kTableLen, int(std::ceil(tmax)) + 2);
-    else j->tableLen = fgk::kTableLen;
-    j->clamp = true;
+    // With the table covering every reachable index no clamp is needed: the
+    // FP32 index exceeds 100 r by at most a relative 1e-6 plus ~1e-3 from
+    // coordinate rounding, inside the slack of tmax.
+    if (tmax + 2 <= 16384) {
+        j->tableLen = std::max(fgk::kTableLen, int(std::ceil(tmax)) + 2);
+        j->clamp = false;
+    } else {
+        j->tableLen = fgk::kTableLen;
+        j->clamp = true;
+    }
     eps32.resize(size_t(j->tableLen));
     for (int k = 0; k < j->tableLen; ++k) {
         double v = d.dist_dep_diel ? eps[std::min(k, fgk::kTableLen - 1)] : 1.0;
@@ -333,14 +367,33 @@ int buildCutoff(fg_job *j, const fg_job_desc &d)
         c.w = A.w ? A.w[i] : 0.0;
         if (j->hasGeom) {
             c.qvol = d.solpar_q * A.vol[i];
+            bool zeroVectors = true;
             for (int k = 0; k < 3; ++k) {
                 c.rv[k] = d.geom.rvector[3 * i + k];
                 c.rv2[k] = d.geom.rvector2[3 * i + k];
+                zeroVectors = zeroVectors && c.rv[k] == 0 && c.rv2[k] == 0;
             }
             c.type = A.type[i];
             c.hbond = A.hbond[i];
             c.rexp = d.geom.rexp[i];
-            c.disorder = d.geom.disorder[i];
+            const int dis = d.geom.disorder[i] ? 1 : 0;
+            int kind = fgk::kAngNone;
+            c.ang[0] = c.ang[1] = c.ang[2] = 1.0;
+            if (c.hbond == fgk::HB_D1 || c.hbond == fgk::HB_A1 || c.hbond == fgk::HB_A2) {
+                if (zeroVectors) {
+                    // with zero bond vectors racc/rdon/Hramp do not depend on the
+                    // point: evaluate the reference's formulas once, with the
+                    // host's libm as the reference does
+                    kind = fgk::kAngConst;
+                    const double zero[3] = {0, 0, 0};
+                    fgk::hbondAngular(c.hbond, c.rexp, dis, true, zero, zero, zero, 1.0, 0.0, 0.0, c.ang[0], c.ang[1],
+                                      c.ang[2]);
+                } else {
+                    kind = fgk::kAngPoint;
+                    j->pointAng = true;
+                }
+            }
+            c.flags = dis | (kind << 1);
         }
         atoms[size_t(i)] = c;
         xyz[size_t(3 * i)] = c.x;
@@ -354,9 +407,10 @@ int buildCutoff(fg_job *j, const fg_job_desc &d)
     if (j->hasGeom)
         for (int i = 0; i < N; ++i) {
             const double solQ = A.solpar[i] + d.solpar_q * std::fabs(A.charge[i]);
-            for (int m = 0; m < j->M; ++m) {
-                ab[size_t(i) * 2 * j->M + 2 * m] = d.maps[m].solpar_probe * A.vol[i];
-                ab[size_t(i) * 2 * j->M + 2 * m + 1] = solQ * d.maps[m].vol_probe;
+            for (int k = 0; k < j->M; ++k) {
+                const fg_map_params &mp = d.maps[j->order[size_t(k)]];
+                ab[size_t(i) * 2 * j->M + 2 * k] = mp.solpar_probe * A.vol[i];
+                ab[size_t(i) * 2 * j->M + 2 * k + 1] = solQ * mp.vol_probe;
             }
         }
     if (int rc = devUpload(j, j->dAB, ab.data(), ab.size())) return rc;
@@ -421,7 +475,12 @@ int buildCutoff(fg_job *j, const fg_job_desc &d)
     const size_t pairN = size_t(std::max(d.tables.n_types, 1)) * fgk::kTableLen * size_t(std::max(j->M, 1));
     std::vector<double> zeros;
     if (j->hasGeom && d.tables.pair) {
-        if (int rc = devUpload(j, j->dPair, d.tables.pair, pairN)) return rc;
+        std::vector<double> perm(pairN);
+        const size_t rows = pairN / size_t(std::max(j->M, 1));
+        for (size_t r = 0; r < rows; ++r)
+            for (int k = 0; k < j->M; ++k)
+                perm[r * size_t(j->M) + size_t(k)] = d.tables.pair[r * size_t(j->M) + size_t(j->order[size_t(k)])];
+        if (int rc = devUpload(j, j->dPair, perm.data(), perm.size())) return rc;
     } else {
         zeros.assign(pairN, 0.0);
         if (int rc = devUpload(j, j->dPair, zeros.data(), zeros.size())) return rc;
@@ -532,24 +591,50 @@ int fg_job_create(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
             return fgc::fail(FG_E_ARG, "fg_job_create: hydrogen type index out of range");
         }
     }
-    j->mp.resize(size_t(j->M));
-    int slots = 0;
+    // Per map, what it does with each atom H-bond class (FG/CalculateGridmaps.cpp:206-230);
+    // maps that keep an H-bond min/max go first in kernel order.
+    std::vector<int32_t> actions(size_t(j->M), 0);
+    std::vector<bool> keeps(size_t(j->M), false);
     for (int m = 0; m < j->M; ++m) {
         const fg_map_params &p = d->maps[m];
-        // maps whose probe keeps an H-bond min/max (FG/CalculateGridmaps.cpp:217-223)
-        const bool keeps = p.is_hbonder && !p.is_covalent && (p.hbond == FG_HB_A1 || p.hbond == FG_HB_DS || p.hbond == FG_HB_D1);
-        j->mp[size_t(m)] = fgk::MapParam{p.hbond, p.is_hbonder, p.is_covalent, keeps ? slots : -1, p.solpar_probe,
-                                         p.vol_probe};
-        if (keeps) ++slots;
+        int32_t acts = 0;
+        for (int h = 0; h < 6; ++h) {
+            int act = fgk::kActAdd;
+            const bool aDon = h == fgk::HB_DS || h == fgk::HB_D1;
+            if (p.is_hbonder) {
+                if ((p.hbond == fgk::HB_AS || p.hbond == fgk::HB_A2) && aDon) act = fgk::kActDonor;
+                else if (p.hbond == fgk::HB_A1 && aDon) act = fgk::kActMinRacc;
+                else if ((p.hbond == fgk::HB_DS || p.hbond == fgk::HB_D1) && h >= fgk::HB_AS) act = fgk::kActMinRdon;
+            }
+            acts |= act << (2 * h);
+            if (act == fgk::kActMinRacc || act == fgk::kActMinRdon) keeps[size_t(m)] = !p.is_covalent;
+        }
+        actions[size_t(m)] = acts;
     }
-    if (slots > fgk::kMaxHbSlots) {
+    for (int pass = 0; pass < 2; ++pass)
+        for (int m = 0; m < j->M; ++m)
+            if (keeps[size_t(m)] == (pass == 0)) j->order.push_back(m);
+    j->KH = int(std::count(keeps.begin(), keeps.end(), true));
+    if (j->KH > fgk::kMaxHbSlots) {
         destroyJob(j);
         return fgc::fail(FG_E_ARG, "fg_job_create: at most 4 maps of donor (DS/D1) or A1 probes are supported");
+    }
+    j->mp.resize(size_t(j->M));
+    for (int k = 0; k < j->M; ++k) {
+        const int m = j->order[size_t(k)];
+        j->mp[size_t(k)] = fgk::MapParam{actions[size_t(m)], d->maps[m].is_covalent, m, 0};
     }
     double gridMaxAbs = 0;
     const int n[3] = {g.nx, g.ny, g.nz}, dv[3] = {g.div2x, g.div2y, g.div2z};
     for (int k = 0; k < 3; ++k)
         gridMaxAbs = std::max({gridMaxAbs, std::fabs(double(0 - dv[k]) * g.spacing), std::fabs(double(n[k] - 1 - dv[k]) * g.spacing)});
+    // points per lane along z: 8 halves the per-atom overhead but needs enough
+    // z-bricks to fill the GPU; FG_ELEC_PZ overrides (tuning)
+    {
+        const long bricks8 = long((g.nx + 15) / 16) * ((g.ny + 15) / 16) * ((g.nz + 7) / 8);
+        j->pz = bricks8 >= 1024 ? 8 : 4;
+        if (const char *e = std::getenv("FG_ELEC_PZ")) j->pz = std::atoi(e) == 8 ? 8 : 4;
+    }
     int rc = buildCutoff(j, *d);
     if (!rc) rc = buildElecTiles(j, *d, gridMaxAbs);
     if (!rc) rc = devAlloc(j, j->dMaps, size_t(j->M + 2) * size_t(j->P));

@@ -16,7 +16,7 @@ namespace fgk {
 __device__ __forceinline__ double dmin_(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double dmax_(double a, double b) { return a > b ? a : b; }
 // Math::Acos with the reference's clamp (FG/math/Math.h:51)
-__device__ __forceinline__ double acosClamped(double f)
+__host__ __device__ __forceinline__ double acosClamped(double f)
 {
     return f > 1 ? 0.0 : (f < -1 ? 3.14159265358979323846 : acos(f));
 }
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const int32_t *counts, int
 
 // racc / rdon / Hramp (FG/CalculateGridmaps.cpp:84-197).  hb, rexp and
 // disorder are wave-uniform.
-__device__ __forceinline__ void hbondAngular(int hb, int rexp, int disorder, bool isClosest,
+__host__ __device__ __forceinline__ void hbondAngular(int hb, int rexp, int disorder, bool isClosest,
                                              const double rv[3], const double rv2[3], const double cHrv[3],
                                              double ux, double uy, double uz,
                                              double &racc, double &rdon, double &Hramp)
@@ -285,7 +285,10 @@ __device__ __forceinline__ int binGuarded(double r, bool &edge)
 
 constexpr int kChunk = 32;  // candidate atoms staged in LDS per round
 
-template <int MM, bool kAffinity>
+// kPointAng: the job has atoms whose directional terms depend on the point
+// (bonded donors/acceptors); without them the FP64 transcendental code and its
+// registers are left out.
+template <int MM, int KH, bool kAffinity, bool kPointAng>
 __global__ __launch_bounds__(kCutThreads) void k_cutoff(CutoffArgs a)
 {
     __shared__ CutAtom sAtom[kChunk];
@@ -332,16 +335,16 @@ __global__ __launch_bounds__(kCutThreads) void k_cutoff(CutoffArgs a)
         cHrv[2] = a.atoms[cH].rv[2];
     }
 
-    double e[MM];
-    double hmin[kMaxHbSlots], hmax[kMaxHbSlots];
+    double e[MM], hmin[KH > 0 ? KH : 1], hmax[KH > 0 ? KH : 1];
     unsigned flags = 0;
 #pragma unroll
     for (int m = 0; m < MM; ++m)
-        e[m] = (kAffinity && m < M && a.covalent && a.mp[m].isCovalent) ? a.covalent[int64_t(m) * a.P + p] : 0.0;
+        e[m] = (kAffinity && m < M && a.covalent && a.mp[m].isCovalent) ? a.covalent[int64_t(a.mp[m].out) * a.P + p]
+                                                                        : 0.0;
 #pragma unroll
-    for (int s = 0; s < kMaxHbSlots; ++s) {
-        hmin[s] = 999999;
-        hmax[s] = -999999;
+    for (int k = 0; k < (KH > 0 ? KH : 1); ++k) {
+        hmin[k] = 999999;
+        hmax[k] = -999999;
     }
     double dmap = 0, enear = 0;
 
@@ -388,59 +391,55 @@ __global__ __launch_bounds__(kCutThreads) void k_cutoff(CutoffArgs a)
             }
             if (a.doElecNear) enear += A.w * dmin_(yinv, 2) * a.eps[idxE];
             if (!kAffinity) continue;
-            const int type = A.type, ahb = A.hbond, dis = A.disorder;
+            // atom fields are the same in every lane: keep them in scalar registers
+            const int type = uniform(A.type), ahb = uniform(A.hbond), fl = uniform(A.flags);
+            const int dis = fl & kFlagDisorder, kind = fl >> 1;
             if (type == a.hydrogen && dis) continue;
             double racc = 1, rdon = 1, Hramp = 1;
-            if (ahb == HB_D1 || ahb == HB_A1 || ahb == HB_A2) {
+            if (kind == kAngConst) {
+                racc = A.ang[0];
+                rdon = A.ang[1];
+                Hramp = A.ang[2];
+            } else if (kPointAng && kind == kAngPoint) {
                 // directional terms use the unit vector exactly as the reference forms it
                 const double invR = 1 / sqrt(rSq);
                 const double rv[3] = {A.rv[0], A.rv[1], A.rv[2]};
                 const double rv2[3] = {A.rv2[0], A.rv2[1], A.rv2[2]};
-                hbondAngular(ahb, A.rexp, dis, uniform(sIdx[c]) == cH, rv, rv2, cHrv, dx * invR, dy * invR, dz * invR,
-                             racc, rdon, Hramp);
+                hbondAngular(ahb, uniform(A.rexp), dis, uniform(sIdx[c]) == cH, rv, rv2, cHrv, dx * invR,
+                             dy * invR, dz * invR, racc, rdon, Hramp);
             }
-            const bool aDon = ahb == HB_DS || ahb == HB_D1;
             const double S = a.desolv[idx];
             const double *row = a.pair + (int64_t(type) * kTableLen + idx) * M;
             const double *rowH = (dis && a.hydrogen >= 0)
                                      ? a.pair + (int64_t(a.hydrogen) * kTableLen + (idx - 110 > 0 ? idx - 110 : 0)) * M
                                      : row;
             const double *ab = &sAB[c * MM * 2];
+            const int shift = 2 * ahb;
 #pragma unroll
             for (int m = 0; m < MM; ++m) {
                 if (m >= M || a.mp[m].isCovalent) continue;
+                const int act = (a.mp[m].actions >> shift) & 3;
                 const double E = row[m];
-                const int mhb = a.mp[m].hbond;
-                double ins = 0;
-                bool doIns = false;
-                if (a.mp[m].isHBonder) {
-                    if ((mhb == HB_AS || mhb == HB_A2) && aDon) {
-                        double rsph = dmin_(dmax_(E / 100, 0), 1);
-                        e[m] += rowH[m] * Hramp * (racc + (1 - racc) * rsph);
-                    } else if (mhb == HB_A1 && aDon) {
-                        double rsph = dmin_(dmax_(E / 100, 0), 1);
-                        ins = E * (racc + (1 - racc) * rsph);
-                        doIns = true;
-                    } else if ((mhb == HB_DS || mhb == HB_D1) && ahb >= HB_AS) {
-                        double rsph = dmin_(dmax_(E / 100, 0), 1);
-                        ins = E * (rdon + (1 - rdon) * rsph);
-                        doIns = true;
+                // one addend per map and a single update of e[m], so the
+                // accumulators never sit in divergent branches (a map that
+                // keeps a min/max adds nothing here: e + 0 == e)
+                double add = E;
+                if (act != kActAdd) {
+                    const double rsph = dmin_(dmax_(E / 100, 0), 1);
+                    if (act == kActDonor) {
+                        add = rowH[m] * Hramp * (racc + (1 - racc) * rsph);
                     } else {
-                        e[m] += E;
-                    }
-                } else {
-                    e[m] += E;
-                }
-                if (doIns) {
-                    const int s = a.mp[m].hbSlot;  // wave-uniform
-#pragma unroll
-                    for (int k = 0; k < kMaxHbSlots; ++k)
-                        if (s == k) {
-                            hmin[k] = dmin_(hmin[k], ins);
-                            hmax[k] = dmax_(hmax[k], ins);
+                        add = 0.0;
+                        if (m < KH) {
+                            const double f = act == kActMinRacc ? racc : rdon;
+                            const double v = E * (f + (1 - f) * rsph);
+                            hmin[m < KH ? m : 0] = dmin_(hmin[m < KH ? m : 0], v);
+                            hmax[m < KH ? m : 0] = dmax_(hmax[m < KH ? m : 0], v);
+                            flags |= 1u << m;
                         }
-                    flags |= 1u << m;
+                    }
                 }
+                e[m] += add;
                 e[m] += ab[2 * m] * S + ab[2 * m + 1] * S;
             }
             dmap += A.qvol * S;
@@ -452,18 +451,8 @@ __global__ __launch_bounds__(kCutThreads) void k_cutoff(CutoffArgs a)
         for (int m = 0; m < MM; ++m) {
             if (m >= M) continue;
             double v = e[m];
-            if (flags & (1u << m)) {
-                const int s = a.mp[m].hbSlot;
-                double mn = 0, mx = 0;
-#pragma unroll
-                for (int k = 0; k < kMaxHbSlots; ++k)
-                    if (s == k) {
-                        mn = hmin[k];
-                        mx = hmax[k];
-                    }
-                v += mn + mx;
-            }
-            a.maps[int64_t(m) * a.P + p] = v;
+            if (m < KH && (flags & (1u << m))) v += hmin[m < KH ? m : 0] + hmax[m < KH ? m : 0];
+            a.maps[int64_t(a.mp[m].out) * a.P + p] = v;
         }
         a.maps[int64_t(M + 1) * a.P + p] = dmap;
     }

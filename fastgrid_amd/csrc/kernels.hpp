@@ -45,20 +45,36 @@ struct alignas(16) CutAtom {
     double qvol;         // solparQ * vol            (FG/CalculateGridmaps.cpp:302)
     double rv[3];        // bond vector
     double rv2[3];       // second bond vector
-    double pad[3];
-    int32_t type, hbond, rexp, disorder;
+    double ang[3];       // racc, rdon, Hramp when they do not depend on the point
+    int32_t type, hbond, rexp, flags;  // flags: kFlagDisorder | angular kind << 1
 };
 static_assert(sizeof(CutAtom) == 128, "CutAtom must stay 128 bytes");
+constexpr int kFlagDisorder = 1;
+// Angular kind of an atom (FG/CalculateGridmaps.cpp:84-197):
+//   0  no directional term (racc = rdon = Hramp = 1),
+//   1  point-independent (unbonded D1/A1/A2: zero bond vectors), values in ang[],
+//   2  evaluated per point.
+constexpr int kAngNone = 0, kAngConst = 1, kAngPoint = 2;
 // Per atom and map, the two desolvation products of
 // FG/CalculateGridmaps.cpp:234-235, formed exactly as the reference forms them:
 // ab[2m] = solparProbe_m * vol_a, ab[2m+1] = (solpar_a + solparQ*|q_a|) * volProbe_m.
 
-// Per affinity map, uniform across the kernel.
-struct MapParam {
-    int32_t hbond, isHBonder, isCovalent, hbSlot;  // hbSlot: min/max slot or -1
-    double solparProbe, volProbe;
-};
+// What map m does with an atom of H-bond class h (FG/CalculateGridmaps.cpp:206-230):
+//   kActAdd     e += E
+//   kActDonor   e += E' * Hramp * (racc + (1 - racc) * rsph)   (acceptor probe, donor atom)
+//   kActMinRacc keep min/max of E * (racc + (1 - racc) * rsph)  (A1 probe, donor atom)
+//   kActMinRdon keep min/max of E * (rdon + (1 - rdon) * rsph)  (donor probe, acceptor atom)
+constexpr int kActAdd = 0, kActDonor = 1, kActMinRacc = 2, kActMinRdon = 3;
 constexpr int kMaxHbSlots = 4;  // maps that keep an H-bond min/max per point
+
+// Per affinity map, uniform across the kernel, in kernel order: maps that keep
+// an H-bond min/max come first.
+struct MapParam {
+    int32_t actions;  // 2 bits per atom H-bond class (6 classes)
+    int32_t isCovalent;
+    int32_t out;      // index of the output map
+    int32_t pad;
+};
 
 struct GridGeom {
     int32_t nx, ny, nz, dx2, dy2, dz2;
@@ -81,7 +97,7 @@ struct CutoffArgs {
     const double *ab;                         // [N][2M] desolvation products
     const int32_t *brickStart, *brickList;    // all candidates
     const int32_t *hStart, *hList;            // DS/D1 candidates
-    const double *pair;    // [T][8192][M]
+    const double *pair;    // [T][8192][M], maps in kernel order
     const double *desolv;  // [8192]
     const double *eps;     // [8192]
     double *maps;          // [M+2][P]: affinity maps, e (near part), d
