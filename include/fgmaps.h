@@ -213,7 +213,7 @@ int fg_prep_summary(const fg_prep *prep, int m, double *min, double *max);
 
 /* .fld + .map files (FG/GridMap.cpp:127-212, FG/Utils.cpp:40-84).  maps[m]
  * for m < n_maps, maps[n_maps] = e, maps[n_maps+1] = d, unrounded.  Also
- * writes the min/max summary into the prep for fg_write_log_summary. */
+ * records each map's min/max in the prep for fg_prep_summary. */
 int fg_write_outputs(fg_prep *prep, double *const *maps, const char *program_name, int threads);
 
 /* ---------------------------------------------------------------- */

@@ -80,3 +80,20 @@ def load_arrays(run_dir, gpf, **kw):
         return fg.Prepared(gpf, **kw).arrays()
     finally:
         os.chdir(cwd)
+
+
+def variant(tmp, name):
+    """The colagen2 receptor with one of the GPF variants of
+    tests/golden/variants/variants.json; returns (run_dir, gpf, names, args)."""
+    meta = json.load(open(os.path.join(GOLDEN, "variants", "variants.json")))[name]
+    run, gpf = colagen2(tmp)
+    path = os.path.join(tmp, "colagen2.gpf")
+    text = open(path).read()
+    for a, b in meta["edits"]:
+        assert a in text, a
+        text = text.replace(a, b)
+    open(path, "w").write(text + "".join(l + "\n" for l in meta["extra"]))
+    return run, gpf, meta["names"], meta["args"]
+
+
+VARIANTS = sorted(json.load(open(os.path.join(GOLDEN, "variants", "variants.json"))))

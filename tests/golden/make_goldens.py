@@ -76,7 +76,7 @@ def colagen2():
     os.makedirs(out, exist_ok=True)
     shutil.copy(os.path.join(REF, "tests", "colagen2.gpf.orig"), os.path.join(out, "colagen2.gpf"))
     with open(os.path.join(REF, "tests", "macromol_1t60_half_prot.pdbqt.orig"), "rb") as f, \
-            gzip.open(os.path.join(out, "macromol_1t60_half_prot.pdbqt.gz"), "wb", compresslevel=9) as g:
+            gzip.GzipFile(os.path.join(out, "macromol_1t60_half_prot.pdbqt.gz"), "wb", compresslevel=9, mtime=0) as g:
         g.write(f.read())
     names = ["C", "HD", "OA", "e", "d"]
     files = [os.path.join(REF, "tests", "original_win32", f"colagen2.{n}.map") for n in names]
@@ -99,6 +99,45 @@ def colagen2():
     json.dump(tables, open(os.path.join(out, "log_tables.json"), "w"))
 
 
+# GPF variants of the colagen2 receptor, each run through the reference binary
+VARIANTS = {
+    "constant_dielectric": ([("dielectric -0.1465", "dielectric 40"), ("npts 40 40 40", "npts 20 20 20")], [], ["--v4"]),
+    "disorder_h": ([("npts 40 40 40", "npts 24 24 24")], ["disorder_h"], ["--v4"]),
+    "covalent_qasp": ([("npts 40 40 40", "npts 16 18 20"), ("spacing 1 ", "spacing 0.375 "),
+                       ("ligand_types  C HD OA", "ligand_types  C HD OA Z"),
+                       ("map colagen2.OA.map", "map colagen2.OA.map\nmap colagen2.Z.map\ncovalentmap 1.7 1000.0 64.5 -16.0 37.2")],
+                      ["qasp 0.01234"], ["--v4"]),
+    "odd_grid_ad41": ([("npts 40 40 40", "npts 21 34 9"), ("spacing 1 ", "spacing 0.4 ")], [], []),
+}
+
+
+def variant_gpf(text, edits, extra):
+    for a, b in edits:
+        assert a in text, a
+        text = text.replace(a, b)
+    return text + "".join(l + "\n" for l in extra)
+
+
+def variants():
+    out = os.path.join(HERE, "variants")
+    os.makedirs(out, exist_ok=True)
+    base = open(os.path.join(REF, "tests", "colagen2.gpf.orig")).read()
+    meta = {}
+    for name, (edits, extra, args) in VARIANTS.items():
+        with tempfile.TemporaryDirectory() as d:
+            shutil.copy(os.path.join(REF, "tests", "macromol_1t60_half_prot.pdbqt.orig"),
+                        os.path.join(d, "macromol_1t60_half_prot.pdbqt"))
+            run = os.path.join(d, "run")
+            os.makedirs(run)
+            text = variant_gpf(base, edits, extra)
+            open(os.path.join(d, "colagen2.gpf"), "w").write(text)
+            run_ref("../colagen2.gpf", run, args)
+            names = [l.split()[1].split(".")[1] for l in text.split("\n") if l.startswith("map ")] + ["e", "d"]
+            save_maps(os.path.join(out, f"{name}.npz"), [os.path.join(run, f"colagen2.{n}.map") for n in names], names)
+            meta[name] = {"args": args, "names": names, "edits": edits, "extra": extra}
+    json.dump(meta, open(os.path.join(out, "variants.json"), "w"), indent=1)
+
+
 def pgp():
     out = os.path.join(HERE, "pgp")
     os.makedirs(out, exist_ok=True)
@@ -106,7 +145,7 @@ def pgp():
     shutil.copy(os.path.join(t, "1pgp.gpf"), out)
     shutil.copy(os.path.join(t, "AD4_parameters.dat"), out)
     with open(os.path.join(t, "1pgp_rec.pdbqt"), "rb") as f, \
-            gzip.open(os.path.join(out, "1pgp_rec.pdbqt.gz"), "wb", compresslevel=9) as g:
+            gzip.GzipFile(os.path.join(out, "1pgp_rec.pdbqt.gz"), "wb", compresslevel=9, mtime=0) as g:
         g.write(f.read())
     names = ["C", "HD", "OA", "P", "e", "d"]
     with tempfile.TemporaryDirectory() as d:
@@ -142,6 +181,7 @@ if __name__ == "__main__":
     if not os.path.exists(REF_BIN):
         sys.exit("build the reference first: make -f oracle/ref.mk")
     colagen2()
+    variants()
     pgp()
     synth()
     print("goldens written under", HERE)

@@ -62,6 +62,13 @@ def test_cli_pgp_fastgrid_and_autogrid_order(tmp_path):
         assert np.abs(got - (fast[n] + delta[n])).max() <= (1 if n == "e" else 0), n
 
 
+@pytest.mark.parametrize("name", fx.VARIANTS)
+def test_cli_gpf_variant_goldens(tmp_path, name):
+    run, gpf, names, args = fx.variant(str(tmp_path), name)
+    fastgrid4(run, gpf, *args)
+    compare_files(run, "colagen2", fx.goldens("variants", f"{name}.npz"))
+
+
 @pytest.mark.parametrize("name", ["micro", "S"])
 def test_cli_synthetic_goldens(tmp_path, name):
     run, gpf, names = fx.synth(str(tmp_path), name)

@@ -101,3 +101,14 @@ def test_pair_tables_match_golden_log(tmp_path):
                     assert abs(round(v, 2) - printed) < 1e-6, (m, t, j, v, printed)
                 else:
                     assert printed == 100000.0
+
+
+@pytest.mark.parametrize("name", fx.VARIANTS)
+def test_gpf_variants_match_reference_binary(tmp_path, name):
+    """Constant dielectric, disorder_h, covalent map + qasp, odd grid with
+    AD4.1 defaults: the reference binary's maps."""
+    run, gpf, names, args = fx.variant(str(tmp_path), name)
+    got = run_oracle(run, gpf, names, "colagen2", v4="--v4" in args)
+    gold = fx.goldens("variants", f"{name}.npz")
+    for n in names:
+        assert np.array_equal(got[n], gold[n]), n
