@@ -5,8 +5,9 @@ gfx950 HIP kernels) and the fastgrid4 command line built beside it.  This
 package is its Python binding; see fastgrid_amd.api.
 """
 from .api import (  # noqa: F401
-    LIB, LIB_PATH, PREC_COMPENSATED, PREC_FP64, PREC_PLAIN, RUN_ALL, RUN_AFFINITY, RUN_ELEC, SIGNATURES,
-    Context, FgError, Job, JobArrays, Prepared, batch_run, device_count, round3, synth_write,
+    BATCH_NO_GRAPH, LIB, LIB_PATH, PREC_COMPENSATED, PREC_FP32, PREC_FP64, PREC_PLAIN, RUN_ALL, RUN_AFFINITY, RUN_ELEC,
+    SIGNATURES, Batch, Context, FgError, Job, JobArrays, Prepared, autotune, batch_run, device_count, round3,
+    synth_write,
 )
 
 BIN_DIR = __import__("os").path.join(__import__("os").path.dirname(__file__), "bin")

@@ -16,8 +16,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfgmaps.so")
 
 FG_OK = 0
-PREC_PLAIN, PREC_COMPENSATED, PREC_FP64 = 0, 1, 2
+PREC_PLAIN, PREC_COMPENSATED, PREC_FP64, PREC_FP32 = 0, 1, 2, 3
 RUN_ELEC, RUN_AFFINITY, RUN_ALL = 1, 2, 3
+BATCH_NO_GRAPH = 1
 TABLE_LEN = 8192
 
 _dp = C.POINTER(C.c_double)
@@ -58,6 +59,10 @@ class FgTiming(C.Structure):
     _fields_ = [("elec_ms", C.c_double), ("affinity_ms", C.c_double), ("total_ms", C.c_double)]
 
 
+class FgShape(C.Structure):
+    _fields_ = [("pz", C.c_int32), ("tile", C.c_int32), ("bx", C.c_int32)]
+
+
 class FgLoadOpts(C.Structure):
     _fields_ = [("v4", C.c_int32), ("ag421_param_order", C.c_int32)]
 
@@ -86,6 +91,14 @@ SIGNATURES = {
     "fg_sync": (C.c_int, [C.c_void_p]),
     "fg_job_fetch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_dp)]),
     "fg_job_cutoff_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]),
+    "fg_job_get_shape": (C.c_int, [C.c_void_p, C.POINTER(FgShape)]),
+    "fg_job_set_shape": (C.c_int, [C.c_void_p, C.POINTER(FgShape)]),
+    "fg_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, C.c_int]),
+    "fg_tune_report": (C.c_char_p, [C.c_void_p]),
+    "fg_batch_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, C.c_uint, C.c_int, C.c_uint,
+                                  C.POINTER(C.c_void_p)]),
+    "fg_batch_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fg_batch_destroy": (None, [C.c_void_p]),
     "fg_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, C.c_uint, C.c_int]),
     "fg_last_timing": (C.c_int, [C.c_void_p, C.POINTER(FgTiming)]),
     "fg_prep_load": (C.c_int, [C.c_char_p, C.POINTER(FgLoadOpts), C.POINTER(C.c_void_p)]),
@@ -380,10 +393,57 @@ class Job:
         _check(LIB.fg_job_cutoff_pairs(self.ctx.handle, self._h, C.byref(v)))
         return int(v.value)
 
+    @property
+    def shape(self) -> tuple[int, int, int]:
+        """(points per lane along z, atoms per tile, brick lanes along x)."""
+        sh = FgShape()
+        _check(LIB.fg_job_get_shape(self._h, C.byref(sh)))
+        return sh.pz, sh.tile, sh.bx
+
+    @shape.setter
+    def shape(self, v: tuple[int, int, int]) -> None:
+        sh = FgShape(*[int(x) for x in v])
+        _check(LIB.fg_job_set_shape(self._h, C.byref(sh)))
+
+
+def _handles(jobs: list[Job]):
+    return (C.c_void_p * len(jobs))(*[j.handle for j in jobs])
+
+
+def autotune(ctx: Context, jobs: list[Job], precision: int = PREC_COMPENSATED) -> str:
+    """Tunes the electrostatic kernel shape of the jobs; returns the report."""
+    _check(LIB.fg_autotune(ctx.handle, _handles(jobs), len(jobs), precision))
+    return (LIB.fg_tune_report(ctx.handle) or b"").decode()
+
+
+class Batch:
+    """Many jobs run as one hipGraph of persistent kernels (fg_batch_*)."""
+
+    def __init__(self, ctx: Context, jobs: list[Job], what: int = RUN_ALL, precision: int = PREC_COMPENSATED,
+                 graph: bool = True):
+        self.ctx = ctx
+        self.jobs = list(jobs)
+        self._h = C.c_void_p()
+        _check(LIB.fg_batch_create(ctx.handle, _handles(self.jobs), len(self.jobs), what, precision,
+                                   0 if graph else BATCH_NO_GRAPH, C.byref(self._h)))
+
+    def launch(self) -> None:
+        _check(LIB.fg_batch_launch(self.ctx.handle, self._h))
+
+    def close(self) -> None:
+        if self._h:
+            LIB.fg_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 def batch_run(ctx: Context, jobs: list[Job], what: int = RUN_ALL, precision: int = PREC_COMPENSATED) -> None:
-    arr = (C.c_void_p * len(jobs))(*[j.handle for j in jobs])
-    _check(LIB.fg_batch_run(ctx.handle, arr, len(jobs), what, precision))
+    _check(LIB.fg_batch_run(ctx.handle, _handles(jobs), len(jobs), what, precision))
 
 
 def synth_write(directory: str, stem: str, seed: int, n_atoms: int, shape: str, dims, npts, ligand_types: str,

@@ -8,6 +8,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
@@ -18,6 +20,7 @@
 #include "capi_common.hpp"
 #include "fgmaps.h"
 #include "kernels.hip"
+#include "kernels_plain.hip"
 
 using fgk::CutAtom;
 
@@ -39,12 +42,12 @@ struct DevBuf {
 }  // namespace
 
 struct fg_ctx {
-    int device = 0;
+    int device = 0, numCUs = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    fg_timing last{0, 0, 0};
     bool haveTiming = false;
-    bool cutoffRan = false, elecRan = false;
+    bool splitTiming = true;  // ev[1] separates the cutoff and electrostatic phases
+    std::string tuneReport;
 };
 
 struct fg_job {
@@ -70,11 +73,19 @@ struct fg_job {
     // electrostatic tiles
     float4 *dTiles = nullptr;
     double *dTiles64 = nullptr;
-    float *dTileBox = nullptr, *dEps32 = nullptr;
-    int nTiles = 0, tableLen = fgk::kTableLen;
+    float *dSubBox = nullptr, *dEps32 = nullptr;
+    int nAtoms32 = 0, tableLen = fgk::kTableLen;
     bool clamp = true;
-    int pz = 4;  // grid points per lane along z in k_elec
+    // k_elec shape: points per lane along z, atoms per tile, lanes along x
+    // (defaults; the autotuner may change them)
+    int pz = 4, tile = 64, bxShift = 4;
     float farCut2 = 0, r2Lo = 0, r2Hi = 0, spacing100 = 0;
+    // plain form (FP32 throughout)
+    float4 *dPlainQ = nullptr;          // (x, y, z, w) per atom, index order
+    fgk::PlainAtom *dPlainAtoms = nullptr;
+    float *dPair32 = nullptr, *dDesolv32 = nullptr, *dAcc32 = nullptr;
+    fgk::PlainMap pmp[16] = {};
+    float solparQ = 0;
     // outputs: [M affinity][e][d]
     double *dMaps = nullptr;
     std::vector<void *> owned;
@@ -197,11 +208,36 @@ int launchCutoff(fg_job *j, bool affinity, int doElecNear)
     return FG_OK;
 }
 
+fgk::ElecArgs elecArgs(const fg_job *j, int pz, int tile, int bxShift)
+{
+    fgk::ElecArgs a{};
+    a.g = j->g;
+    a.nAtoms = j->nAtoms32;
+    a.tableLen = j->tableLen;
+    a.tile = tile;
+    a.bxShift = bxShift;
+    const int bxl = 1 << bxShift, byl = fgk::kElecThreads >> bxShift;
+    a.nbx = (j->g.nx + bxl - 1) / bxl;
+    a.nby = (j->g.ny + byl - 1) / byl;
+    a.nbz = (j->g.nz + pz - 1) / pz;
+    a.clampMax = j->tableLen - 1;
+    a.P = j->P;
+    a.tiles = j->dTiles;
+    a.tiles64 = j->dTiles64;
+    a.subBox = j->dSubBox;
+    a.eps32 = j->dEps32;
+    a.e = j->dMaps + int64_t(j->M) * j->P;
+    a.farCut2 = j->farCut2;
+    a.r2Lo = j->r2Lo;
+    a.r2Hi = j->r2Hi;
+    a.spacing100 = j->spacing100;
+    return a;
+}
+
 template <int PZ>
 void launchElecPZ(fg_job *j, const fgk::ElecArgs &a, bool comp, size_t lds)
 {
-    dim3 grid(unsigned((j->g.nx + fgk::kElecBX - 1) / fgk::kElecBX), unsigned((j->g.ny + fgk::kElecBY - 1) / fgk::kElecBY),
-              unsigned((j->g.nz + PZ - 1) / PZ));
+    dim3 grid(unsigned(a.nbx * a.nby * a.nbz));
     dim3 block(fgk::kElecThreads);
     hipStream_t s = j->ctx->stream;
     if (comp && j->clamp) hipLaunchKernelGGL((fgk::k_elec<PZ, true, true>), grid, block, lds, s, a);
@@ -220,21 +256,7 @@ int launchElec(fg_job *j, int precision)
         FG_HIP(hipGetLastError());
         return FG_OK;
     }
-    fgk::ElecArgs a{};
-    a.g = j->g;
-    a.nTiles = j->nTiles;
-    a.tableLen = j->tableLen;
-    a.mode = precision;
-    a.P = j->P;
-    a.tiles = j->dTiles;
-    a.tiles64 = j->dTiles64;
-    a.tileBox = j->dTileBox;
-    a.eps32 = j->dEps32;
-    a.e = e;
-    a.farCut2 = j->farCut2;
-    a.r2Lo = j->r2Lo;
-    a.r2Hi = j->r2Hi;
-    a.spacing100 = j->spacing100;
+    const fgk::ElecArgs a = elecArgs(j, j->pz, j->tile, j->bxShift);
     size_t lds = size_t(j->tableLen) * sizeof(float);
     const bool comp = precision == FG_PREC_COMPENSATED;
     if (j->pz == 8) launchElecPZ<8>(j, a, comp, lds);
@@ -243,13 +265,14 @@ int launchElec(fg_job *j, int precision)
     return FG_OK;
 }
 
-// Electrostatic far-field tiles: atoms in Morton order of 2 A cells, kTile
-// per tile, coordinates and w scaled by 100 (FP32), plus FP64 coordinates
-// for the exact near/far predicate and outward-rounded tile boxes.  Also the
-// guard band of the near/far predicate (see ElecArgs).
+// Electrostatic far-field tiles: atoms in Morton order of 2 A cells, padded to
+// a multiple of kSubTile, coordinates and w scaled by 100 (FP32), plus FP64
+// coordinates for the exact near/far predicate and outward-rounded boxes of
+// every kSubTile atoms.  Also the guard band of the near/far predicate (see
+// ElecArgs).
 int buildElecTiles(fg_job *j, const fg_job_desc &d, double gridMaxAbs)
 {
-    const int N = j->N, TT = fgk::kTile;
+    const int N = j->N, TT = fgk::kSubTile;
     const fg_atoms &A = d.atoms;
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int i = 0; i < N; ++i) {
@@ -267,11 +290,12 @@ int buildElecTiles(fg_job *j, const fg_job_desc &d, double gridMaxAbs)
         order[size_t(i)] = {spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2), i};
     }
     std::stable_sort(order.begin(), order.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
-    j->nTiles = (N + TT - 1) / TT;
-    const size_t NT = size_t(j->nTiles) * TT;
-    std::vector<float4> tiles(NT);
-    std::vector<double> t64(NT * 3);
-    std::vector<float> box(size_t(j->nTiles) * 6);
+    const int nSub = (N + TT - 1) / TT;
+    j->nAtoms32 = nSub * TT;
+    const size_t NT = size_t(j->nAtoms32);
+    std::vector<float4> tiles(std::max<size_t>(NT, 1));
+    std::vector<double> t64(std::max<size_t>(NT * 3, 1));
+    std::vector<float> box(std::max<size_t>(size_t(nSub) * 6, 1));
     const double s = j->g.spacing;
     for (size_t q = 0; q < NT; ++q) {
         double c[3], w;
@@ -292,7 +316,7 @@ int buildElecTiles(fg_job *j, const fg_job_desc &d, double gridMaxAbs)
         tiles[q] = make_float4(float(c[0] * 100.0), float(c[1] * 100.0), float(c[2] * 100.0), float(w * 100.0));
         for (int k = 0; k < 3; ++k) t64[3 * q + k] = c[k];
     }
-    for (int t = 0; t < j->nTiles; ++t) {
+    for (int t = 0; t < nSub; ++t) {
         float bl[3] = {1e30f, 1e30f, 1e30f}, bh[3] = {-1e30f, -1e30f, -1e30f};
         for (int q = t * TT; q < (t + 1) * TT; ++q) {
             const float v[3] = {tiles[size_t(q)].x, tiles[size_t(q)].y, tiles[size_t(q)].z};
@@ -347,7 +371,7 @@ int buildElecTiles(fg_job *j, const fg_job_desc &d, double gridMaxAbs)
     }
     if (int rc = devUpload(j, j->dTiles, tiles.data(), tiles.size())) return rc;
     if (int rc = devUpload(j, j->dTiles64, t64.data(), t64.size())) return rc;
-    if (int rc = devUpload(j, j->dTileBox, box.data(), box.size())) return rc;
+    if (int rc = devUpload(j, j->dSubBox, box.data(), box.size())) return rc;
     if (int rc = devUpload(j, j->dEps32, eps32.data(), eps32.size())) return rc;
     return FG_OK;
 }
@@ -497,11 +521,445 @@ int buildCutoff(fg_job *j, const fg_job_desc &d)
     return FG_OK;
 }
 
+// FP32 inputs of the plain form: atoms in index order, tables in output map order.
+int buildPlain(fg_job *j, const fg_job_desc &d)
+{
+    const int N = j->N;
+    const fg_atoms &A = d.atoms;
+    std::vector<float4> q(size_t(std::max(N, 1)));
+    std::vector<fgk::PlainAtom> pa(size_t(std::max(N, 1)));
+    for (int i = 0; i < N; ++i) {
+        q[size_t(i)] = make_float4(float(A.x[i]), float(A.y[i]), float(A.z[i]), float(A.w ? A.w[i] : 0.0));
+        fgk::PlainAtom c{};
+        c.x = float(A.x[i]);
+        c.y = float(A.y[i]);
+        c.z = float(A.z[i]);
+        if (j->hasGeom) {
+            c.vol = float(A.vol[i]);
+            c.solparQ = float(A.solpar[i] + d.solpar_q * std::fabs(A.charge[i]));
+            for (int k = 0; k < 3; ++k) {
+                c.rv[k] = float(d.geom.rvector[3 * i + k]);
+                c.rv2[k] = float(d.geom.rvector2[3 * i + k]);
+            }
+            c.type = A.type[i];
+            c.hbond = A.hbond[i];
+            c.rexp = d.geom.rexp[i];
+            c.disorder = d.geom.disorder[i] ? 1 : 0;
+        }
+        pa[size_t(i)] = c;
+    }
+    if (int rc = devUpload(j, j->dPlainQ, q.data(), q.size())) return rc;
+    if (int rc = devUpload(j, j->dPlainAtoms, pa.data(), pa.size())) return rc;
+    const size_t pairN = size_t(std::max(d.tables.n_types, 1)) * fgk::kTableLen * size_t(std::max(j->M, 1));
+    std::vector<float> pair(pairN, 0.f), desolv(fgk::kTableLen, 0.f);
+    if (j->hasGeom) {
+        for (size_t k = 0; k < pairN && d.tables.pair; ++k) pair[k] = float(d.tables.pair[k]);
+        for (int k = 0; k < fgk::kTableLen; ++k) desolv[size_t(k)] = float(d.tables.desolv[k]);
+        for (int m = 0; m < j->M; ++m) {
+            const fg_map_params &p = d.maps[m];
+            j->pmp[m] = fgk::PlainMap{p.hbond, p.is_hbonder, p.is_covalent, 0, float(p.solpar_probe), float(p.vol_probe)};
+        }
+    }
+    j->solparQ = float(d.solpar_q);
+    if (int rc = devUpload(j, j->dPair32, pair.data(), pair.size())) return rc;
+    if (int rc = devUpload(j, j->dDesolv32, desolv.data(), desolv.size())) return rc;
+    if (int rc = devAlloc(j, j->dAcc32, size_t(j->P))) return rc;
+    FG_HIP(hipMemset(j->dAcc32, 0, size_t(j->P) * sizeof(float)));
+    return FG_OK;
+}
+
+// The plain form: cutoff maps (per-point lists) then the chunked e-map.
+int runPlainAffinity(fg_job *j)
+{
+    fgk::PlainCutArgs a{};
+    a.g = j->g;
+    a.nbx = j->nbx;
+    a.nby = j->nby;
+    a.nbz = j->nbz;
+    a.M = j->M;
+    a.hydrogen = j->hydrogen;
+    a.P = j->P;
+    a.solparQ = j->solparQ;
+    a.atoms = j->dPlainAtoms;
+    a.brickStart = j->dStart;
+    a.brickList = j->dList;
+    a.hStart = j->dHStart;
+    a.hList = j->dHList;
+    a.pair = j->dPair32;
+    a.desolv = j->dDesolv32;
+    a.maps = j->dMaps;
+    a.covalent = j->dCovalent;
+    for (int m = 0; m < j->M && m < 16; ++m) a.mp[m] = j->pmp[m];
+    const dim3 grid(unsigned((j->P + fgk::kPlainCutThreads - 1) / fgk::kPlainCutThreads)), block(fgk::kPlainCutThreads);
+    hipStream_t s = j->ctx->stream;
+    if (j->M <= 4) hipLaunchKernelGGL(fgk::k_plain_cutoff<4>, grid, block, 0, s, a);
+    else if (j->M <= 8) hipLaunchKernelGGL(fgk::k_plain_cutoff<8>, grid, block, 0, s, a);
+    else hipLaunchKernelGGL(fgk::k_plain_cutoff<16>, grid, block, 0, s, a);
+    FG_HIP(hipGetLastError());
+    return FG_OK;
+}
+
+int runPlainElec(fg_job *j)
+{
+    hipStream_t s = j->ctx->stream;
+    fgk::PlainElecArgs a{};
+    a.g = j->g;
+    a.ddd = j->distDepDiel;
+    a.atoms = j->dPlainQ;
+    a.eps = j->dEps32;
+    a.acc = j->dAcc32;
+    const dim3 grid(unsigned((j->g.nx + fgk::kPlainBX - 1) / fgk::kPlainBX),
+                    unsigned((j->g.ny + fgk::kPlainBY - 1) / fgk::kPlainBY),
+                    unsigned((j->g.nz + fgk::kPlainPZ - 1) / fgk::kPlainPZ));
+    for (int a0 = 0; a0 < j->N; a0 += fgk::kPlainAtomsPerLaunch) {
+        a.a0 = a0;
+        a.a1 = std::min(j->N, a0 + fgk::kPlainAtomsPerLaunch);
+        hipLaunchKernelGGL(fgk::k_plain_elec, grid, dim3(fgk::kPlainThreads), 0, s, a);
+    }
+    hipLaunchKernelGGL(fgk::k_plain_elec_finish, dim3(unsigned((j->P + 255) / 256)), dim3(256), 0, s, j->P, j->dAcc32,
+                       j->dMaps + int64_t(j->M) * j->P);
+    FG_HIP(hipGetLastError());
+    return FG_OK;
+}
+
 void destroyJob(fg_job *j)
 {
     if (!j) return;
     for (void *p : j->owned) (void)hipFree(p);
     delete j;
+}
+
+// ------------------------------------------------------------------------
+// Autotuner of the electrostatic kernel shape
+// ------------------------------------------------------------------------
+template <int PZ>
+void launchElecShape(fg_job *j, int tile, int bxShift, bool comp)
+{
+    const fgk::ElecArgs a = elecArgs(j, PZ, tile, bxShift);
+    launchElecPZ<PZ>(j, a, comp, size_t(j->tableLen) * sizeof(float));
+}
+
+// Device time of one electrostatic launch of shape (pz, tile, bxShift), ms.
+int timeShape(fg_job *j, int pz, int tile, int bxShift, bool comp, float &ms)
+{
+    fg_ctx *ctx = j->ctx;
+    auto go = [&]() {
+        if (pz == 8) launchElecShape<8>(j, tile, bxShift, comp);
+        else launchElecShape<4>(j, tile, bxShift, comp);
+    };
+    go();  // warm-up
+    FG_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    go();
+    go();
+    FG_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
+    FG_HIP(hipGetLastError());
+    FG_HIP(hipEventSynchronize(ctx->ev[3]));
+    FG_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]));
+    ms *= 0.5f;
+    return FG_OK;
+}
+
+// Padded xy area of a 256-lane brick of width 1 << shift over the grid.
+long paddedArea(const fg_job *j, int shift)
+{
+    const int bxl = 1 << shift, byl = fgk::kElecThreads >> shift;
+    return long((j->g.nx + bxl - 1) / bxl) * bxl * long((j->g.ny + byl - 1) / byl) * byl;
+}
+
+int ilog2(int64_t v)
+{
+    int r = 0;
+    while (v > 1) {
+        v >>= 1;
+        ++r;
+    }
+    return r;
+}
+
+int tuneJobs(fg_ctx *ctx, fg_job *const *jobs, int n, bool comp)
+{
+    static const int kPz[2] = {4, 8}, kTiles[3] = {32, 64, 128}, kShifts[3] = {3, 4, 5};
+    // size classes: (atom octave, point octave, best-padding brick width)
+    struct Cls {
+        int key[3];
+        std::vector<int> members;
+        float t[2][3][3];
+    };
+    std::vector<Cls> cls;
+    for (int i = 0; i < n; ++i) {
+        const fg_job *j = jobs[i];
+        int best = 4;
+        for (int sh : {3, 5})
+            if (paddedArea(j, sh) < paddedArea(j, best)) best = sh;
+        const int key[3] = {ilog2(std::max(j->N, 1)), ilog2(j->P), best};
+        auto it = std::find_if(cls.begin(), cls.end(), [&](const Cls &c) {
+            return c.key[0] == key[0] && c.key[1] == key[1] && c.key[2] == key[2];
+        });
+        if (it == cls.end()) {
+            cls.push_back(Cls{{key[0], key[1], key[2]}, {}, {}});
+            it = cls.end() - 1;
+        }
+        it->members.push_back(i);
+    }
+    double total[2] = {0, 0};
+    std::string report;
+    for (Cls &c : cls) {
+        std::sort(c.members.begin(), c.members.end(), [&](int x, int y) {
+            return double(jobs[x]->N) * double(jobs[x]->P) < double(jobs[y]->N) * double(jobs[y]->P);
+        });
+        fg_job *rep = jobs[c.members[c.members.size() / 2]];
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 3; ++b)
+                for (int d = 0; d < 3; ++d)
+                    if (int rc = timeShape(rep, kPz[a], kTiles[b], kShifts[d], comp, c.t[a][b][d])) return rc;
+        const double repWork = std::max(1.0, double(rep->N) * double(rep->P));
+        double work = 0;
+        for (int i : c.members) work += double(jobs[i]->N) * double(jobs[i]->P);
+        for (int a = 0; a < 2; ++a) {
+            float best = 1e30f;
+            for (int b = 0; b < 3; ++b)
+                for (int d = 0; d < 3; ++d) best = std::min(best, c.t[a][b][d]);
+            total[a] += best * work / repWork;
+        }
+    }
+    const int pzi = total[1] < total[0] ? 1 : 0;
+    char line[256];
+    std::snprintf(line, sizeof line, "pz %d for all jobs (modelled totals: pz4 %.3f ms, pz8 %.3f ms)\n", kPz[pzi],
+                  total[0], total[1]);
+    report += line;
+    for (Cls &c : cls) {
+        int bb = 0, bd = 0;
+        for (int b = 0; b < 3; ++b)
+            for (int d = 0; d < 3; ++d)
+                if (c.t[pzi][b][d] < c.t[pzi][bb][bd]) {
+                    bb = b;
+                    bd = d;
+                }
+        for (int i : c.members) {
+            jobs[i]->pz = kPz[pzi];
+            jobs[i]->tile = kTiles[bb];
+            jobs[i]->bxShift = kShifts[bd];
+        }
+        const fg_job *rep = jobs[c.members[c.members.size() / 2]];
+        std::snprintf(line, sizeof line,
+                      "class atoms~2^%d points~2^%d bx%d: %zu jobs, median %d atoms %dx%dx%d -> tile %d bx %d "
+                      "(%.4f ms; default shape %.4f ms)\n",
+                      c.key[0], c.key[1], 1 << c.key[2], c.members.size(), rep->N, rep->g.nx, rep->g.ny, rep->g.nz,
+                      kTiles[bb], 1 << kShifts[bd], c.t[pzi][bb][bd], c.t[pzi][1][1]);
+        report += line;
+    }
+    ctx->tuneReport = report;
+    return FG_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------
+// Batches
+// ------------------------------------------------------------------------
+struct fg_batch {
+    fg_ctx *ctx = nullptr;
+    std::vector<fg_job *> jobs;  // largest first
+    bool aff = false, near = false, elec = false, comp = true;
+    int MM = 4, KH = 0, pz = 4;
+    bool pointAng = false, clamp = false;
+    int sharedTableLen = 0, maxTableLen = 0;
+    int cutItems = 0, elecItems = 0, cutWGs = 0, elecWGs = 0;
+    fgk::ListArgs *dList = nullptr;
+    fgk::CutoffArgs *dCut = nullptr;
+    fgk::ElecArgs *dElec = nullptr;
+    int32_t *dCutBase = nullptr, *dElecBase = nullptr, *dCounters = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void destroyBatch(fg_batch *b)
+{
+    if (!b) return;
+    if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (void *p : {static_cast<void *>(b->dList), static_cast<void *>(b->dCut), static_cast<void *>(b->dElec),
+                    static_cast<void *>(b->dCutBase), static_cast<void *>(b->dElecBase),
+                    static_cast<void *>(b->dCounters)})
+        if (p) (void)hipFree(p);
+    delete b;
+}
+
+template <class T>
+int batchUpload(T *&dst, const std::vector<T> &src)
+{
+    FG_HIP(hipMalloc(&dst, std::max<size_t>(src.size(), 1) * sizeof(T)));
+    if (!src.empty()) FG_HIP(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return FG_OK;
+}
+
+template <class K>
+int persistentWGs(fg_batch *b, K kernel, int threads, size_t lds, int items, int &out)
+{
+    int perCU = 0;
+    FG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void *>(kernel), threads, lds));
+    out = std::max(1, std::min(items, std::max(perCU, 1) * b->ctx->numCUs));
+    return FG_OK;
+}
+
+// With launch false, only sizes the persistent grid (done before a capture).
+template <int MM, int KH, bool kAff, bool kPA>
+int launchCutoffBatch(fg_batch *b, bool launch)
+{
+    auto k = fgk::k_cutoff_batch<MM, KH, kAff, kPA>;
+    if (!b->cutWGs)
+        if (int rc = persistentWGs(b, k, fgk::kCutThreads, 0, b->cutItems, b->cutWGs)) return rc;
+    if (!launch) return FG_OK;
+    hipLaunchKernelGGL(k, dim3(b->cutWGs), dim3(fgk::kCutThreads), 0, b->ctx->stream, b->dCut, b->dCutBase,
+                       int(b->jobs.size()), b->dCounters);
+    return FG_OK;
+}
+
+template <int MM, int KH>
+int launchCutoffBatchPA(fg_batch *b, bool launch)
+{
+    return b->pointAng ? launchCutoffBatch<MM, KH, true, true>(b, launch)
+                       : launchCutoffBatch<MM, KH, true, false>(b, launch);
+}
+
+template <int MM>
+int launchCutoffBatchKH(fg_batch *b, bool launch)
+{
+    if (b->KH == 0) return launchCutoffBatchPA<MM, 0>(b, launch);
+    if (b->KH <= 2) return launchCutoffBatchPA<MM, 2>(b, launch);
+    return launchCutoffBatchPA<MM, 4>(b, launch);
+}
+
+template <int PZ, bool kComp, bool kClamp>
+int launchElecBatch(fg_batch *b, bool launch)
+{
+    auto k = fgk::k_elec_batch<PZ, kComp, kClamp>;
+    const size_t lds = size_t(b->sharedTableLen > 0 ? b->sharedTableLen : b->maxTableLen) * sizeof(float);
+    if (!b->elecWGs)
+        if (int rc = persistentWGs(b, k, fgk::kElecThreads, lds, b->elecItems, b->elecWGs)) return rc;
+    if (!launch) return FG_OK;
+    hipLaunchKernelGGL(k, dim3(b->elecWGs), dim3(fgk::kElecThreads), lds, b->ctx->stream, b->dElec, b->dElecBase,
+                       int(b->jobs.size()), b->dCounters + 1, b->sharedTableLen);
+    return FG_OK;
+}
+
+template <int PZ>
+int launchElecBatchPZ(fg_batch *b, bool launch)
+{
+    if (b->comp)
+        return b->clamp ? launchElecBatch<PZ, true, true>(b, launch) : launchElecBatch<PZ, true, false>(b, launch);
+    return b->clamp ? launchElecBatch<PZ, false, true>(b, launch) : launchElecBatch<PZ, false, false>(b, launch);
+}
+
+// The batch's launch sequence on the context's stream; `mid` (if any) is
+// recorded between the cutoff and electrostatic phases.  With launch false,
+// only sizes the persistent grids.
+int enqueueBatch(fg_batch *b, hipEvent_t mid, bool launch = true)
+{
+    hipStream_t s = b->ctx->stream;
+    const int n = int(b->jobs.size());
+    if (launch) hipLaunchKernelGGL(fgk::k_fill_i32, dim3(1), dim3(256), 0, s, b->dCounters, 2, 0);
+    if (b->aff || b->near) {
+        if (launch) {
+            hipLaunchKernelGGL(fgk::k_brick_lists_batch<false>, dim3(b->cutItems), dim3(fgk::kListThreads), 0, s,
+                               b->dList, b->dCutBase, n);
+            hipLaunchKernelGGL(fgk::k_scan_counts_batch, dim3(n), dim3(1024), 0, s, b->dList);
+            hipLaunchKernelGGL(fgk::k_brick_lists_batch<true>, dim3(b->cutItems), dim3(fgk::kListThreads), 0, s,
+                               b->dList, b->dCutBase, n);
+        }
+        int rc;
+        if (!b->aff) rc = launchCutoffBatch<1, 0, false, false>(b, launch);
+        else if (b->MM == 4) rc = launchCutoffBatchKH<4>(b, launch);
+        else if (b->MM == 8) rc = launchCutoffBatchKH<8>(b, launch);
+        else rc = launchCutoffBatchKH<16>(b, launch);
+        if (rc) return rc;
+    }
+    if (mid && launch) FG_HIP(hipEventRecord(mid, s));
+    if (b->elec) {
+        int rc = b->pz == 8 ? launchElecBatchPZ<8>(b, launch) : launchElecBatchPZ<4>(b, launch);
+        if (rc) return rc;
+    }
+    FG_HIP(hipGetLastError());
+    return FG_OK;
+}
+
+int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned what, bool comp, bool useGraph)
+{
+    b->ctx = ctx;
+    b->jobs.assign(jobs, jobs + n);
+    // largest jobs first: the items at the end of the queue are the short ones
+    std::stable_sort(b->jobs.begin(), b->jobs.end(), [](const fg_job *x, const fg_job *y) { return x->N > y->N; });
+    b->comp = comp;
+    b->elec = (what & FG_RUN_ELEC) != 0;
+    b->near = b->elec && comp;
+    b->aff = (what & FG_RUN_AFFINITY) != 0;
+    // one pz for the batch: the one carrying the most work
+    double work[2] = {0, 0};
+    int maxM = 0;
+    for (fg_job *j : b->jobs) {
+        work[j->pz == 8 ? 1 : 0] += double(j->N) * double(j->P);
+        maxM = std::max(maxM, j->M);
+        b->KH = std::max(b->KH, j->KH);
+        b->pointAng = b->pointAng || j->pointAng;
+        b->clamp = b->clamp || j->clamp;
+        b->maxTableLen = std::max(b->maxTableLen, j->tableLen);
+    }
+    b->pz = work[1] > work[0] ? 8 : 4;
+    b->MM = maxM <= 4 ? 4 : maxM <= 8 ? 8 : 16;
+    if (b->aff && maxM == 0) b->aff = false;
+    // one LDS table for all jobs when their dielectric tables agree
+    {
+        bool same = true;
+        std::vector<float> t0(static_cast<size_t>(fgk::kTableLen)), t1(static_cast<size_t>(fgk::kTableLen));
+        FG_HIP(hipMemcpy(t0.data(), b->jobs[0]->dEps32, t0.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 1; i < b->jobs.size() && same; ++i) {
+            FG_HIP(hipMemcpy(t1.data(), b->jobs[i]->dEps32, t1.size() * sizeof(float), hipMemcpyDeviceToHost));
+            same = t0 == t1;
+        }
+        b->sharedTableLen = same ? b->maxTableLen : 0;
+    }
+    std::vector<fgk::ListArgs> la;
+    std::vector<fgk::CutoffArgs> ca;
+    std::vector<fgk::ElecArgs> ea;
+    std::vector<int32_t> cutBase(1, 0), elecBase(1, 0);
+    for (fg_job *j : b->jobs) {
+        la.push_back(listArgs(j));
+        ca.push_back(cutoffArgs(j, b->near ? 1 : 0));
+        fgk::ElecArgs e = elecArgs(j, b->pz, j->tile, j->bxShift);
+        if (b->sharedTableLen > 0) e.clampMax = b->sharedTableLen - 1;
+        ea.push_back(e);
+        const int64_t c = int64_t(cutBase.back()) + j->nb, el = int64_t(elecBase.back()) + int64_t(e.nbx) * e.nby * e.nbz;
+        if (c > INT32_MAX || el > INT32_MAX) return fgc::fail(FG_E_ARG, "fg_batch_create: too many work items");
+        cutBase.push_back(int32_t(c));
+        elecBase.push_back(int32_t(el));
+    }
+    b->cutItems = cutBase.back();
+    b->elecItems = elecBase.back();
+    if (int rc = batchUpload(b->dList, la)) return rc;
+    if (int rc = batchUpload(b->dCut, ca)) return rc;
+    if (int rc = batchUpload(b->dElec, ea)) return rc;
+    if (int rc = batchUpload(b->dCutBase, cutBase)) return rc;
+    if (int rc = batchUpload(b->dElecBase, elecBase)) return rc;
+    if (int rc = batchUpload(b->dCounters, std::vector<int32_t>{0, 0})) return rc;
+    // persistent grid sizes (queried outside any capture)
+    if (int rc = enqueueBatch(b, nullptr, false)) return rc;
+    if (!useGraph) return FG_OK;
+    hipStream_t s = ctx->stream;
+    FG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = enqueueBatch(b, nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fgc::fail(FG_E_DEVICE, std::string("fg_batch_create: capture: ") + hipGetErrorString(e));
+    b->graph = g;
+    FG_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return FG_OK;
 }
 
 }  // namespace
@@ -534,6 +992,9 @@ int fg_create(int device, fg_ctx **out)
             delete c;
             return fgc::fail(FG_E_DEVICE, "fg_create: hipEventCreate failed");
         }
+    if (hipDeviceGetAttribute(&c->numCUs, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        c->numCUs <= 0)
+        c->numCUs = 256;
     *out = c;
     return FG_OK;
 }
@@ -637,6 +1098,7 @@ int fg_job_create(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
     }
     int rc = buildCutoff(j, *d);
     if (!rc) rc = buildElecTiles(j, *d, gridMaxAbs);
+    if (!rc) rc = buildPlain(j, *d);
     if (!rc) rc = devAlloc(j, j->dMaps, size_t(j->M + 2) * size_t(j->P));
     if (!rc) {
         hipError_t e = hipMemsetAsync(j->dMaps, 0, size_t(j->M + 2) * size_t(j->P) * sizeof(double), ctx->stream);
@@ -660,7 +1122,7 @@ void fg_job_destroy(fg_job *job)
 int fg_job_run(fg_ctx *ctx, fg_job *j, unsigned what, int precision)
 {
     if (!ctx || !j || j->ctx != ctx) return fgc::fail(FG_E_ARG, "fg_job_run: job does not belong to this context");
-    if (precision < FG_PREC_PLAIN || precision > FG_PREC_FP64) return fgc::fail(FG_E_ARG, "fg_job_run: bad precision");
+    if (precision < FG_PREC_PLAIN || precision > FG_PREC_FP32) return fgc::fail(FG_E_ARG, "fg_job_run: bad precision");
     if ((what & FG_RUN_AFFINITY) && j->M > 0 && !j->hasGeom)
         return fgc::fail(FG_E_ARG, "fg_job_run: job has no affinity inputs");
     FG_HIP(hipSetDevice(ctx->device));
@@ -668,6 +1130,22 @@ int fg_job_run(fg_ctx *ctx, fg_job *j, unsigned what, int precision)
     const bool elec = what & FG_RUN_ELEC, aff = (what & FG_RUN_AFFINITY) != 0;
     const bool near = elec && precision == FG_PREC_COMPENSATED;
     FG_HIP(hipEventRecord(ctx->ev[0], s));
+    if (precision == FG_PREC_PLAIN) {
+        if (aff && j->hasGeom) {
+            fgk::ListArgs la = listArgs(j);
+            if (int rc = runListCount(j, la)) return rc;
+            hipLaunchKernelGGL(fgk::k_brick_lists<true>, dim3(j->nb), dim3(fgk::kListThreads), 0, s, la);
+            FG_HIP(hipGetLastError());
+            if (int rc = runPlainAffinity(j)) return rc;
+        }
+        FG_HIP(hipEventRecord(ctx->ev[1], s));
+        if (elec)
+            if (int rc = runPlainElec(j)) return rc;
+        FG_HIP(hipEventRecord(ctx->ev[2], s));
+        ctx->haveTiming = true;
+        ctx->splitTiming = true;
+        return FG_OK;
+    }
     if (aff || near) {
         fgk::ListArgs la = listArgs(j);
         if (int rc = runListCount(j, la)) return rc;
@@ -680,6 +1158,7 @@ int fg_job_run(fg_ctx *ctx, fg_job *j, unsigned what, int precision)
         if (int rc = launchElec(j, precision)) return rc;
     FG_HIP(hipEventRecord(ctx->ev[2], s));
     ctx->haveTiming = true;
+    ctx->splitTiming = true;
     return FG_OK;
 }
 
@@ -695,9 +1174,11 @@ int fg_last_timing(fg_ctx *ctx, fg_timing *t)
     if (!ctx || !t) return fgc::fail(FG_E_ARG, "fg_last_timing: NULL argument");
     if (!ctx->haveTiming) return fgc::fail(FG_E_ARG, "fg_last_timing: nothing has run");
     FG_HIP(hipEventSynchronize(ctx->ev[2]));
-    float a = 0, b = 0, c = 0;
-    FG_HIP(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]));
-    FG_HIP(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
+    float a = -1, b = -1, c = 0;
+    if (ctx->splitTiming) {
+        FG_HIP(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]));
+        FG_HIP(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
+    }
     FG_HIP(hipEventElapsedTime(&c, ctx->ev[0], ctx->ev[2]));
     t->affinity_ms = a;
     t->elec_ms = b;
@@ -741,12 +1222,91 @@ int fg_job_cutoff_pairs(fg_ctx *ctx, fg_job *j, int64_t *pairs)
     return FG_OK;
 }
 
+int fg_job_get_shape(const fg_job *job, fg_shape *out)
+{
+    if (!job || !out) return fgc::fail(FG_E_ARG, "fg_job_get_shape: NULL argument");
+    out->pz = job->pz;
+    out->tile = job->tile;
+    out->bx = 1 << job->bxShift;
+    return FG_OK;
+}
+
+int fg_job_set_shape(fg_job *job, const fg_shape *sh)
+{
+    if (!job || !sh) return fgc::fail(FG_E_ARG, "fg_job_set_shape: NULL argument");
+    const int shift = sh->bx == 8 ? 3 : sh->bx == 16 ? 4 : sh->bx == 32 ? 5 : -1;
+    if ((sh->pz != 4 && sh->pz != 8) || (sh->tile != 32 && sh->tile != 64 && sh->tile != 128) || shift < 0)
+        return fgc::fail(FG_E_ARG, "fg_job_set_shape: pz must be 4 or 8, tile 32/64/128, bx 8/16/32");
+    job->pz = sh->pz;
+    job->tile = sh->tile;
+    job->bxShift = shift;
+    return FG_OK;
+}
+
+int fg_autotune(fg_ctx *ctx, fg_job *const *jobs, int n, int precision)
+{
+    if (!ctx || (!jobs && n > 0)) return fgc::fail(FG_E_ARG, "fg_autotune: NULL argument");
+    if (precision != FG_PREC_COMPENSATED && precision != FG_PREC_FP32)
+        return fgc::fail(FG_E_ARG, "fg_autotune: precision must be FG_PREC_COMPENSATED or FG_PREC_FP32");
+    for (int i = 0; i < n; ++i)
+        if (!jobs[i] || jobs[i]->ctx != ctx) return fgc::fail(FG_E_ARG, "fg_autotune: job does not belong to this context");
+    FG_HIP(hipSetDevice(ctx->device));
+    return tuneJobs(ctx, jobs, n, precision == FG_PREC_COMPENSATED);
+}
+
+const char *fg_tune_report(const fg_ctx *ctx) { return ctx ? ctx->tuneReport.c_str() : ""; }
+
+int fg_batch_create(fg_ctx *ctx, fg_job *const *jobs, int n, unsigned what, int precision, unsigned flags,
+                    fg_batch **out)
+{
+    if (!ctx || !out || (!jobs && n > 0)) return fgc::fail(FG_E_ARG, "fg_batch_create: NULL argument");
+    *out = nullptr;
+    if (precision != FG_PREC_COMPENSATED && precision != FG_PREC_FP32)
+        return fgc::fail(FG_E_ARG, "fg_batch_create: precision must be FG_PREC_COMPENSATED or FG_PREC_FP32");
+    if (n <= 0) return fgc::fail(FG_E_ARG, "fg_batch_create: no jobs");
+    for (int i = 0; i < n; ++i) {
+        if (!jobs[i] || jobs[i]->ctx != ctx) return fgc::fail(FG_E_ARG, "fg_batch_create: job does not belong to this context");
+        if ((what & FG_RUN_AFFINITY) && jobs[i]->M > 0 && !jobs[i]->hasGeom)
+            return fgc::fail(FG_E_ARG, "fg_batch_create: a job has no affinity inputs");
+    }
+    FG_HIP(hipSetDevice(ctx->device));
+    fg_batch *b = new fg_batch();
+    int rc = buildBatch(ctx, b, jobs, n, what, precision == FG_PREC_COMPENSATED, (flags & FG_BATCH_NO_GRAPH) == 0);
+    if (rc) {
+        destroyBatch(b);
+        return rc;
+    }
+    *out = b;
+    return FG_OK;
+}
+
+int fg_batch_launch(fg_ctx *ctx, fg_batch *b)
+{
+    if (!ctx || !b || b->ctx != ctx) return fgc::fail(FG_E_ARG, "fg_batch_launch: batch does not belong to this context");
+    FG_HIP(hipSetDevice(ctx->device));
+    FG_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    if (b->exec) {
+        FG_HIP(hipGraphLaunch(b->exec, ctx->stream));
+        ctx->splitTiming = false;
+    } else {
+        if (int rc = enqueueBatch(b, ctx->ev[1])) return rc;
+        ctx->splitTiming = true;
+    }
+    FG_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+    ctx->haveTiming = true;
+    return FG_OK;
+}
+
+void fg_batch_destroy(fg_batch *b) { destroyBatch(b); }
+
 int fg_batch_run(fg_ctx *ctx, fg_job *const *jobs, int n, unsigned what, int precision)
 {
-    if (!ctx || (!jobs && n > 0)) return fgc::fail(FG_E_ARG, "fg_batch_run: NULL argument");
-    for (int i = 0; i < n; ++i)
-        if (int rc = fg_job_run(ctx, jobs[i], what, precision)) return rc;
-    return FG_OK;
+    fg_batch *b = nullptr;
+    if (int rc = fg_batch_create(ctx, jobs, n, what, precision, 0, &b)) return rc;
+    int rc = fg_batch_launch(ctx, b);
+    if (!rc) rc = fg_sync(ctx);
+    fg_batch_destroy(b);
+    return rc;
 }
 
 int fg_elec_map(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, const double *eps, int ddd, int precision,

@@ -10,7 +10,7 @@
 // and have no effect on the maps.
 //
 // Additional switches:
-//   --precision=plain|compensated|fp64   electrostatic-map form (default compensated)
+//   --precision=plain|compensated|fp64|fp32   computation form (default compensated)
 //   --ag421-param-order                  read parameter_file where the GPF names it
 #include <sys/wait.h>
 #include <unistd.h>
@@ -53,7 +53,7 @@ struct Options {
                  "                    (POSIX only), this must be the first parameter if used\n"
                  "      --v4          set the AutoGrid 4.0 default parameter library\n\n"
                  "Advanced:\n"
-                 "      --precision=plain|compensated|fp64  electrostatic map form, default: compensated\n"
+                 "      --precision=plain|compensated|fp64|fp32  computation form, default: compensated\n"
                  "      --ag421-param-order  read parameter_file where the GPF names it (AutoGrid 4.2.1)\n"
                  "      --cuda-ddd=g|c|t|i, --cuda-slices=y|n, --cuda-thread=y|n, --cuda-unroll=y|n,\n"
                  "      --no-cogrid, --no-nns   accepted for compatibility (no effect on the maps)\n\n",
@@ -117,6 +117,7 @@ Options parse(int argc, char **argv)
         else if (a == "--precision=plain") o.precision = FG_PREC_PLAIN;
         else if (a == "--precision=compensated") o.precision = FG_PREC_COMPENSATED;
         else if (a == "--precision=fp64") o.precision = FG_PREC_FP64;
+        else if (a == "--precision=fp32") o.precision = FG_PREC_FP32;
         else if (a.rfind("--cuda-ddd=", 0) == 0 || a.rfind("--cuda-unroll=", 0) == 0 ||
                  a.rfind("--cuda-thread=", 0) == 0 || a.rfind("--cuda-slices=", 0) == 0 || a == "--no-cogrid" ||
                  a == "--no-nns") {

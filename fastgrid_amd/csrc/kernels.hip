@@ -15,10 +15,21 @@ namespace fgk {
 // ------------------------------------------------------------------------
 __device__ __forceinline__ double dmin_(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double dmax_(double a, double b) { return a > b ? a : b; }
+// libm by precision: the FP64 forms call the double functions, the FP32
+// plain form the float ones
+__host__ __device__ __forceinline__ double acos_(double x) { return acos(x); }
+__host__ __device__ __forceinline__ float acos_(float x) { return acosf(x); }
+__host__ __device__ __forceinline__ double cos_(double x) { return cos(x); }
+__host__ __device__ __forceinline__ float cos_(float x) { return cosf(x); }
+__host__ __device__ __forceinline__ double sin_(double x) { return sin(x); }
+__host__ __device__ __forceinline__ float sin_(float x) { return sinf(x); }
+__host__ __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
+__host__ __device__ __forceinline__ float sqrt_(float x) { return sqrtf(x); }
 // Math::Acos with the reference's clamp (FG/math/Math.h:51)
-__host__ __device__ __forceinline__ double acosClamped(double f)
+template <class T>
+__host__ __device__ __forceinline__ T acosClamped(T f)
 {
-    return f > 1 ? 0.0 : (f < -1 ? 3.14159265358979323846 : acos(f));
+    return f > 1 ? T(0) : (f < -1 ? T(3.14159265358979323846) : acos_(f));
 }
 // angstromToIndex (FG/autogrid.h:120-125)
 __device__ __forceinline__ int binOf(double r)
@@ -50,15 +61,20 @@ struct ListArgs {
     int32_t *list, *hList;
 };
 
+struct ListLds {
+    int32_t cellOff[kMaxCellsPerBrick + 1];
+    int32_t cellId[kMaxCellsPerBrick];
+    int32_t scanBuf[kListThreads], scanH[kListThreads];
+    int32_t base, baseH;
+};
+
+// Candidate list of brick b: counts it (kFill false) or writes it.
 template <bool kFill>
-__global__ __launch_bounds__(kListThreads) void k_brick_lists(ListArgs a)
+__device__ __forceinline__ void brickList(const ListArgs &a, int b, ListLds &S)
 {
-    __shared__ int32_t cellOff[kMaxCellsPerBrick + 1];
-    __shared__ int32_t cellId[kMaxCellsPerBrick];
-    __shared__ int32_t scanBuf[kListThreads], scanH[kListThreads];
-    __shared__ int32_t base, baseH;
+    int32_t *cellOff = S.cellOff, *cellId = S.cellId, *scanBuf = S.scanBuf, *scanH = S.scanH;
+    int32_t &base = S.base, &baseH = S.baseH;
     const int tid = threadIdx.x;
-    const int b = blockIdx.x;
     const int bx = b % a.nbx, by = (b / a.nbx) % a.nby, bz = b / (a.nbx * a.nby);
     const int ib[3] = {bx * kCutBX, by * kCutBY, bz * kCutBZ};
     const int ext[3] = {kCutBX, kCutBY, kCutBZ};
@@ -166,11 +182,17 @@ __global__ __launch_bounds__(kListThreads) void k_brick_lists(ListArgs a)
     }
 }
 
-// Exclusive scan of counts[i*2 + which] into out[0..n] (out[n] = total), one
-// workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan_counts(const int32_t *counts, int n, int which, int32_t *out)
+template <bool kFill>
+__global__ __launch_bounds__(kListThreads) void k_brick_lists(ListArgs a)
 {
-    __shared__ int32_t part[1024];
+    __shared__ ListLds S;
+    brickList<kFill>(a, int(blockIdx.x), S);
+}
+
+// Exclusive scan of counts[i*2 + which] into out[0..n] (out[n] = total) by
+// one workgroup of 1024 threads.
+__device__ __forceinline__ void scanCounts(const int32_t *counts, int n, int which, int32_t *out, int32_t *part)
+{
     const int tid = threadIdx.x;
     const int per = (n + 1023) / 1024;
     const int b0 = tid * per, b1 = min(n, b0 + per);
@@ -192,33 +214,41 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const int32_t *counts, int
     if (tid == 1023) out[n] = part[1023];
 }
 
+__global__ __launch_bounds__(1024) void k_scan_counts(const int32_t *counts, int n, int which, int32_t *out)
+{
+    __shared__ int32_t part[1024];
+    scanCounts(counts, n, which, out, part);
+}
+
 // ------------------------------------------------------------------------
 // Cutoff kernel: affinity maps + desolvation map + near-field electrostatics
 // (FG/CalculateGridmaps.cpp:238-316, FG/electrostatics/Cpu.cpp:67-79 for the
 // pairs with r^2 <= 64).
 // ------------------------------------------------------------------------
 
-// racc / rdon / Hramp (FG/CalculateGridmaps.cpp:84-197).  hb, rexp and
-// disorder are wave-uniform.
-__host__ __device__ __forceinline__ void hbondAngular(int hb, int rexp, int disorder, bool isClosest,
-                                             const double rv[3], const double rv2[3], const double cHrv[3],
-                                             double ux, double uy, double uz,
-                                             double &racc, double &rdon, double &Hramp)
+// racc / rdon / Hramp (FG/CalculateGridmaps.cpp:84-197), in FP64 for the
+// exact forms and FP32 for the plain form.  hb, rexp and disorder are
+// wave-uniform in the cutoff kernel.
+template <class T>
+__host__ __device__ __forceinline__ void hbondAngular(int hb, int rexp, int disorder, bool isClosest, const T rv[3],
+                                                      const T rv2[3], const T cHrv[3], T ux, T uy, T uz, T &racc,
+                                                      T &rdon, T &Hramp)
 {
+    const T halfPi = T(3.14159265358979323846 / 2);
     racc = 1;
     rdon = 1;
     Hramp = 1;
-    const double cosT = -(ux * rv[0] + uy * rv[1] + uz * rv[2]);
+    const T cosT = -(ux * rv[0] + uy * rv[1] + uz * rv[2]);
     if (hb == HB_D1) {
         racc = 0;
         if (cosT > 0) {
-            double r = cosT;
+            T r = cosT;
             if (rexp == 4) r = r * r;
             if (rexp == 4 || rexp == 2) r = r * r;
             racc = r;
             if (!isClosest) {
-                double th = acosClamped(cHrv[0] * rv[0] + cHrv[1] * rv[1] + cHrv[2] * rv[2]);
-                Hramp = 0.5 - 0.5 * cos(th * (120.0 / 90.0));
+                T th = acosClamped(cHrv[0] * rv[0] + cHrv[1] * rv[1] + cHrv[2] * rv[2]);
+                Hramp = T(0.5) - T(0.5) * cos_(th * T(120.0 / 90.0));
             }
         }
     } else if (hb == HB_A1) {
@@ -228,31 +258,30 @@ __host__ __device__ __forceinline__ void hbondAngular(int hb, int rexp, int diso
         rdon = 0;
         if (disorder) {
             racc = 0;
-            double th = acosClamped(cosT);
-            if (th <= 1.24791 + (3.14159265358979323846 / 2)) {
-                double c = cos(th - 1.24791);
+            T th = acosClamped(cosT);
+            if (th <= T(1.24791) + halfPi) {
+                T c = cos_(th - T(1.24791));
                 c = c * c;
                 rdon = c * c;
                 racc = rdon;
             }
         } else {
-            double r = 0;
+            T r = 0;
             if (cosT >= 0) {
-                double cx = uy * rv2[2] - uz * rv2[1];
-                double cy = uz * rv2[0] - ux * rv2[2];
-                double cz = ux * rv2[1] - uy * rv2[0];
-                double rd2 = cx * cx + cy * cy + cz * cz;
-                if (rd2 < 1.0E-6) rd2 = 1.0E-6;
-                double inv = 1 / sqrt(rd2);
-                double ti = fabs(acosClamped(inv * (cx * rv[0] + cy * rv[1] + cz * rv[2])) -
-                                 (3.14159265358979323846 / 2));
-                r = 0.9 + 0.1 * sin(ti + ti);
-            } else if (cosT >= -0.34202) {
-                double t = 0.116978 - cosT * cosT;
-                r = 562.25 * (t * t * t);
+                T cx = uy * rv2[2] - uz * rv2[1];
+                T cy = uz * rv2[0] - ux * rv2[2];
+                T cz = ux * rv2[1] - uy * rv2[0];
+                T rd2 = cx * cx + cy * cy + cz * cz;
+                if (rd2 < T(1.0E-6)) rd2 = T(1.0E-6);
+                T inv = 1 / sqrt_(rd2);
+                T ti = fabs(acosClamped(inv * (cx * rv[0] + cy * rv[1] + cz * rv[2])) - halfPi);
+                r = T(0.9) + T(0.1) * sin_(ti + ti);
+            } else if (cosT >= T(-0.34202)) {
+                T t = T(0.116978) - cosT * cosT;
+                r = T(562.25) * (t * t * t);
             }
-            double t0 = (3.14159265358979323846 / 2) - acosClamped(ux * rv2[0] + uy * rv2[1] + uz * rv2[2]);
-            r *= cos(t0);
+            T t0 = halfPi - acosClamped(ux * rv2[0] + uy * rv2[1] + uz * rv2[2]);
+            r *= cos_(t0);
             rdon = r;
         }
     }
@@ -288,19 +317,20 @@ constexpr int kChunk = 32;  // candidate atoms staged in LDS per round
 // kPointAng: the job has atoms whose directional terms depend on the point
 // (bonded donors/acceptors); without them the FP64 transcendental code and its
 // registers are left out.
-template <int MM, int KH, bool kAffinity, bool kPointAng>
-__global__ __launch_bounds__(kCutThreads) void k_cutoff(CutoffArgs a)
-{
-    __shared__ CutAtom sAtom[kChunk];
-    __shared__ double sAB[kAffinity ? kChunk * 2 * MM : 1];
-    __shared__ int32_t sIdx[kChunk];
+template <int MM, bool kAffinity>
+struct CutoffLds {
+    CutAtom atom[kChunk];
+    double ab[kAffinity ? kChunk * 2 * MM : 1];
+    int32_t idx[kChunk];
+};
 
-    // XCD-aware brick order: consecutive workgroups go to different XCDs, so
-    // hand each XCD a contiguous run of bricks (neighbours share atoms in L2)
-    // (bijection on the first 8*floor(nb/8) workgroups, identity on the tail)
-    const int nb = a.nbx * a.nby * a.nbz;
-    const int full = (nb / 8) * 8, g = int(blockIdx.x);
-    const int b = g < full ? (g % 8) * (full / 8) + g / 8 : g;
+// One brick of the cutoff kernel; S is the workgroup's LDS staging area.
+template <int MM, int KH, bool kAffinity, bool kPointAng>
+__device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLds<MM, kAffinity> &S)
+{
+    CutAtom *sAtom = S.atom;
+    double *sAB = S.ab;
+    int32_t *sIdx = S.idx;
     const int bx = b % a.nbx, by = (b / a.nbx) % a.nby, bz = b / (a.nbx * a.nby);
     // wave = 4x4x4 cube of points, workgroup = 2x2x1 waves
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
@@ -459,55 +489,78 @@ __global__ __launch_bounds__(kCutThreads) void k_cutoff(CutoffArgs a)
     if (a.doElecNear) a.maps[int64_t(M) * a.P + p] = enear;
 }
 
+template <int MM, int KH, bool kAffinity, bool kPointAng>
+__global__ __launch_bounds__(kCutThreads) void k_cutoff(CutoffArgs a)
+{
+    __shared__ CutoffLds<MM, kAffinity> S;
+    // XCD-aware brick order: consecutive workgroups go to different XCDs, so
+    // hand each XCD a contiguous run of bricks (neighbours share atoms in L2)
+    // (bijection on the first 8*floor(nb/8) workgroups, identity on the tail)
+    const int nb = a.nbx * a.nby * a.nbz;
+    const int full = (nb / 8) * 8, g = int(blockIdx.x);
+    const int b = g < full ? (g % 8) * (full / 8) + g / 8 : g;
+    cutoffBrick<MM, KH, kAffinity, kPointAng>(a, b, S);
+}
+
 // ------------------------------------------------------------------------
-// Far-field electrostatics in FP32 (compensated) or all-pairs FP32 (plain).
+// Far-field electrostatics in FP32 (compensated) or all-pairs FP32.
 // Coordinates are in units of 0.01 A, so 100*r = r2 * rsq(r2) indexes the
 // dielectric table directly and w is stored as 100*w.
 // ------------------------------------------------------------------------
-template <int PZ, bool kComp, bool kClamp>
-__global__ __launch_bounds__(kElecThreads) void k_elec(ElecArgs a)
-{
-    extern __shared__ float eps[];
-    const int tid = threadIdx.x;
-    for (int i = tid; i < a.tableLen; i += kElecThreads) eps[i] = a.eps32[i];
-    __syncthreads();
 
-    const int lx = tid & 15, ly = tid >> 4;
-    const int ix = blockIdx.x * kElecBX + lx, iy = blockIdx.y * kElecBY + ly, iz0 = blockIdx.z * PZ;
+// Loads the dielectric table into LDS (all lanes of the workgroup).
+__device__ __forceinline__ void loadEps(float *eps, const float *src, int n)
+{
+    for (int i = threadIdx.x; i < n; i += kElecThreads) eps[i] = src[i];
+}
+
+// One brick of the electrostatic map: 256 lanes in x, y, PZ points per lane
+// along z.  `eps` is the table in LDS.
+template <int PZ, bool kComp, bool kClamp>
+__device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float *eps)
+{
+    const int tid = threadIdx.x;
+    const int sh = a.bxShift, bxl = 1 << sh, byl = kElecThreads >> sh;
+    const int bx = b % a.nbx, by = (b / a.nbx) % a.nby, bz = b / (a.nbx * a.nby);
+    const int ix0 = bx * bxl, iy0 = by * byl, iz0 = bz * PZ;
+    const int ix = ix0 + (tid & (bxl - 1)), iy = iy0 + (tid >> sh);
     const float s100 = a.spacing100;
     const float gx = float(ix - a.g.dx2) * s100, gy = float(iy - a.g.dy2) * s100;
     float gz[PZ];
 #pragma unroll
     for (int k = 0; k < PZ; ++k) gz[k] = float(iz0 + k - a.g.dz2) * s100;
     // brick box (scaled), clamped to the grid
-    const int bx1 = min(int(blockIdx.x) * kElecBX + kElecBX - 1, a.g.nx - 1);
-    const int by1 = min(int(blockIdx.y) * kElecBY + kElecBY - 1, a.g.ny - 1);
+    const int bx1 = min(ix0 + bxl - 1, a.g.nx - 1), by1 = min(iy0 + byl - 1, a.g.ny - 1);
     const int bz1 = min(iz0 + PZ - 1, a.g.nz - 1);
-    const float blo[3] = {float(int(blockIdx.x) * kElecBX - a.g.dx2) * s100,
-                          float(int(blockIdx.y) * kElecBY - a.g.dy2) * s100, float(iz0 - a.g.dz2) * s100};
+    const float blo[3] = {float(ix0 - a.g.dx2) * s100, float(iy0 - a.g.dy2) * s100, float(iz0 - a.g.dz2) * s100};
     const float bhi[3] = {float(bx1 - a.g.dx2) * s100, float(by1 - a.g.dy2) * s100, float(bz1 - a.g.dz2) * s100};
+    const unsigned clampMax = unsigned(a.clampMax);
 
     float hi[PZ], lo[PZ], part[PZ];
 #pragma unroll
     for (int k = 0; k < PZ; ++k) hi[k] = lo[k] = part[k] = 0.f;
 
-    for (int t = 0; t < a.nTiles; ++t) {
+    for (int j0 = 0; j0 < a.nAtoms; j0 += a.tile) {
+        const int j1 = min(j0 + a.tile, a.nAtoms);
         bool far = true;
         if (kComp) {
-            const float *tb = a.tileBox + 6 * t;
-            float dd = 0;
+            // the tile is far when every sub-tile box is beyond farCut
+            for (int q = j0; q < j1; q += kSubTile) {
+                const float *tb = a.subBox + 6 * (q / kSubTile);
+                float dd = 0;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                float d = fmaxf(fmaxf(tb[k] - bhi[k], blo[k] - tb[3 + k]), 0.f);
-                dd = __builtin_fmaf(d, d, dd);
+                for (int k = 0; k < 3; ++k) {
+                    float d = fmaxf(fmaxf(tb[k] - bhi[k], blo[k] - tb[3 + k]), 0.f);
+                    dd = __builtin_fmaf(d, d, dd);
+                }
+                far = far && dd > a.farCut2;
             }
-            far = uniform(dd > a.farCut2);
+            far = uniform(far);
         }
-        const float4 *T = a.tiles + int64_t(t) * kTile;
         if (far) {
 #pragma unroll 2
-            for (int j = 0; j < kTile; ++j) {
-                const float4 at = T[j];
+            for (int j = j0; j < j1; ++j) {
+                const float4 at = a.tiles[j];
                 const float dx = at.x - gx, dy = at.y - gy;
                 const float dxy2 = __builtin_fmaf(dx, dx, dy * dy);
                 float ri[PZ], ev[PZ];
@@ -518,7 +571,7 @@ __global__ __launch_bounds__(kElecThreads) void k_elec(ElecArgs a)
                     const float r2 = __builtin_fmaf(dz, dz, dxy2);
                     ri[k] = __builtin_amdgcn_rsqf(r2);
                     bin[k] = unsigned(r2 * ri[k]);
-                    if (kClamp) bin[k] = min(bin[k], unsigned(a.tableLen - 1));
+                    if (kClamp) bin[k] = min(bin[k], clampMax);
                 }
 #pragma unroll
                 for (int k = 0; k < PZ; ++k) ev[k] = eps[bin[k]];
@@ -531,9 +584,9 @@ __global__ __launch_bounds__(kElecThreads) void k_elec(ElecArgs a)
                 }
             }
         } else {
-            const double *T64 = a.tiles64 + int64_t(t) * kTile * 3;
-            for (int j = 0; j < kTile; ++j) {
-                const float4 at = T[j];
+            for (int j = j0; j < j1; ++j) {
+                const float4 at = a.tiles[j];
+                const double *T64 = a.tiles64 + int64_t(j) * 3;
                 const float dx = at.x - gx, dy = at.y - gy;
                 const float dxy2 = __builtin_fmaf(dx, dx, dy * dy);
 #pragma unroll
@@ -544,14 +597,14 @@ __global__ __launch_bounds__(kElecThreads) void k_elec(ElecArgs a)
                     const bool amb = !include && r2 >= a.r2Lo;
                     if (amb) {
                         // exact FP64 predicate of the cutoff kernel: r^2 <= 64 is near
-                        const double ddx = T64[3 * j] - (ix - a.g.dx2) * a.g.spacing;
-                        const double ddy = T64[3 * j + 1] - (iy - a.g.dy2) * a.g.spacing;
-                        const double ddz = T64[3 * j + 2] - (iz0 + k - a.g.dz2) * a.g.spacing;
+                        const double ddx = T64[0] - (ix - a.g.dx2) * a.g.spacing;
+                        const double ddy = T64[1] - (iy - a.g.dy2) * a.g.spacing;
+                        const double ddz = T64[2] - (iz0 + k - a.g.dz2) * a.g.spacing;
                         include = !(ddx * ddx + ddy * ddy + ddz * ddz <= 64.0);
                     }
                     const float ri = __builtin_amdgcn_rsqf(r2);
                     unsigned bin = unsigned(r2 * ri);
-                    bin = min(bin, unsigned(a.tableLen - 1));
+                    bin = min(bin, clampMax);
                     const float v = __builtin_fmaf(at.w * ri, eps[bin], part[k]);
                     part[k] = include ? v : part[k];
                 }
@@ -578,6 +631,111 @@ __global__ __launch_bounds__(kElecThreads) void k_elec(ElecArgs a)
         const int64_t p = (int64_t(iz) * a.g.ny + iy) * a.g.nx + ix;
         if (kComp) a.e[p] = a.e[p] + (double(hi[k]) + double(lo[k]));
         else a.e[p] = double(part[k]);
+    }
+}
+
+// One job: one workgroup per brick.
+template <int PZ, bool kComp, bool kClamp>
+__global__ __launch_bounds__(kElecThreads) void k_elec(ElecArgs a)
+{
+    extern __shared__ float eps[];
+    loadEps(eps, a.eps32, a.tableLen);
+    __syncthreads();
+    elecBrick<PZ, kComp, kClamp>(a, int(blockIdx.x), eps);
+}
+
+// ------------------------------------------------------------------------
+// Batches: many jobs in one launch.  Work items are (job, brick) pairs,
+// numbered job after job; base[j] is the first item of job j and
+// base[nJobs] the total.  The persistent kernels take items from a counter
+// until none are left; the counter is cleared by k_fill_i32 before them.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ int jobOfItem(const int32_t *base, int nJobs, int item)
+{
+    int lo = 0, hi = nJobs - 1;  // last job with base <= item
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Next work item of the workgroup (every lane gets the same value).
+__device__ __forceinline__ int nextItem(int *counter, int *slot)
+{
+    __syncthreads();  // every lane has read the previous item
+    if (threadIdx.x == 0) *slot = atomicAdd(counter, 1);
+    __syncthreads();
+    return *slot;
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int32_t *p, int n, int32_t v)
+{
+    const int i = int(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// sharedTableLen > 0: every job uses the same table (entries past 8191
+// repeat entry 8191), loaded once from jobs[0] with that length; else each
+// job's table is loaded when the workgroup moves to it.
+template <int PZ, bool kComp, bool kClamp>
+__global__ __launch_bounds__(kElecThreads) void k_elec_batch(const ElecArgs *jobs, const int32_t *base, int nJobs,
+                                                             int *counter, int sharedTableLen)
+{
+    extern __shared__ float eps[];
+    __shared__ int slot;
+    const int total = base[nJobs];
+    int loaded = -1;
+    if (sharedTableLen > 0) loadEps(eps, jobs[0].eps32, sharedTableLen);
+    for (;;) {
+        const int item = nextItem(counter, &slot);
+        if (item >= total) break;
+        const int job = uniform(jobOfItem(base, nJobs, item));
+        const ElecArgs &a = jobs[job];
+        if (sharedTableLen <= 0 && job != loaded) {
+            loadEps(eps, a.eps32, a.tableLen);
+            loaded = job;
+        }
+        __syncthreads();  // table loaded
+        elecBrick<PZ, kComp, kClamp>(a, item - base[job], eps);
+    }
+}
+
+// Candidate lists of every job: one workgroup per (job, brick) item.
+template <bool kFill>
+__global__ __launch_bounds__(kListThreads) void k_brick_lists_batch(const ListArgs *jobs, const int32_t *base,
+                                                                   int nJobs)
+{
+    __shared__ ListLds S;
+    const int item = int(blockIdx.x);
+    const int job = uniform(jobOfItem(base, nJobs, item));
+    brickList<kFill>(jobs[job], item - base[job], S);
+}
+
+// Both scans of every job: one workgroup per job.
+__global__ __launch_bounds__(1024) void k_scan_counts_batch(const ListArgs *jobs)
+{
+    __shared__ int32_t part[1024];
+    const ListArgs &a = jobs[blockIdx.x];
+    const int nb = a.nbx * a.nby * a.nbz;
+    scanCounts(a.counts, nb, 0, const_cast<int32_t *>(a.start), part);
+    __syncthreads();
+    scanCounts(a.counts, nb, 1, const_cast<int32_t *>(a.hStart), part);
+}
+
+template <int MM, int KH, bool kAffinity, bool kPointAng>
+__global__ __launch_bounds__(kCutThreads) void k_cutoff_batch(const CutoffArgs *jobs, const int32_t *base, int nJobs,
+                                                              int *counter)
+{
+    __shared__ CutoffLds<MM, kAffinity> S;
+    __shared__ int slot;
+    const int total = base[nJobs];
+    for (;;) {
+        const int item = nextItem(counter, &slot);
+        if (item >= total) break;
+        const int job = uniform(jobOfItem(base, nJobs, item));
+        cutoffBrick<MM, KH, kAffinity, kPointAng>(jobs[job], item - base[job], S);
     }
 }
 

@@ -10,8 +10,9 @@
 //    reference's operation order (no contraction), so results match the
 //    reference CPU path (FG/CalculateGridmaps.cpp:84-316).
 //  * k_elec: far-field electrostatics in FP32.  One workgroup per brick of
-//    16x16xPZ points; each lane owns PZ points along z.  Receptor atoms are
-//    spatially sorted into tiles of kTile atoms; each tile's contribution is
+//    256 lanes (16x16, 8x32 or 32x8 in x, y) x PZ points along z.  Receptor
+//    atoms are spatially sorted into tiles (a tuned multiple of kSubTile
+//    atoms); each tile's contribution is
 //    summed in FP32 and folded into a compensated two-float accumulator, and
 //    the final per-point value is formed in FP64.  The inverse dielectric
 //    table lives in LDS.  Pairs whose exact FP64 distance is <= 8 A belong
@@ -30,11 +31,10 @@
 namespace fgk {
 
 constexpr int kTableLen = 8192;
-constexpr int kTile = 64;          // atoms per electrostatic tile
+constexpr int kSubTile = 32;       // atoms per culling box of the electrostatic tiles
 constexpr int kCutBX = 8, kCutBY = 8, kCutBZ = 4;  // k_cutoff brick (points)
 constexpr int kCutThreads = kCutBX * kCutBY * kCutBZ;
-constexpr int kElecBX = 16, kElecBY = 16;          // k_elec brick in x, y
-constexpr int kElecThreads = kElecBX * kElecBY;
+constexpr int kElecThreads = 256;                  // k_elec lanes per brick
 
 enum { HB_NON = 0, HB_DS = 1, HB_D1 = 2, HB_AS = 3, HB_A1 = 4, HB_A2 = 5 };
 
@@ -107,14 +107,19 @@ struct CutoffArgs {
 
 struct ElecArgs {
     GridGeom g;
-    int32_t nTiles, tableLen, mode, pad;  // mode: 0 plain, 1 compensated
+    int32_t nAtoms;          // atoms padded to a multiple of kSubTile
+    int32_t tableLen;        // entries of eps32
+    int32_t tile;            // atoms per tile (a multiple of kSubTile)
+    int32_t bxShift;         // lanes along x = 1 << bxShift (8, 16 or 32); 256 lanes per brick
+    int32_t nbx, nby, nbz;   // bricks per axis
+    int32_t clampMax;        // last usable table index
     int64_t P;
-    const float4 *tiles;     // [nTiles*kTile] (x, y, z, w) scaled by 100 (w: 100*w)
-    const double *tiles64;   // [nTiles*kTile][3] FP64 coordinates (for the exact mask)
-    const float *tileBox;    // [nTiles][6] lo xyz, hi xyz (scaled)
+    const float4 *tiles;     // [nAtoms] (x, y, z, w) scaled by 100 (w: 100*w)
+    const double *tiles64;   // [nAtoms][3] FP64 coordinates (for the exact mask)
+    const float *subBox;     // [nAtoms/kSubTile][6] lo xyz, hi xyz (scaled)
     const float *eps32;      // [tableLen]
-    double *e;               // [P] electrostatic map (near part already there in mode 1)
-    float farCut2;           // (800 + margin)^2 for tile culling (scaled units)
+    double *e;               // [P] electrostatic map (near part already there when compensated)
+    float farCut2;           // (900)^2: sub-tiles farther than 9 A hold only far pairs
     float r2Lo, r2Hi;        // ambiguity band around 64e4 in scaled r^2
     float spacing100;        // 100 * spacing
 };

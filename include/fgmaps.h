@@ -43,13 +43,20 @@ extern "C" {
 /* H-bond classes, FG/ParameterLibrary.h:30-38 */
 enum { FG_HB_NON = 0, FG_HB_DS = 1, FG_HB_D1 = 2, FG_HB_AS = 3, FG_HB_A1 = 4, FG_HB_A2 = 5 };
 
-/* Precision forms of the electrostatic map (the affinity/desolvation maps are
- * always computed in the reference's FP64 arithmetic). */
+/* Computation forms.  COMPENSATED, FP64 and FP32 compute the affinity and
+ * desolvation maps in the reference's FP64 arithmetic and differ only in the
+ * electrostatic map; PLAIN is FP32 throughout. */
 enum {
-    FG_PREC_PLAIN = 0,       /* FP32 pair math, FP32 accumulation, FP32 bin decision */
-    FG_PREC_COMPENSATED = 1, /* FP32 pair math, compensated two-float cross-tile sum,
-                                FP64 final sum, exact FP64 bin decision in a guard band */
-    FG_PREC_FP64 = 2         /* FP64 everywhere, the reference's operation order */
+    FG_PREC_PLAIN = 0,       /* the reference's GPU method in FP32, one job per launch:
+                                every lane sums 8 z-points over all atoms in chunks of
+                                4000 per launch (FG/electrostatics/Cuda.cpp:75-199,
+                                cuda_internal/Kernels.inl:174-220), and the cutoff maps
+                                per point in FP32 (FG/CalculateGridmaps.cpp:199-316) */
+    FG_PREC_COMPENSATED = 1, /* FP32 far-field pair math, compensated two-float cross-tile
+                                sum, FP64 final sum; pairs within 8 A in FP64 */
+    FG_PREC_FP64 = 2,        /* FP64 everywhere, the reference's operation order */
+    FG_PREC_FP32 = 3         /* the tiled far-field kernel of COMPENSATED over all pairs
+                                with a plain FP32 sum (no compensation, no FP64 near part) */
 };
 
 /* What a job run computes. */
@@ -170,12 +177,57 @@ int fg_job_fetch(fg_ctx *ctx, fg_job *job, double *const *out);
 /* Number of (point, atom) pairs with r <= 8 A (computed on the device). */
 int fg_job_cutoff_pairs(fg_ctx *ctx, fg_job *job, int64_t *pairs);
 
-/* Many jobs in one persistent launch per map kind, captured once in a
- * hipGraph and replayed.  Jobs must have been created on ctx. */
+/* ---------------------------------------------------------------- */
+/* Electrostatic kernel shapes and the autotuner (no reference      */
+/* counterpart: the reference picks its CUDA launch shape by fixed  */
+/* rules, FG/electrostatics/Cuda.cpp:75-128)                        */
+/* ---------------------------------------------------------------- */
+/* Shape of a job's electrostatic kernel: grid points per lane along z (4 or
+ * 8), receptor atoms per tile (32, 64 or 128) and lanes of the 256-lane
+ * brick along x (8, 16 or 32). */
+typedef struct {
+    int32_t pz, tile, bx;
+} fg_shape;
+int fg_job_get_shape(const fg_job *job, fg_shape *out);
+int fg_job_set_shape(fg_job *job, const fg_shape *shape);
+
+/* Groups the jobs into size classes (atom count and point count octaves,
+ * best-padding brick width), times every shape on the median job of each
+ * class with HIP events, and gives every job its class's fastest shape.  The
+ * points per lane along z are then made common to all the jobs (a batch
+ * launches one kernel for all of them): the value with the least total
+ * modelled time wins.  precision: FG_PREC_COMPENSATED or FG_PREC_FP32.  The
+ * jobs' maps are undefined until their next run.  fg_tune_report() returns
+ * one text line per class of the last call on ctx. */
+int fg_autotune(fg_ctx *ctx, fg_job *const *jobs, int n_jobs, int precision);
+const char *fg_tune_report(const fg_ctx *ctx);
+
+/* ---------------------------------------------------------------- */
+/* Batches: many jobs in one hipGraph of persistent kernels (no     */
+/* reference counterpart: the reference runs one receptor per      */
+/* process)                                                         */
+/* ---------------------------------------------------------------- */
+typedef struct fg_batch fg_batch;
+#define FG_BATCH_NO_GRAPH 1u /* launch the kernels directly on each run (split timing) */
+
+/* Prepares one run of jobs[0..n_jobs) (created on ctx; they must outlive the
+ * batch): the candidate lists of every job (two kernels over all (job,
+ * brick) items and one scan per job), then one persistent cutoff kernel and
+ * one persistent electrostatic kernel, each taking (job, brick) items from a
+ * device counter, largest jobs first.  The sequence is captured once into a
+ * hipGraph.  precision: FG_PREC_COMPENSATED or FG_PREC_FP32.  Every job
+ * keeps its own tile size and brick width; all must share one pz. */
+int fg_batch_create(fg_ctx *ctx, fg_job *const *jobs, int n_jobs, unsigned what, int precision, unsigned flags,
+                    fg_batch **out);
+/* Enqueues one run of the batch on the context's stream; returns at once. */
+int fg_batch_launch(fg_ctx *ctx, fg_batch *batch);
+void fg_batch_destroy(fg_batch *batch);
+/* fg_batch_create + fg_batch_launch + fg_sync + fg_batch_destroy. */
 int fg_batch_run(fg_ctx *ctx, fg_job *const *jobs, int n_jobs, unsigned what, int precision);
 
-/* Device time of the kernels of the last fg_job_run / fg_batch_run on ctx,
- * measured with HIP events on the context's stream (ms). */
+/* Device time of the kernels of the last fg_job_run / fg_batch_launch on ctx,
+ * measured with HIP events on the context's stream (ms).  A batch replayed
+ * from its graph reports total_ms only (elec_ms = affinity_ms = -1). */
 typedef struct {
     double elec_ms;      /* electrostatic kernel(s) */
     double affinity_ms;  /* cutoff kernel(s), incl. the per-brick atom lists */

@@ -97,3 +97,16 @@ def variant(tmp, name):
 
 
 VARIANTS = sorted(json.load(open(os.path.join(GOLDEN, "variants", "variants.json"))))
+
+
+POCKET_DENSITY = 0.03  # atoms per A^3 of a synthetic pocket sphere
+
+
+def pocket(tmp, name, seed, n_atoms, npts, ligand_types="A C HD N OA SA"):
+    """A synthetic receptor pocket: n_atoms packed in a sphere of density
+    POCKET_DENSITY, grid of npts points (odd per axis) at 0.375 A on its
+    centroid.  Returns the prepared JobArrays."""
+    import fastgrid_amd as fg
+    radius = (3.0 * n_atoms / (4.0 * np.pi * POCKET_DENSITY)) ** (1.0 / 3.0)
+    fg.synth_write(tmp, name, seed, n_atoms, "sphere", radius, npts, ligand_types)
+    return load_arrays(tmp, name + ".gpf")

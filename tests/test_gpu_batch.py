@@ -1,0 +1,162 @@
+"""Batched form and autotuner on an MI355X.
+
+A batch runs many jobs as one hipGraph of persistent kernels.  Each grid
+point's arithmetic depends on the tile size (where the two-float fold
+happens) but not on the brick shape, the points per lane or the order in
+which workgroups take bricks, so a batch must reproduce every job's own run
+bit for bit; against the FP64 oracle the compensated bound of the one-job
+tests applies."""
+import numpy as np
+import pytest
+
+from tests import fixtures as fx
+
+pytestmark = pytest.mark.gpu
+
+# (seed, atoms, points per axis, ligand types): small to large, odd and
+# lopsided grids, one job with 8 maps among 6-map jobs
+POCKETS = [
+    (11, 600, (21, 23, 25), "A C HD N OA SA"),
+    (12, 1500, (41, 21, 61), "A C HD N OA SA"),
+    (13, 3000, (33, 55, 21), "A C HD N NA OA SA S"),
+    (14, 5000, (61, 61, 61), "A C HD N OA SA"),
+    (15, 800, (101, 21, 21), "A C HD N OA SA"),
+    (16, 9000, (45, 47, 43), "A C HD N OA SA"),
+]
+
+
+@pytest.fixture(scope="module")
+def pockets(tmp_path_factory):
+    tmp = str(tmp_path_factory.mktemp("pockets"))
+    return [fx.pocket(tmp, f"p{i}", seed, n, npts, lig) for i, (seed, n, npts, lig) in enumerate(POCKETS)]
+
+
+def _jobs(ctx, arrays):
+    import fastgrid_amd as fg
+    return [fg.Job(ctx, a) for a in arrays]
+
+
+def _single(jobs, prec, what=None):
+    import fastgrid_amd as fg
+    out = []
+    for j in jobs:
+        j.run(fg.RUN_ALL if what is None else what, prec)
+        out.append(j.fetch())
+    return out
+
+
+def _same(a, b):
+    return all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_reproduces_single_runs(ctx, pockets, graph):
+    import fastgrid_amd as fg
+    jobs = _jobs(ctx, pockets)
+    for prec in (fg.PREC_COMPENSATED, fg.PREC_FP32):
+        want = _single(jobs, prec)
+        for j in jobs:  # poison the maps: the batch must write every point
+            j.run(fg.RUN_ALL, fg.PREC_PLAIN)
+        b = fg.Batch(ctx, jobs, fg.RUN_ALL, prec, graph=graph)
+        b.launch()
+        ctx.sync()
+        for i, j in enumerate(jobs):
+            assert _same(j.fetch(), want[i]), (prec, i)
+        b.close()
+    for j in jobs:
+        j.close()
+
+
+@pytest.mark.parametrize("what", [1, 2])
+def test_batch_partial_runs(ctx, pockets, what):
+    """Electrostatic map only (with its FP64 near part) / affinity maps only."""
+    import fastgrid_amd as fg
+    jobs = _jobs(ctx, pockets[:4])
+    want = _single(jobs, fg.PREC_COMPENSATED, what)
+    b = fg.Batch(ctx, jobs, what, fg.PREC_COMPENSATED)
+    b.launch()
+    ctx.sync()
+    for i, j in enumerate(jobs):
+        got = j.fetch()
+        M = pockets[i].n_maps
+        idx = [M] if what == fg.RUN_ELEC else list(range(M)) + [M + 1]
+        for m in idx:
+            assert np.array_equal(got[m], want[i][m]), (i, m)
+    b.close()
+
+
+def test_graph_replays_agree(ctx, pockets):
+    """Replays of one graph, with other work and synchronisations between
+    them, give the eager result every time (the work counters are cleared by
+    a kernel inside the graph)."""
+    import fastgrid_amd as fg
+    jobs = _jobs(ctx, pockets[:3])
+    other = fg.Job(ctx, pockets[3])
+    want = _single(jobs, fg.PREC_COMPENSATED)
+    b = fg.Batch(ctx, jobs)
+    for rep in range(4):
+        b.launch()
+        ctx.sync()
+        for i, j in enumerate(jobs):
+            assert _same(j.fetch(), want[i]), (rep, i)
+        other.run(fg.RUN_ALL, fg.PREC_COMPENSATED)
+        ctx.sync()
+    b.close()
+
+
+def test_batch_against_oracle(ctx, pockets):
+    import fastgrid_amd as fg
+    from oracle import oracle as orc
+    jobs = _jobs(ctx, pockets)
+    fg.batch_run(ctx, jobs)
+    for a, j in zip(pockets, jobs):
+        out = j.fetch()
+        M = a.n_maps
+        om, od = orc.affinity_maps(a)
+        oe = orc.elec_map(a)
+        for m in range(M):
+            assert np.array_equal(fx.milli(out[m]), fx.milli(om[m])), m
+        assert np.array_equal(fx.milli(out[M + 1]), fx.milli(od))
+        assert np.abs(out[M] - oe).max() < 3e-4
+    for j in jobs:
+        j.close()
+
+
+def test_shapes_and_autotuner(ctx, pockets):
+    """Every shape is within the compensated bound of the oracle; shapes with
+    one tile size agree bit for bit; the autotuner gives every job a valid
+    shape and one common pz."""
+    import fastgrid_amd as fg
+    from oracle import oracle as orc
+    a = pockets[1]
+    job = fg.Job(ctx, a)
+    oe = orc.elec_map(a)
+    M = a.n_maps
+    by_tile = {}
+    for pz in (4, 8):
+        for tile in (32, 64, 128):
+            for bx in (8, 16, 32):
+                job.shape = (pz, tile, bx)
+                assert job.shape == (pz, tile, bx)
+                job.run(fg.RUN_ALL, fg.PREC_COMPENSATED)
+                e = job.fetch()[M]
+                assert np.abs(e - oe).max() < 3e-4, (pz, tile, bx)
+                if tile in by_tile:
+                    assert np.array_equal(e, by_tile[tile]), (pz, tile, bx)
+                else:
+                    by_tile[tile] = e
+    with pytest.raises(fg.FgError):
+        job.shape = (6, 64, 16)
+    jobs = _jobs(ctx, pockets)
+    report = fg.autotune(ctx, jobs)
+    print(report)
+    assert report.startswith("pz ")
+    shapes = [j.shape for j in jobs]
+    assert len({s[0] for s in shapes}) == 1
+    for pz, tile, bx in shapes:
+        assert pz in (4, 8) and tile in (32, 64, 128) and bx in (8, 16, 32)
+    # tuned jobs still reproduce through the batch
+    want = _single(jobs, fg.PREC_COMPENSATED)
+    fg.batch_run(ctx, jobs)
+    for i, j in enumerate(jobs):
+        assert _same(j.fetch(), want[i]), i
