@@ -1,27 +1,38 @@
 """Benchmark of the grid-map hot path on one MI355X.
 
-Workload: the 50,000-atom receptor (random sequential packing in a
-70 x 70 x 50 A slab, min separation 1.2 A, protein-like type/charge mix,
+Headline workload (L): the 50,000-atom receptor (random sequential packing in
+a 70 x 70 x 50 A slab, min separation 1.2 A, protein-like type/charge mix,
 seed 3) on a 127^3 grid at 0.375 A with 8 affinity maps (A C HD N NA OA SA S)
 plus the electrostatic and desolvation maps.  Inputs are synthetic and
 device-resident before timing.  One step = one full map set: per-brick
 candidate lists, the cutoff kernel (affinity + desolvation + near-field
 electrostatics, FP64) and the far-field electrostatic kernel (FP32 pair math,
-compensated two-float cross-tile sums, FP64 final sum).
+compensated two-float cross-tile sums, FP64 final sum), with the kernel shape
+picked by the autotuner before timing.
 
 Metric: Gpair-evals/s = (N*P + P_cut) / t_step, P_cut = (point, atom) pairs
-with r <= 8 A.  Also reported: seconds per full map set, the plain FP32
-one-job-per-launch form, the per-map error of both forms against the FP64
-form, the roofline of the electrostatic kernel, and the FP64 CPU oracle timed
-on a bounded sample of the same workload (cpu_baseline).
+with r <= 8 A.  Reported beside it:
+  * the plain form (the reference's GPU method in FP32, one job per launch)
+    and the tiled FP32 form without compensation, on the same inputs;
+  * per-map max / RMS error of every form against the FP64 form (the
+    reference's arithmetic on the GPU);
+  * the roofline of the electrostatic kernel (VALU-bound) with its HBM
+    traffic from the committed rocprofv3 counter summary;
+  * end-to-end seconds of the fastgrid4 command line (parse to files);
+  * the batch: 256 synthetic pockets (atoms log-uniform in 600-12000, odd
+    grid edges uniform in 21-101 per axis, 6 affinity maps) as one hipGraph of
+    persistent kernels, against 256 sequential plain launches;
+  * the FP64 CPU oracle timed on a bounded sample of L (cpu_baseline).
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-batch] [--stress]
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
+import subprocess
 import sys
 import tempfile
 import time
@@ -32,9 +43,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3      # MI355X FP32 vector peak
+HBM_PEAK_GBS = 8000.0
 FLOP_PER_ELEC_PAIR = 15       # algorithmic FLOPs of one (point, atom) Coulomb term
-WORKLOAD = dict(tag="L", seed=3, n_atoms=50000, shape="box", dims=(70.0, 70.0, 50.0), npts=(127, 127, 127),
-                ligand_types="A C HD N NA OA SA S")
+WORKLOAD_L = dict(tag="L", seed=3, n_atoms=50000, shape="box", dims=(70.0, 70.0, 50.0), npts=(127, 127, 127),
+                  ligand_types="A C HD N NA OA SA S")
+WORKLOAD_X = dict(tag="X", seed=5, n_atoms=250000, shape="box", dims=(130.0, 130.0, 90.0), npts=(161, 161, 161),
+                  ligand_types="A C HD N NA OA SA S")
+BATCH = dict(pockets=256, seed=7, atoms=(600, 12000), edges=(21, 101), ligand_types="A C HD N OA SA", density=0.03)
+# HBM bytes per launch of the electrostatic kernel on L, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE scaled by a device copy of known size
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_elec_L_hbm_traffic.json")
 
 
 def log(*a):
@@ -45,8 +63,7 @@ def r3(a):
     return np.where(np.abs(a) < 0.0005, 0.0, np.floor(a * 1000 + 0.5) * 0.001)
 
 
-def make_inputs(fg, workdir):
-    w = WORKLOAD
+def make_inputs(fg, workdir, w):
     fg.synth_write(workdir, w["tag"], w["seed"], w["n_atoms"], w["shape"], w["dims"], w["npts"], w["ligand_types"])
     cwd = os.getcwd()
     os.chdir(workdir)
@@ -74,8 +91,17 @@ def time_steps(fg, ctx, job, steps, warmup, precision):
     return t, float(np.mean(elec)), float(np.mean(aff))
 
 
+def map_errors(names, out, ref):
+    res = {}
+    for m, name in enumerate(names):
+        d = np.abs(out[m] - ref[m])
+        res[name] = {"max_abs": float(d.max()), "rms": float(np.sqrt((d * d).mean())),
+                     "max_abs_rounded": float(np.abs(r3(out[m]) - r3(ref[m])).max())}
+    return res
+
+
 def cpu_baseline(arrays, planes=12):
-    """FP64 oracle (OpenMP, all host cores) on the first `planes` z-planes."""
+    """FP64 oracle (OpenMP, all host cores) on `planes` central z-planes."""
     from oracle import oracle as orc
     import copy
     a = copy.copy(arrays)
@@ -92,10 +118,25 @@ def cpu_baseline(arrays, planes=12):
     t2 = time.perf_counter()
     pcut = orc.cutoff_pairs(a, threads=threads)
     pairs = a.n_atoms * a.P + pcut
-    return dict(value=pairs / (t2 - t0) / 1e9, unit="Gpair-evals/s", cores=threads, kind="port",
-                sample=f"FP64 oracle, {planes} central z-planes of the 127^3 grid ({a.P} points, all {a.n_atoms} atoms): "
-                       f"e-map {t1 - t0:.2f} s, affinity+d maps {t2 - t1:.2f} s",
-                seconds_per_map_set_extrapolated=(t2 - t0) * nz0 / planes)
+    return dict(value=round(pairs / (t2 - t0) / 1e9, 3), unit="Gpair-evals/s", cores=threads, kind="port",
+                sample=f"FP64 oracle (the reference's CPU algorithm restated in C, OpenMP), {planes} central z-planes "
+                       f"of the 127^3 grid ({a.P} points, all {a.n_atoms} atoms): e-map {t1 - t0:.2f} s, "
+                       f"affinity+d maps {t2 - t1:.2f} s",
+                seconds_per_map_set_extrapolated=round((t2 - t0) * nz0 / planes, 2))
+
+
+def end_to_end(fg, workdir, tag):
+    """fastgrid4 from GPF to written .map/.fld files (wall clock, second run)."""
+    exe = os.path.join(fg.BIN_DIR, "fastgrid4")
+    times = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        p = subprocess.run([exe, "-p", tag + ".gpf", "-l", tag + ".glg"], cwd=workdir, stdout=subprocess.PIPE,
+                           stderr=subprocess.PIPE, text=True, timeout=600)
+        times.append(time.perf_counter() - t0)
+        if p.returncode != 0:
+            raise RuntimeError(p.stderr)
+    return round(times[-1], 3)
 
 
 def copy_calibration(nbytes=1 << 30):
@@ -115,12 +156,160 @@ def copy_calibration(nbytes=1 << 30):
     print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
 
 
+def traffic_entry(shape):
+    """HBM bytes per electrostatic launch on L from the committed counter
+    summary, when it was measured with this kernel shape."""
+    try:
+        t = json.load(open(TRAFFIC_FILE))
+    except (OSError, ValueError):
+        return None
+    if tuple(t.get("shape", ())) != tuple(shape):
+        return None
+    return t
+
+
+def pocket_specs(n):
+    rng = np.random.default_rng(BATCH["seed"])
+    lo, hi = BATCH["atoms"]
+    edges = np.arange(BATCH["edges"][0], BATCH["edges"][1] + 1, 2)
+    specs = []
+    for i in range(n):
+        atoms = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        npts = tuple(int(v) for v in rng.choice(edges, 3))
+        specs.append((1000 + i, atoms, npts))
+    return specs
+
+
+def run_batch(fg, ctx, steps, warmup, n_pockets):
+    """The 256-pocket batch: one hipGraph of persistent kernels against one
+    plain launch per pocket (and one compensated launch per pocket)."""
+    with tempfile.TemporaryDirectory() as wd:
+        arrays = []
+        for i, (seed, atoms, npts) in enumerate(pocket_specs(n_pockets)):
+            radius = (3.0 * atoms / (4.0 * math.pi * BATCH["density"])) ** (1.0 / 3.0)
+            fg.synth_write(wd, f"p{i}", seed, atoms, "sphere", radius, npts, BATCH["ligand_types"])
+            cwd = os.getcwd()
+            os.chdir(wd)
+            try:
+                arrays.append(fg.Prepared(f"p{i}.gpf").arrays())
+            finally:
+                os.chdir(cwd)
+    jobs = [fg.Job(ctx, a) for a in arrays]
+    pairs = sum(a.n_atoms * a.P for a in arrays)
+    pcut = sum(j.cutoff_pairs() for j in jobs)
+    log(f"batch: {n_pockets} pockets, sum N*P = {pairs:.4e}, sum P_cut = {pcut:.4e}")
+    t0 = time.perf_counter()
+    report = fg.autotune(ctx, jobs)
+    t_tune = time.perf_counter() - t0
+    log(report.splitlines()[0], f"({t_tune:.1f} s)")
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        ctx.sync()
+        return (time.perf_counter() - t0) / steps
+
+    batch = fg.Batch(ctx, jobs, fg.RUN_ALL, fg.PREC_COMPENSATED)
+    t_batch = timed(batch.launch)
+    log(f"batched compensated: {t_batch * 1e3:.2f} ms per batch")
+
+    def seq(prec):
+        def go():
+            for j in jobs:
+                j.run(fg.RUN_ALL, prec)
+        return go
+
+    t_plain = timed(seq(fg.PREC_PLAIN))
+    log(f"sequential plain: {t_plain * 1e3:.2f} ms")
+    t_seq = timed(seq(fg.PREC_COMPENSATED))
+    log(f"sequential compensated: {t_seq * 1e3:.2f} ms")
+
+    # errors against the FP64 form, per map kind over all pockets
+    batch.launch()
+    ctx.sync()
+    names = None
+    worst = {"compensated": {}, "plain": {}}
+    for a, j in zip(arrays, jobs):
+        comp = j.fetch()
+        j.run(fg.RUN_ALL, fg.PREC_FP64)
+        ref = j.fetch()
+        j.run(fg.RUN_ALL, fg.PREC_PLAIN)
+        plain = j.fetch()
+        names = BATCH["ligand_types"].split() + ["e", "d"]
+        for form, out in (("compensated", comp), ("plain", plain)):
+            for name, e in map_errors(names, out, ref).items():
+                w = worst[form].setdefault(name, {"max_abs": 0.0, "max_abs_rounded": 0.0})
+                w["max_abs"] = max(w["max_abs"], e["max_abs"])
+                w["max_abs_rounded"] = max(w["max_abs_rounded"], e["max_abs_rounded"])
+    batch.close()
+    for j in jobs:
+        j.close()
+    total = pairs + pcut
+    return {
+        "workload": f"{n_pockets} synthetic pockets: atoms log-uniform in {BATCH['atoms']}, odd grid edges uniform "
+                    f"in {BATCH['edges']} per axis at 0.375 A, sphere of density {BATCH['density']} A^-3, "
+                    f"6 affinity maps ({BATCH['ligand_types']}) + e + d, seed {BATCH['seed']}",
+        "elec_pairs": pairs, "cutoff_pairs": pcut,
+        "batched": {"ms_per_batch": round(t_batch * 1e3, 3), "pockets_per_s": round(n_pockets / t_batch, 1),
+                    "gpair_evals_per_s": round(total / t_batch / 1e9, 2),
+                    "form": "compensated, one hipGraph of persistent kernels, autotuned shapes"},
+        "sequential_plain": {"ms_per_batch": round(t_plain * 1e3, 3), "pockets_per_s": round(n_pockets / t_plain, 1),
+                             "gpair_evals_per_s": round(total / t_plain / 1e9, 2)},
+        "sequential_compensated": {"ms_per_batch": round(t_seq * 1e3, 3),
+                                   "pockets_per_s": round(n_pockets / t_seq, 1),
+                                   "gpair_evals_per_s": round(total / t_seq / 1e9, 2)},
+        "speedup_vs_sequential_plain": round(t_plain / t_batch, 2),
+        "speedup_vs_sequential_compensated": round(t_seq / t_batch, 2),
+        "autotune_s": round(t_tune, 2),
+        "autotune_report": report.splitlines()[0],
+        "max_error_vs_fp64": worst,
+    }
+
+
+def run_workload(fg, ctx, w, steps, warmup, tune=True):
+    """One large job: FP64 form (error reference), plain, FP32 and
+    compensated forms.  Returns a dict of results and the job arrays."""
+    with tempfile.TemporaryDirectory() as wd:
+        log("generating", w)
+        prep, a = make_inputs(fg, wd, w)
+    N, P, M = a.n_atoms, a.P, a.n_maps
+    job = fg.Job(ctx, a)
+    pcut = job.cutoff_pairs()
+    elec_pairs = N * P
+    log(f"N={N} P={P} maps={M}+2 N*P={elec_pairs:.4e} P_cut={pcut:.4e}")
+    job.run(fg.RUN_ALL, fg.PREC_FP64)
+    ctx.sync()
+    ref = job.fetch()
+    t64 = ctx.timing()
+    log(f"fp64 form: elec {t64.elec_ms:.1f} ms")
+    report = fg.autotune(ctx, [job]) if tune else ""
+    shape = job.shape
+    log("tuned shape (pz, tile, bx):", shape)
+    names = prep.map_types()[:M] + ["e", "d"]
+    res = {"N": N, "P": P, "M": M, "elec_pairs": elec_pairs, "cutoff_pairs": pcut, "fp64_elec_ms": t64.elec_ms,
+           "shape": shape, "tune_report": report}
+    for key, prec in (("plain", fg.PREC_PLAIN), ("fp32", fg.PREC_FP32), ("compensated", fg.PREC_COMPENSATED)):
+        t, el, af = time_steps(fg, ctx, job, steps, warmup, prec)
+        res[key] = {"t": t, "elec_ms": el, "cutoff_ms": af, "errors": map_errors(names, job.fetch(), ref)}
+        log(f"{key}: {t * 1e3:.2f} ms/step (elec {el:.2f}, cutoff+lists {af:.2f})")
+    job.close()
+    return res, a
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-pocket batch")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end command-line timing")
+    ap.add_argument("--pockets", type=int, default=BATCH["pockets"])
+    ap.add_argument("--stress", action="store_true", help="also run the 250k-atom / 161^3 stress size")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (reference dispatch for HBM byte counters)")
     args = ap.parse_args()
@@ -133,43 +322,14 @@ def main():
         sys.exit(2)
 
     import fastgrid_amd as fg
-    with tempfile.TemporaryDirectory() as wd:
-        log("generating", WORKLOAD)
-        prep, a = make_inputs(fg, wd)
-    N, P, M = a.n_atoms, a.P, a.n_maps
     ctx = fg.Context(0)
-    job = fg.Job(ctx, a)
-    pcut = job.cutoff_pairs()
-    elec_pairs = N * P
-    log(f"N={N} P={P} maps={M}+2 N*P={elec_pairs:.4e} P_cut={pcut:.4e}")
-
-    # FP64 form: the reference's arithmetic on the GPU (error reference)
-    job.run(fg.RUN_ALL, fg.PREC_FP64)
-    ctx.sync()
-    ref = job.fetch()
-    t64 = ctx.timing()
-    log(f"fp64 form: elec {t64.elec_ms:.1f} ms")
-
-    t_plain, elec_plain, aff_plain = time_steps(fg, ctx, job, args.steps, args.warmup, fg.PREC_PLAIN)
-    plain = job.fetch()
-    log(f"plain: {t_plain * 1e3:.2f} ms/step (elec {elec_plain:.2f}, cutoff {aff_plain:.2f})")
-    t_comp, elec_comp, aff_comp = time_steps(fg, ctx, job, args.steps, args.warmup, fg.PREC_COMPENSATED)
-    comp = job.fetch()
-    log(f"compensated: {t_comp * 1e3:.2f} ms/step (elec {elec_comp:.2f}, cutoff {aff_comp:.2f})")
-
-    def errs(out):
-        res = {}
-        names = prep.map_types()[:M] + ["e", "d"]
-        for m, name in enumerate(names):
-            d = np.abs(out[m] - ref[m])
-            res[name] = {"max_abs": float(d.max()), "rms": float(np.sqrt((d * d).mean())),
-                         "max_abs_rounded": float(np.abs(r3(out[m]) - r3(ref[m])).max())}
-        return res
-
-    err_comp, err_plain = errs(comp), errs(plain)
-    value = (elec_pairs + pcut) / t_comp / 1e9
-    far_pairs = elec_pairs - pcut
-    achieved = FLOP_PER_ELEC_PAIR * far_pairs / (elec_comp * 1e-3) / 1e12
+    L, a = run_workload(fg, ctx, WORKLOAD_L, args.steps, args.warmup)
+    comp, plain, fp32 = L["compensated"], L["plain"], L["fp32"]
+    tot_pairs = L["elec_pairs"] + L["cutoff_pairs"]
+    value = tot_pairs / comp["t"] / 1e9
+    far_pairs = L["elec_pairs"] - L["cutoff_pairs"]
+    achieved = FLOP_PER_ELEC_PAIR * far_pairs / (comp["elec_ms"] * 1e-3) / 1e12
+    tr = traffic_entry(L["shape"])
     line = {
         "metric": "Gpair-evals/s + s per full map set (127^3 grid, 50k atoms) vs plain GPU & CPU",
         "value": round(value, 2),
@@ -177,7 +337,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(t_comp * 1e3, 3),
+        "ms_per_step": round(comp["t"] * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -185,20 +345,55 @@ def main():
         "data": "synthetic (seeded random-sequential-packing receptor, AD4.1 default parameters)",
         "config": {
             "workload": "L: 50,000-atom 70x70x50 A slab, 127^3 grid @ 0.375 A, 8 affinity maps + e + d",
-            "n_atoms": N, "grid": list(a.n), "affinity_maps": M, "precision": "compensated",
-            "elec_pairs": elec_pairs, "cutoff_pairs": pcut,
+            "n_atoms": L["N"], "grid": list(WORKLOAD_L["npts"]), "affinity_maps": L["M"],
+            "precision": "compensated (FP32 far-field pairs, two-float cross-tile sums, FP64 final sum; "
+                         "pairs within 8 A and the affinity/desolvation maps in FP64)",
+            "elec_pairs": L["elec_pairs"], "cutoff_pairs": L["cutoff_pairs"],
+            "elec_kernel_shape": {"pz": L["shape"][0], "tile": L["shape"][1], "bx": L["shape"][2]},
         },
-        "seconds_per_map_set": round(t_comp, 5),
-        "plain_gpu": {"value": round((elec_pairs + pcut) / t_plain / 1e9, 2), "unit": "Gpair-evals/s",
-                      "ms_per_step": round(t_plain * 1e3, 3), "elec_kernel_ms": round(elec_plain, 3)},
-        "kernel_ms": {"elec_far": round(elec_comp, 3), "cutoff_and_lists": round(aff_comp, 3),
-                      "elec_fp64_form": round(t64.elec_ms, 3)},
-        "error_vs_fp64": {"compensated": err_comp, "plain": err_plain},
+        "seconds_per_map_set": round(comp["t"], 5),
+        "plain_gpu": {"value": round(tot_pairs / plain["t"] / 1e9, 2), "unit": "Gpair-evals/s",
+                      "ms_per_step": round(plain["t"] * 1e3, 3), "elec_ms": round(plain["elec_ms"], 3),
+                      "cutoff_ms": round(plain["cutoff_ms"], 3),
+                      "form": "the reference's GPU method in FP32 (8 z-points per lane, 4000 atoms per launch, "
+                              "global dielectric table) and a per-point FP32 cutoff kernel"},
+        "speedup_vs_plain_gpu": round(plain["t"] / comp["t"], 2),
+        "fp32_tiled": {"value": round(tot_pairs / fp32["t"] / 1e9, 2), "ms_per_step": round(fp32["t"] * 1e3, 3),
+                       "elec_ms": round(fp32["elec_ms"], 3)},
+        "kernel_ms": {"elec_far": round(comp["elec_ms"], 3), "cutoff_and_lists": round(comp["cutoff_ms"], 3),
+                      "elec_fp64_form": round(L["fp64_elec_ms"], 3)},
+        "error_vs_fp64": {"compensated": comp["errors"], "fp32_tiled": fp32["errors"], "plain": plain["errors"]},
         "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                     "traffic": tr["bytes_per_launch"] if tr else None,
                      "kernel": "k_elec (compensated far field)",
-                     "flops_per_launch": FLOP_PER_ELEC_PAIR * far_pairs},
+                     "flops_per_launch": FLOP_PER_ELEC_PAIR * far_pairs,
+                     "hbm_gbs": round(tr["bytes_per_launch"] / (comp["elec_ms"] * 1e-3) / 1e9, 1) if tr else None,
+                     "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if tr else None},
     }
+    if not args.no_e2e:
+        with tempfile.TemporaryDirectory() as wd:
+            w = WORKLOAD_L
+            fg.synth_write(wd, w["tag"], w["seed"], w["n_atoms"], w["shape"], w["dims"], w["npts"], w["ligand_types"])
+            line["seconds_per_map_set_end_to_end"] = end_to_end(fg, wd, w["tag"])
+            log("end to end:", line["seconds_per_map_set_end_to_end"], "s")
+    if not args.no_batch:
+        line["batch"] = run_batch(fg, ctx, max(1, args.steps // 2), 1, args.pockets)
+    if args.stress:
+        X, _ = run_workload(fg, ctx, WORKLOAD_X, max(1, args.steps // 5), 1)
+        xt = X["elec_pairs"] + X["cutoff_pairs"]
+        line["stress"] = {
+            "workload": "X: 250,000-atom 130x130x90 A box, 161^3 grid @ 0.375 A, 8 affinity maps + e + d",
+            "compensated": {"value": round(xt / X["compensated"]["t"] / 1e9, 2),
+                            "ms_per_step": round(X["compensated"]["t"] * 1e3, 2),
+                            "elec_ms": round(X["compensated"]["elec_ms"], 2),
+                            "roofline_frac": round(FLOP_PER_ELEC_PAIR * (X["elec_pairs"] - X["cutoff_pairs"]) /
+                                                   (X["compensated"]["elec_ms"] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},
+            "plain": {"value": round(xt / X["plain"]["t"] / 1e9, 2), "ms_per_step": round(X["plain"]["t"] * 1e3, 2)},
+            "fp32_tiled": {"value": round(xt / X["fp32"]["t"] / 1e9, 2)},
+            "error_vs_fp64": {k: X[k]["errors"] for k in ("compensated", "fp32", "plain")},
+            "shape": X["shape"],
+        }
     if not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(a)

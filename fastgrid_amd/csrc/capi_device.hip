@@ -48,6 +48,10 @@ struct fg_ctx {
     bool haveTiming = false;
     bool splitTiming = true;  // ev[1] separates the cutoff and electrostatic phases
     std::string tuneReport;
+    // live jobs and batches: fg_destroy detaches them, so destroying them
+    // afterwards only frees their memory
+    std::vector<fg_job *> jobs;
+    std::vector<fg_batch *> batches;
 };
 
 struct fg_job {
@@ -221,6 +225,7 @@ fgk::ElecArgs elecArgs(const fg_job *j, int pz, int tile, int bxShift)
     a.nby = (j->g.ny + byl - 1) / byl;
     a.nbz = (j->g.nz + pz - 1) / pz;
     a.clampMax = j->tableLen - 1;
+    a.floorBitsNeg4 = int32_t(0u - 4u * fgk::kFloorBits);
     a.P = j->P;
     a.tiles = j->dTiles;
     a.tiles64 = j->dTiles64;
@@ -622,9 +627,16 @@ int runPlainElec(fg_job *j)
     return FG_OK;
 }
 
+template <class T>
+void forget(std::vector<T *> &v, T *x)
+{
+    v.erase(std::remove(v.begin(), v.end(), x), v.end());
+}
+
 void destroyJob(fg_job *j)
 {
     if (!j) return;
+    if (j->ctx) forget(j->ctx->jobs, j);
     for (void *p : j->owned) (void)hipFree(p);
     delete j;
 }
@@ -778,7 +790,10 @@ namespace {
 void destroyBatch(fg_batch *b)
 {
     if (!b) return;
-    if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
+    if (b->ctx) {
+        (void)hipStreamSynchronize(b->ctx->stream);
+        forget(b->ctx->batches, b);
+    }
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (void *p : {static_cast<void *>(b->dList), static_cast<void *>(b->dCut), static_cast<void *>(b->dElec),
@@ -1004,6 +1019,8 @@ void fg_destroy(fg_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (fg_job *j : ctx->jobs) j->ctx = nullptr;
+    for (fg_batch *b : ctx->batches) b->ctx = nullptr;
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1109,6 +1126,7 @@ int fg_job_create(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
         destroyJob(j);
         return rc;
     }
+    ctx->jobs.push_back(j);
     *out = j;
     return FG_OK;
 }
@@ -1276,6 +1294,7 @@ int fg_batch_create(fg_ctx *ctx, fg_job *const *jobs, int n, unsigned what, int 
         destroyBatch(b);
         return rc;
     }
+    ctx->batches.push_back(b);
     *out = b;
     return FG_OK;
 }

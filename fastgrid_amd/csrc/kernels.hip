@@ -508,6 +508,9 @@ __global__ __launch_bounds__(kCutThreads) void k_cutoff(CutoffArgs a)
 // dielectric table directly and w is stored as 100*w.
 // ------------------------------------------------------------------------
 
+constexpr float kFloorBias = 8388607.5f;       // 2^23 - 1/2
+constexpr unsigned kFloorBits = 0x4B000000u;   // bit pattern of 2^23
+
 // Loads the dielectric table into LDS (all lanes of the workgroup).
 __device__ __forceinline__ void loadEps(float *eps, const float *src, int n)
 {
@@ -535,6 +538,11 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
     const float blo[3] = {float(ix0 - a.g.dx2) * s100, float(iy0 - a.g.dy2) * s100, float(iz0 - a.g.dz2) * s100};
     const float bhi[3] = {float(bx1 - a.g.dx2) * s100, float(by1 - a.g.dy2) * s100, float(bz1 - a.g.dz2) * s100};
     const unsigned clampMax = unsigned(a.clampMax);
+    const unsigned clampBits = kFloorBits + clampMax;
+    // byte offset that turns a biased bit pattern into a table address; read
+    // from the arguments so that it stays in a scalar register
+    const unsigned biasBytes = unsigned(a.floorBitsNeg4);
+    const char *epsBytes = reinterpret_cast<const char *>(eps);
 
     float hi[PZ], lo[PZ], part[PZ];
 #pragma unroll
@@ -570,11 +578,24 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
                     const float dz = at.z - gz[k];
                     const float r2 = __builtin_fmaf(dz, dz, dxy2);
                     ri[k] = __builtin_amdgcn_rsqf(r2);
-                    bin[k] = unsigned(r2 * ri[k]);
-                    if (kClamp) bin[k] = min(bin[k], clampMax);
+                    if (kComp) {
+                        // floor(100 r) in one operation: r2 * ri + (2^23 - 1/2)
+                        // lands in [2^23, 2^24), where the FP32 spacing is 1,
+                        // so its bit pattern is 2^23's plus round(100 r - 1/2)
+                        // (far pairs have 100 r >= 900)
+                        bin[k] = __float_as_uint(__builtin_fmaf(r2, ri[k], kFloorBias));
+                        if (kClamp) bin[k] = min(bin[k], clampBits);
+                    } else {
+                        bin[k] = unsigned(r2 * ri[k]);
+                        if (kClamp) bin[k] = min(bin[k], clampMax);
+                    }
                 }
 #pragma unroll
-                for (int k = 0; k < PZ; ++k) ev[k] = eps[bin[k]];
+                for (int k = 0; k < PZ; ++k) {
+                    // eps[bits - 2^23's bits] as one shift-add with a uniform byte offset
+                    ev[k] = kComp ? *reinterpret_cast<const float *>(epsBytes + ((bin[k] << 2) + biasBytes))
+                                  : eps[bin[k]];
+                }
 #pragma unroll
                 for (int k = 0; k < PZ; ++k) {
                     // min(1/r, 2) in scaled units; the far-field pairs of the
@@ -602,11 +623,15 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
                         const double ddz = T64[2] - (iz0 + k - a.g.dz2) * a.g.spacing;
                         include = !(ddx * ddx + ddy * ddy + ddz * ddz <= 64.0);
                     }
+                    // every lane does the arithmetic and excluded pairs add
+                    // 0; the bin is the far loop's, clamped into the table
+                    // for the excluded ones (r = 0 gives a NaN pattern)
                     const float ri = __builtin_amdgcn_rsqf(r2);
-                    unsigned bin = unsigned(r2 * ri);
-                    bin = min(bin, clampMax);
-                    const float v = __builtin_fmaf(at.w * ri, eps[bin], part[k]);
-                    part[k] = include ? v : part[k];
+                    unsigned bin = __float_as_uint(__builtin_fmaf(r2, ri, kFloorBias));
+                    bin = min(max(bin, kFloorBits), clampBits);
+                    const float ris = include ? ri : 0.f;
+                    const float ev = *reinterpret_cast<const float *>(epsBytes + ((bin << 2) + biasBytes));
+                    part[k] = __builtin_fmaf(at.w * ris, ev, part[k]);
                 }
             }
         }

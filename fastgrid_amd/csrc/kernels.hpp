@@ -113,6 +113,8 @@ struct ElecArgs {
     int32_t bxShift;         // lanes along x = 1 << bxShift (8, 16 or 32); 256 lanes per brick
     int32_t nbx, nby, nbz;   // bricks per axis
     int32_t clampMax;        // last usable table index
+    int32_t floorBitsNeg4;   // -4 * bit pattern of 2^23 (mod 2^32): the table's byte offset for biased bins
+    int32_t pad;
     int64_t P;
     const float4 *tiles;     // [nAtoms] (x, y, z, w) scaled by 100 (w: 100*w)
     const double *tiles64;   // [nAtoms][3] FP64 coordinates (for the exact mask)
