@@ -74,6 +74,10 @@ struct fg_job {
     fgk::CellGrid cg{};
     int nbx = 0, nby = 0, nbz = 0, nb = 0;
     int32_t *dCounts = nullptr, *dStart = nullptr, *dHStart = nullptr, *dList = nullptr, *dHList = nullptr;
+    // type-sorted candidate lists for the typed cutoff kernel (pair-table rows in LDS)
+    bool typed = false;
+    int32_t addOnlyClasses = 0;  // bit h: H-bond class h only adds (see CutoffArgs)
+    int32_t *dType = nullptr, *dSorted = nullptr, *dTypeStart = nullptr;
     // electrostatic tiles
     float4 *dTiles = nullptr;
     double *dTiles64 = nullptr;
@@ -155,6 +159,10 @@ fgk::ListArgs listArgs(const fg_job *j)
     la.hStart = j->dHStart;
     la.list = j->dList;
     la.hList = j->dHList;
+    la.atomType = j->dType;
+    la.sorted = j->dSorted;
+    la.typeStart = j->dTypeStart;
+    la.T = j->T;
     return la;
 }
 
@@ -172,9 +180,12 @@ fgk::CutoffArgs cutoffArgs(const fg_job *j, int doElecNear)
     a.atoms = j->dAtoms;
     a.ab = j->dAB;
     a.brickStart = j->dStart;
-    a.brickList = j->dList;
+    a.brickList = j->typed ? j->dSorted : j->dList;
     a.hStart = j->dHStart;
     a.hList = j->dHList;
+    a.typeStart = j->dTypeStart;
+    a.T = j->T;
+    a.addOnlyClasses = j->addOnlyClasses;
     a.pair = j->dPair;
     a.desolv = j->dDesolv;
     a.eps = j->dEps;
@@ -184,30 +195,51 @@ fgk::CutoffArgs cutoffArgs(const fg_job *j, int doElecNear)
     return a;
 }
 
-template <int MM, bool PA>
+template <int MM, bool PA, bool TY>
 void launchCutoffKH(fg_job *j, const fgk::CutoffArgs &a, dim3 grid, dim3 block)
 {
     hipStream_t s = j->ctx->stream;
-    if (j->KH == 0) hipLaunchKernelGGL((fgk::k_cutoff<MM, 0, true, PA>), grid, block, 0, s, a);
-    else if (j->KH <= 2) hipLaunchKernelGGL((fgk::k_cutoff<MM, 2, true, PA>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((fgk::k_cutoff<MM, 4, true, PA>), grid, block, 0, s, a);
+    if (j->KH == 0) hipLaunchKernelGGL((fgk::k_cutoff<MM, 0, true, PA, TY>), grid, block, 0, s, a);
+    else if (j->KH <= 2) hipLaunchKernelGGL((fgk::k_cutoff<MM, 2, true, PA, TY>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((fgk::k_cutoff<MM, 4, true, PA, TY>), grid, block, 0, s, a);
 }
 
-template <int MM>
+template <int MM, bool TY>
 void launchCutoffM(fg_job *j, const fgk::CutoffArgs &a, dim3 grid, dim3 block)
 {
-    if (j->pointAng) launchCutoffKH<MM, true>(j, a, grid, block);
-    else launchCutoffKH<MM, false>(j, a, grid, block);
+    if (j->pointAng) launchCutoffKH<MM, true, TY>(j, a, grid, block);
+    else launchCutoffKH<MM, false, TY>(j, a, grid, block);
+}
+
+// Candidate lists of the job (count, scans, fill, and the type sort when the
+// typed kernel runs).
+int runLists(fg_job *j, bool affinity)
+{
+    fgk::ListArgs la = listArgs(j);
+    if (int rc = runListCount(j, la)) return rc;
+    hipLaunchKernelGGL(fgk::k_brick_lists<true>, dim3(j->nb), dim3(fgk::kListThreads), 0, j->ctx->stream, la);
+    if (affinity && j->typed)
+        hipLaunchKernelGGL(fgk::k_type_sort, dim3(j->nb), dim3(fgk::kListThreads), 0, j->ctx->stream, la);
+    FG_HIP(hipGetLastError());
+    return FG_OK;
 }
 
 int launchCutoff(fg_job *j, bool affinity, int doElecNear)
 {
     fgk::CutoffArgs a = cutoffArgs(j, doElecNear);
     dim3 grid(unsigned(j->nb)), block(fgk::kCutThreads);
-    if (!affinity) hipLaunchKernelGGL((fgk::k_cutoff<1, 0, false, false>), grid, block, 0, j->ctx->stream, a);
-    else if (j->M <= 4) launchCutoffM<4>(j, a, grid, block);
-    else if (j->M <= 8) launchCutoffM<8>(j, a, grid, block);
-    else launchCutoffM<16>(j, a, grid, block);
+    if (!affinity) {
+        a.brickList = j->dList;
+        hipLaunchKernelGGL((fgk::k_cutoff<1, 0, false, false, false>), grid, block, 0, j->ctx->stream, a);
+    } else if (j->M <= 4) {
+        if (j->typed) launchCutoffM<4, true>(j, a, grid, block);
+        else launchCutoffM<4, false>(j, a, grid, block);
+    } else if (j->M <= 8) {
+        if (j->typed) launchCutoffM<8, true>(j, a, grid, block);
+        else launchCutoffM<8, false>(j, a, grid, block);
+    } else {
+        launchCutoffM<16, false>(j, a, grid, block);
+    }
     FG_HIP(hipGetLastError());
     return FG_OK;
 }
@@ -500,6 +532,16 @@ int buildCutoff(fg_job *j, const fg_job_desc &d)
     FG_HIP(hipStreamSynchronize(j->ctx->stream));
     if (int rc = devAlloc(j, j->dList, size_t(tot))) return rc;
     if (int rc = devAlloc(j, j->dHList, size_t(totH))) return rc;
+    // the typed cutoff kernel: up to 8 maps (its LDS holds one type's rows for
+    // 8 maps) and up to kMaxTypes receptor types
+    j->typed = j->hasGeom && j->M > 0 && j->M <= 8 && j->T >= 1 && j->T <= fgk::kMaxTypes;
+    if (j->typed) {
+        std::vector<int32_t> ty(size_t(std::max(N, 1)), 0);
+        for (int i = 0; i < N; ++i) ty[size_t(i)] = A.type[i];
+        if (int rc = devUpload(j, j->dType, ty.data(), ty.size())) return rc;
+        if (int rc = devAlloc(j, j->dSorted, size_t(tot))) return rc;
+        if (int rc = devAlloc(j, j->dTypeStart, size_t(j->nb) * size_t(j->T + 1))) return rc;
+    }
     // tables
     const size_t pairN = size_t(std::max(d.tables.n_types, 1)) * fgk::kTableLen * size_t(std::max(j->M, 1));
     std::vector<double> zeros;
@@ -774,7 +816,7 @@ struct fg_batch {
     std::vector<fg_job *> jobs;  // largest first
     bool aff = false, near = false, elec = false, comp = true;
     int MM = 4, KH = 0, pz = 4;
-    bool pointAng = false, clamp = false;
+    bool pointAng = false, clamp = false, typed = true;
     int sharedTableLen = 0, maxTableLen = 0;
     int cutItems = 0, elecItems = 0, cutWGs = 0, elecWGs = 0;
     fgk::ListArgs *dList = nullptr;
@@ -821,10 +863,10 @@ int persistentWGs(fg_batch *b, K kernel, int threads, size_t lds, int items, int
 }
 
 // With launch false, only sizes the persistent grid (done before a capture).
-template <int MM, int KH, bool kAff, bool kPA>
+template <int MM, int KH, bool kAff, bool kPA, bool kTY>
 int launchCutoffBatch(fg_batch *b, bool launch)
 {
-    auto k = fgk::k_cutoff_batch<MM, KH, kAff, kPA>;
+    auto k = fgk::k_cutoff_batch<MM, KH, kAff, kPA, kTY>;
     if (!b->cutWGs)
         if (int rc = persistentWGs(b, k, fgk::kCutThreads, 0, b->cutItems, b->cutWGs)) return rc;
     if (!launch) return FG_OK;
@@ -836,8 +878,11 @@ int launchCutoffBatch(fg_batch *b, bool launch)
 template <int MM, int KH>
 int launchCutoffBatchPA(fg_batch *b, bool launch)
 {
-    return b->pointAng ? launchCutoffBatch<MM, KH, true, true>(b, launch)
-                       : launchCutoffBatch<MM, KH, true, false>(b, launch);
+    if (MM <= 8 && b->typed)
+        return b->pointAng ? launchCutoffBatch<MM, KH, true, true, MM <= 8>(b, launch)
+                           : launchCutoffBatch<MM, KH, true, false, MM <= 8>(b, launch);
+    return b->pointAng ? launchCutoffBatch<MM, KH, true, true, false>(b, launch)
+                       : launchCutoffBatch<MM, KH, true, false, false>(b, launch);
 }
 
 template <int MM>
@@ -884,9 +929,12 @@ int enqueueBatch(fg_batch *b, hipEvent_t mid, bool launch = true)
             hipLaunchKernelGGL(fgk::k_scan_counts_batch, dim3(n), dim3(1024), 0, s, b->dList);
             hipLaunchKernelGGL(fgk::k_brick_lists_batch<true>, dim3(b->cutItems), dim3(fgk::kListThreads), 0, s,
                                b->dList, b->dCutBase, n);
+            if (b->typed)
+                hipLaunchKernelGGL(fgk::k_type_sort_batch, dim3(b->cutItems), dim3(fgk::kListThreads), 0, s, b->dList,
+                                   b->dCutBase, n);
         }
         int rc;
-        if (!b->aff) rc = launchCutoffBatch<1, 0, false, false>(b, launch);
+        if (!b->aff) rc = launchCutoffBatch<1, 0, false, false, false>(b, launch);
         else if (b->MM == 4) rc = launchCutoffBatchKH<4>(b, launch);
         else if (b->MM == 8) rc = launchCutoffBatchKH<8>(b, launch);
         else rc = launchCutoffBatchKH<16>(b, launch);
@@ -921,6 +969,7 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
         b->pointAng = b->pointAng || j->pointAng;
         b->clamp = b->clamp || j->clamp;
         b->maxTableLen = std::max(b->maxTableLen, j->tableLen);
+        b->typed = b->typed && j->typed;
     }
     b->pz = work[1] > work[0] ? 8 : 4;
     b->MM = maxM <= 4 ? 4 : maxM <= 8 ? 8 : 16;
@@ -943,6 +992,7 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
     for (fg_job *j : b->jobs) {
         la.push_back(listArgs(j));
         ca.push_back(cutoffArgs(j, b->near ? 1 : 0));
+        if (!(b->typed && b->aff)) ca.back().brickList = j->dList;
         fgk::ElecArgs e = elecArgs(j, b->pz, j->tile, j->bxShift);
         if (b->sharedTableLen > 0) e.clampMax = b->sharedTableLen - 1;
         ea.push_back(e);
@@ -1093,6 +1143,13 @@ int fg_job_create(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
         for (int m = 0; m < j->M; ++m)
             if (keeps[size_t(m)] == (pass == 0)) j->order.push_back(m);
     j->KH = int(std::count(keeps.begin(), keeps.end(), true));
+    bool anyCovalent = false;
+    for (int m = 0; m < j->M; ++m) anyCovalent = anyCovalent || d->maps[m].is_covalent;
+    for (int h = 0; h < 6 && !anyCovalent; ++h) {
+        bool add = true;
+        for (int m = 0; m < j->M; ++m) add = add && ((actions[size_t(m)] >> (2 * h)) & 3) == fgk::kActAdd;
+        if (add) j->addOnlyClasses |= 1 << h;
+    }
     if (j->KH > fgk::kMaxHbSlots) {
         destroyJob(j);
         return fgc::fail(FG_E_ARG, "fg_job_create: at most 4 maps of donor (DS/D1) or A1 probes are supported");
@@ -1150,10 +1207,7 @@ int fg_job_run(fg_ctx *ctx, fg_job *j, unsigned what, int precision)
     FG_HIP(hipEventRecord(ctx->ev[0], s));
     if (precision == FG_PREC_PLAIN) {
         if (aff && j->hasGeom) {
-            fgk::ListArgs la = listArgs(j);
-            if (int rc = runListCount(j, la)) return rc;
-            hipLaunchKernelGGL(fgk::k_brick_lists<true>, dim3(j->nb), dim3(fgk::kListThreads), 0, s, la);
-            FG_HIP(hipGetLastError());
+            if (int rc = runLists(j, false)) return rc;
             if (int rc = runPlainAffinity(j)) return rc;
         }
         FG_HIP(hipEventRecord(ctx->ev[1], s));
@@ -1165,11 +1219,9 @@ int fg_job_run(fg_ctx *ctx, fg_job *j, unsigned what, int precision)
         return FG_OK;
     }
     if (aff || near) {
-        fgk::ListArgs la = listArgs(j);
-        if (int rc = runListCount(j, la)) return rc;
-        hipLaunchKernelGGL(fgk::k_brick_lists<true>, dim3(j->nb), dim3(fgk::kListThreads), 0, s, la);
-        FG_HIP(hipGetLastError());
-        if (int rc = launchCutoff(j, aff && j->hasGeom, near ? 1 : 0)) return rc;
+        const bool affMaps = aff && j->hasGeom;
+        if (int rc = runLists(j, affMaps)) return rc;
+        if (int rc = launchCutoff(j, affMaps, near ? 1 : 0)) return rc;
     }
     FG_HIP(hipEventRecord(ctx->ev[1], s));
     if (elec)
@@ -1221,15 +1273,15 @@ int fg_job_cutoff_pairs(fg_ctx *ctx, fg_job *j, int64_t *pairs)
     if (!ctx || !j || !pairs) return fgc::fail(FG_E_ARG, "fg_job_cutoff_pairs: NULL argument");
     FG_HIP(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
-    fgk::ListArgs la = listArgs(j);
-    if (int rc = runListCount(j, la)) return rc;
-    hipLaunchKernelGGL(fgk::k_brick_lists<true>, dim3(j->nb), dim3(fgk::kListThreads), 0, s, la);
+    if (int rc = runLists(j, false)) return rc;
+    fgk::CutoffArgs ca = cutoffArgs(j, 0);
+    ca.brickList = j->dList;  // the unsorted lists
     unsigned long long *d = nullptr;
     FG_HIP(hipMalloc(&d, sizeof *d));
     unsigned long long h = 0;
     hipError_t e = hipMemsetAsync(d, 0, sizeof *d, s);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(fgk::k_count_pairs, dim3(j->nb), dim3(fgk::kCutThreads), 0, s, cutoffArgs(j, 0), d);
+        hipLaunchKernelGGL(fgk::k_count_pairs, dim3(j->nb), dim3(fgk::kCutThreads), 0, s, ca, d);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, s);

@@ -59,6 +59,11 @@ struct ListArgs {
     const int32_t *start;     // [nb+1] (fill pass)
     const int32_t *hStart;    // [nb+1] (fill pass)
     int32_t *list, *hList;
+    // type sort (k_type_sort): receptor type per atom, output list, run starts
+    const int32_t *atomType;
+    int32_t *sorted;
+    int32_t *typeStart;       // [nb][T+1]
+    int32_t T;
 };
 
 struct ListLds {
@@ -312,25 +317,40 @@ __device__ __forceinline__ int binGuarded(double r, bool &edge)
     return k < kTableLen - 1 ? k : kTableLen - 1;
 }
 
-constexpr int kChunk = 32;  // candidate atoms staged in LDS per round
+constexpr int kChunk = 32;     // candidate atoms staged in LDS per round
+constexpr int kRowBins = 801;  // table bins of pairs with r <= 8 A: floor(100 r) <= 800
+constexpr int kMaxTypes = 32;  // receptor types the type-sorted lists support
 
-// kPointAng: the job has atoms whose directional terms depend on the point
-// (bonded donors/acceptors); without them the FP64 transcendental code and its
-// registers are left out.
-template <int MM, bool kAffinity>
+// LDS of one cutoff workgroup.  kTyped: the brick's candidates are sorted by
+// receptor type and the pair-table entries of the current type for the first
+// MM/2 maps (bins 0..800) are staged here; the other maps and the
+// desolvation value are read through the vector L1.  The per-lane table
+// reads of the pair loop (one bin per lane) are gathers either way; splitting
+// them keeps the LDS and the L1 busy at the same time.
+template <int MM>
+constexpr int ldsMaps()
+{
+    return MM / 2 > 0 ? MM / 2 : 1;
+}
+template <int MM, bool kAffinity, bool kTyped>
 struct CutoffLds {
     CutAtom atom[kChunk];
     double ab[kAffinity ? kChunk * 2 * MM : 1];
     int32_t idx[kChunk];
+    double row[kTyped ? kRowBins * ldsMaps<MM>() : 1];
+    double eps[kTyped ? kRowBins : 1];
 };
 
-// One brick of the cutoff kernel; S is the workgroup's LDS staging area.
-template <int MM, int KH, bool kAffinity, bool kPointAng>
-__device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLds<MM, kAffinity> &S)
+// One brick of the cutoff kernel; L is the workgroup's LDS.
+// kPointAng: the job has atoms whose directional terms depend on the point
+// (bonded donors/acceptors); without them the FP64 transcendental code and its
+// registers are left out.
+template <int MM, int KH, bool kAffinity, bool kPointAng, bool kTyped>
+__device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLds<MM, kAffinity, kTyped> &L)
 {
-    CutAtom *sAtom = S.atom;
-    double *sAB = S.ab;
-    int32_t *sIdx = S.idx;
+    CutAtom *sAtom = L.atom;
+    double *sAB = L.ab;
+    int32_t *sIdx = L.idx;
     const int bx = b % a.nbx, by = (b / a.nbx) % a.nby, bz = b / (a.nbx * a.nby);
     // wave = 4x4x4 cube of points, workgroup = 2x2x1 waves
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
@@ -343,6 +363,13 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
     const double gz = (iz - a.g.dz2) * a.g.spacing;
     const int64_t p = valid ? (int64_t(iz) * a.g.ny + iy) * a.g.nx + ix : 0;
     const int M = a.M;
+
+    if (kTyped) {
+        // rows of the brick-independent tables for r <= 8 A (the barrier of
+        // the first chunk orders these stores before their reads)
+        __syncthreads();  // the previous brick of a persistent workgroup is done with them
+        for (int q = tid; q < kRowBins; q += kCutThreads) L.eps[q] = a.eps[q];
+    }
 
     // pass 1: nearest DS/D1 atom, lowest index on ties (FG/CalculateGridmaps.cpp:65-82)
     int cH = 0;
@@ -378,101 +405,140 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
     }
     double dmap = 0, enear = 0;
 
-    const int c0 = a.brickStart[b], c1 = a.brickStart[b + 1];
-    for (int cb = c0; cb < c1; cb += kChunk) {
-        const int nc = min(kChunk, c1 - cb);
-        __syncthreads();
-        // stage nc atom records (16-byte pieces) and their desolvation products
-        if (tid < nc) sIdx[tid] = a.brickList[cb + tid];
-        __syncthreads();
-        for (int q = tid; q < nc * 8; q += kCutThreads) {
-            const float4 *src = reinterpret_cast<const float4 *>(a.atoms + sIdx[q >> 3]);
-            reinterpret_cast<float4 *>(sAtom)[q] = src[q & 7];
+    // candidates: one run over the brick's list, or one run per receptor
+    // type over the type-sorted list with that type's rows in LDS
+    const int nRuns = kTyped ? a.T : 1;
+    for (int run = 0; run < nRuns; ++run) {
+        int c0, c1;
+        if (kTyped) {
+            c0 = a.typeStart[b * (a.T + 1) + run];
+            c1 = a.typeStart[b * (a.T + 1) + run + 1];
+            if (c0 == c1) continue;
+            __syncthreads();  // every lane is done with the previous type's rows
+            const double *src = a.pair + int64_t(run) * kTableLen * M;
+            constexpr int ML = ldsMaps<MM>();
+            for (int q = tid; q < kRowBins * ML; q += kCutThreads) {
+                const int bin = q / ML, m = q - bin * ML;
+                L.row[q] = m < M ? src[bin * M + m] : 0.0;
+            }
+        } else {
+            c0 = a.brickStart[b];
+            c1 = a.brickStart[b + 1];
         }
-        if (kAffinity)
-            for (int q = tid; q < nc * MM; q += kCutThreads) {
-                const int c = q / MM, m = q - c * MM;
-                if (m < M) {
-                    const double *src = a.ab + int64_t(sIdx[c]) * 2 * M + 2 * m;
-                    sAB[(c * MM + m) * 2] = src[0];
-                    sAB[(c * MM + m) * 2 + 1] = src[1];
-                }
+        for (int cb = c0; cb < c1; cb += kChunk) {
+            const int nc = min(kChunk, c1 - cb);
+            __syncthreads();  // every lane is done with the previous chunk
+            // stage nc atom records (16-byte pieces) and their desolvation products
+            for (int q = tid; q < nc * 8; q += kCutThreads) {
+                const int ia = a.brickList[cb + (q >> 3)];
+                if ((q & 7) == 0) sIdx[q >> 3] = ia;
+                reinterpret_cast<float4 *>(sAtom)[q] = reinterpret_cast<const float4 *>(a.atoms + ia)[q & 7];
             }
-        __syncthreads();
-        for (int c = 0; c < nc; ++c) {
-            const CutAtom &A = sAtom[c];
-            const double dx = A.x - gx, dy = A.y - gy, dz = A.z - gz;
-            const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
-            double rSq = dx2 + dy2 + dz2;
-            if (!valid || rSq > 64.0) continue;
-            if (rSq < 1.0E-24) rSq = 1.0E-24;
-            // guarded fast geometry (see binGuarded)
-            const double yinv = rsqrtFast(rSq);
-            bool edge;
-            int idx = binGuarded(rSq * yinv, edge);
-            int idxE = idx;
-            if (edge) {
-                // exact: the cutoff loop's 1/(1/sqrt(r^2)) and the e-map's
-                // sqrt(dx^2 + (dy^2 + dz^2))  (FG/CalculateGridmaps.cpp:286-289,
-                // FG/electrostatics/Cpu.cpp:70-75)
-                const double invR = 1 / sqrt(rSq);
-                idx = binOf(1 / invR);
-                idxE = binOf(sqrt(dx2 + (dy2 + dz2)));
-            }
-            if (a.doElecNear) enear += A.w * dmin_(yinv, 2) * a.eps[idxE];
-            if (!kAffinity) continue;
-            // atom fields are the same in every lane: keep them in scalar registers
-            const int type = uniform(A.type), ahb = uniform(A.hbond), fl = uniform(A.flags);
-            const int dis = fl & kFlagDisorder, kind = fl >> 1;
-            if (type == a.hydrogen && dis) continue;
-            double racc = 1, rdon = 1, Hramp = 1;
-            if (kind == kAngConst) {
-                racc = A.ang[0];
-                rdon = A.ang[1];
-                Hramp = A.ang[2];
-            } else if (kPointAng && kind == kAngPoint) {
-                // directional terms use the unit vector exactly as the reference forms it
-                const double invR = 1 / sqrt(rSq);
-                const double rv[3] = {A.rv[0], A.rv[1], A.rv[2]};
-                const double rv2[3] = {A.rv2[0], A.rv2[1], A.rv2[2]};
-                hbondAngular(ahb, uniform(A.rexp), dis, uniform(sIdx[c]) == cH, rv, rv2, cHrv, dx * invR,
-                             dy * invR, dz * invR, racc, rdon, Hramp);
-            }
-            const double S = a.desolv[idx];
-            const double *row = a.pair + (int64_t(type) * kTableLen + idx) * M;
-            const double *rowH = (dis && a.hydrogen >= 0)
-                                     ? a.pair + (int64_t(a.hydrogen) * kTableLen + (idx - 110 > 0 ? idx - 110 : 0)) * M
-                                     : row;
-            const double *ab = &sAB[c * MM * 2];
-            const int shift = 2 * ahb;
-#pragma unroll
-            for (int m = 0; m < MM; ++m) {
-                if (m >= M || a.mp[m].isCovalent) continue;
-                const int act = (a.mp[m].actions >> shift) & 3;
-                const double E = row[m];
-                // one addend per map and a single update of e[m], so the
-                // accumulators never sit in divergent branches (a map that
-                // keeps a min/max adds nothing here: e + 0 == e)
-                double add = E;
-                if (act != kActAdd) {
-                    const double rsph = dmin_(dmax_(E / 100, 0), 1);
-                    if (act == kActDonor) {
-                        add = rowH[m] * Hramp * (racc + (1 - racc) * rsph);
-                    } else {
-                        add = 0.0;
-                        if (m < KH) {
-                            const double f = act == kActMinRacc ? racc : rdon;
-                            const double v = E * (f + (1 - f) * rsph);
-                            hmin[m < KH ? m : 0] = dmin_(hmin[m < KH ? m : 0], v);
-                            hmax[m < KH ? m : 0] = dmax_(hmax[m < KH ? m : 0], v);
-                            flags |= 1u << m;
-                        }
+            if (kAffinity)
+                for (int q = tid; q < nc * MM; q += kCutThreads) {
+                    const int c = q / MM, m = q - c * MM;
+                    if (m < M) {
+                        const double *src = a.ab + int64_t(a.brickList[cb + c]) * 2 * M + 2 * m;
+                        sAB[(c * MM + m) * 2] = src[0];
+                        sAB[(c * MM + m) * 2 + 1] = src[1];
                     }
                 }
-                e[m] += add;
-                e[m] += ab[2 * m] * S + ab[2 * m + 1] * S;
+            __syncthreads();
+            for (int c = 0; c < nc; ++c) {
+                const CutAtom &A = sAtom[c];
+                const double dx = A.x - gx, dy = A.y - gy, dz = A.z - gz;
+                const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+                double rSq = dx2 + dy2 + dz2;
+                if (!valid || rSq > 64.0) continue;
+                if (rSq < 1.0E-24) rSq = 1.0E-24;
+                // guarded fast geometry (see binGuarded)
+                const double yinv = rsqrtFast(rSq);
+                bool edge;
+                int idx = binGuarded(rSq * yinv, edge);
+                int idxE = idx;
+                if (edge) {
+                    // exact: the cutoff loop's 1/(1/sqrt(r^2)) and the e-map's
+                    // sqrt(dx^2 + (dy^2 + dz^2))  (FG/CalculateGridmaps.cpp:286-289,
+                    // FG/electrostatics/Cpu.cpp:70-75)
+                    const double invR = 1 / sqrt(rSq);
+                    idx = binOf(1 / invR);
+                    idxE = binOf(sqrt(dx2 + (dy2 + dz2)));
+                }
+                // r <= 8 A: both bins are <= 800
+                if (a.doElecNear) enear += A.w * dmin_(yinv, 2) * (kTyped ? L.eps[idxE] : a.eps[idxE]);
+                if (!kAffinity) continue;
+                // atom fields are the same in every lane: keep them in scalar registers
+                const int type = uniform(A.type), ahb = uniform(A.hbond), fl = uniform(A.flags);
+                const int dis = fl & kFlagDisorder, kind = fl >> 1;
+                if (type == a.hydrogen && dis) continue;
+                double racc = 1, rdon = 1, Hramp = 1;
+                if (kind == kAngConst) {
+                    racc = A.ang[0];
+                    rdon = A.ang[1];
+                    Hramp = A.ang[2];
+                } else if (kPointAng && kind == kAngPoint) {
+                    // directional terms use the unit vector exactly as the reference forms it
+                    const double invR = 1 / sqrt(rSq);
+                    const double rv[3] = {A.rv[0], A.rv[1], A.rv[2]};
+                    const double rv2[3] = {A.rv2[0], A.rv2[1], A.rv2[2]};
+                    hbondAngular(ahb, uniform(A.rexp), dis, uniform(sIdx[c]) == cH, rv, rv2, cHrv, dx * invR,
+                                 dy * invR, dz * invR, racc, rdon, Hramp);
+                }
+                const double S = a.desolv[idx];
+                const double *grow = a.pair + (int64_t(type) * kTableLen + idx) * M;
+                const double *lrow = kTyped ? &L.row[idx * ldsMaps<MM>()] : grow;
+                // map m's entry: LDS for the first MM/2 maps of the typed kernel
+#define FG_ROW(m) ((kTyped && (m) < ldsMaps<MM>()) ? lrow[(m)] : grow[(m)])
+                const double *row = grow;
+                const double *rowH =
+                    (dis && a.hydrogen >= 0)
+                        ? a.pair + (int64_t(a.hydrogen) * kTableLen + (idx - 110 > 0 ? idx - 110 : 0)) * M
+                        : row;
+                const double *ab = &sAB[c * MM * 2];
+                const int shift = 2 * ahb;
+                if (((a.addOnlyClasses >> ahb) & 1) && kTyped) {
+                    // every map adds the table value for this atom (no H-bond
+                    // term, no covalent map): no per-map control flow.  Maps
+                    // m >= M accumulate values from the staged area that are
+                    // never stored.
+#pragma unroll
+                    for (int m = 0; m < MM; ++m) {
+                        e[m] += FG_ROW(m);
+                        e[m] += ab[2 * m] * S + ab[2 * m + 1] * S;
+                    }
+                    dmap += A.qvol * S;
+                    continue;
+                }
+#pragma unroll
+                for (int m = 0; m < MM; ++m) {
+                    if (m >= M || a.mp[m].isCovalent) continue;
+                    const int act = (a.mp[m].actions >> shift) & 3;
+                    const double E = FG_ROW(m);
+                    // one addend per map and a single update of e[m], so the
+                    // accumulators never sit in divergent branches (a map that
+                    // keeps a min/max adds nothing here: e + 0 == e)
+                    double add = E;
+                    if (act != kActAdd) {
+                        const double rsph = dmin_(dmax_(E / 100, 0), 1);
+                        if (act == kActDonor) {
+                            add = rowH[m] * Hramp * (racc + (1 - racc) * rsph);
+                        } else {
+                            add = 0.0;
+                            if (m < KH) {
+                                const double f = act == kActMinRacc ? racc : rdon;
+                                const double v = E * (f + (1 - f) * rsph);
+                                hmin[m < KH ? m : 0] = dmin_(hmin[m < KH ? m : 0], v);
+                                hmax[m < KH ? m : 0] = dmax_(hmax[m < KH ? m : 0], v);
+                                flags |= 1u << m;
+                            }
+                        }
+                    }
+                    e[m] += add;
+                    e[m] += ab[2 * m] * S + ab[2 * m + 1] * S;
+                }
+                dmap += A.qvol * S;
+#undef FG_ROW
             }
-            dmap += A.qvol * S;
         }
     }
     if (!valid) return;
@@ -489,17 +555,71 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
     if (a.doElecNear) a.maps[int64_t(M) * a.P + p] = enear;
 }
 
-template <int MM, int KH, bool kAffinity, bool kPointAng>
+template <int MM, int KH, bool kAffinity, bool kPointAng, bool kTyped>
 __global__ __launch_bounds__(kCutThreads) void k_cutoff(CutoffArgs a)
 {
-    __shared__ CutoffLds<MM, kAffinity> S;
+    __shared__ CutoffLds<MM, kAffinity, kTyped> L;
     // XCD-aware brick order: consecutive workgroups go to different XCDs, so
     // hand each XCD a contiguous run of bricks (neighbours share atoms in L2)
     // (bijection on the first 8*floor(nb/8) workgroups, identity on the tail)
     const int nb = a.nbx * a.nby * a.nbz;
     const int full = (nb / 8) * 8, g = int(blockIdx.x);
     const int b = g < full ? (g % 8) * (full / 8) + g / 8 : g;
-    cutoffBrick<MM, KH, kAffinity, kPointAng>(a, b, S);
+    cutoffBrick<MM, KH, kAffinity, kPointAng, kTyped>(a, b, L);
+}
+
+// Stable counting sort of brick b's candidate list by receptor type: list ->
+// sorted, typeStart[b*(T+1) + t] = first entry of type t (typeStart[..+T] =
+// end).  Ranks within a chunk come from wave ballots, so the order inside a
+// type is the list order on every run.
+__device__ __forceinline__ void typeSortBrick(const ListArgs &a, int b, int32_t *cnt, int32_t *runStart,
+                                              int32_t *wcnt)
+{
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int T = a.T;
+    const int s0 = a.start[b], s1 = a.start[b + 1];
+    if (tid < T) cnt[tid] = 0;
+    __syncthreads();
+    for (int i = s0 + tid; i < s1; i += kListThreads) atomicAdd(&cnt[a.atomType[a.list[i]]], 1);
+    __syncthreads();
+    if (tid == 0) {
+        int run = s0;
+        for (int t = 0; t < T; ++t) {
+            runStart[t] = run;
+            a.typeStart[b * (T + 1) + t] = run;
+            run += cnt[t];
+        }
+        a.typeStart[b * (T + 1) + T] = s1;
+    }
+    __syncthreads();
+    for (int base = s0; base < s1; base += kListThreads) {
+        const int i = base + tid;
+        const int ia = i < s1 ? a.list[i] : -1;
+        const int t = ia >= 0 ? a.atomType[ia] : -1;
+        int rank = 0;
+        for (int tt = 0; tt < T; ++tt) {
+            const uint64_t m = __ballot(t == tt);
+            if (t == tt)
+                rank = int(__builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u)));
+            if (lane == 0) wcnt[w * kMaxTypes + tt] = int(__popcll(m));
+        }
+        __syncthreads();
+        if (ia >= 0) {
+            int pos = runStart[t] + rank;
+            for (int v = 0; v < w; ++v) pos += wcnt[v * kMaxTypes + t];
+            a.sorted[pos] = ia;
+        }
+        __syncthreads();
+        if (tid < T) runStart[tid] += wcnt[tid] + wcnt[kMaxTypes + tid] + wcnt[2 * kMaxTypes + tid] +
+                                      wcnt[3 * kMaxTypes + tid];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kListThreads) void k_type_sort(ListArgs a)
+{
+    __shared__ int32_t cnt[kMaxTypes], runStart[kMaxTypes], wcnt[4 * kMaxTypes];
+    typeSortBrick(a, int(blockIdx.x), cnt, runStart, wcnt);
 }
 
 // ------------------------------------------------------------------------
@@ -749,19 +869,28 @@ __global__ __launch_bounds__(1024) void k_scan_counts_batch(const ListArgs *jobs
     scanCounts(a.counts, nb, 1, const_cast<int32_t *>(a.hStart), part);
 }
 
-template <int MM, int KH, bool kAffinity, bool kPointAng>
+template <int MM, int KH, bool kAffinity, bool kPointAng, bool kTyped>
 __global__ __launch_bounds__(kCutThreads) void k_cutoff_batch(const CutoffArgs *jobs, const int32_t *base, int nJobs,
                                                               int *counter)
 {
-    __shared__ CutoffLds<MM, kAffinity> S;
+    __shared__ CutoffLds<MM, kAffinity, kTyped> L;
     __shared__ int slot;
     const int total = base[nJobs];
     for (;;) {
         const int item = nextItem(counter, &slot);
         if (item >= total) break;
         const int job = uniform(jobOfItem(base, nJobs, item));
-        cutoffBrick<MM, KH, kAffinity, kPointAng>(jobs[job], item - base[job], S);
+        cutoffBrick<MM, KH, kAffinity, kPointAng, kTyped>(jobs[job], item - base[job], L);
     }
+}
+
+// Type sort of every job's lists: one workgroup per (job, brick) item.
+__global__ __launch_bounds__(kListThreads) void k_type_sort_batch(const ListArgs *jobs, const int32_t *base, int nJobs)
+{
+    __shared__ int32_t cnt[kMaxTypes], runStart[kMaxTypes], wcnt[4 * kMaxTypes];
+    const int item = int(blockIdx.x);
+    const int job = uniform(jobOfItem(base, nJobs, item));
+    typeSortBrick(jobs[job], item - base[job], cnt, runStart, wcnt);
 }
 
 // ------------------------------------------------------------------------

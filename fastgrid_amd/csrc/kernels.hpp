@@ -95,8 +95,12 @@ struct CutoffArgs {
     int64_t P;
     const CutAtom *atoms;
     const double *ab;                         // [N][2M] desolvation products
-    const int32_t *brickStart, *brickList;    // all candidates
+    const int32_t *brickStart, *brickList;    // all candidates (type-sorted for the typed kernel)
     const int32_t *hStart, *hList;            // DS/D1 candidates
+    const int32_t *typeStart;                 // [nb][T+1] runs of one receptor type in brickList
+    int32_t T;                                // receptor types (rows of the pair table)
+    int32_t addOnlyClasses;                   // bit h: atoms of H-bond class h add E to every map
+                                              // and no map is covalent
     const double *pair;    // [T][8192][M], maps in kernel order
     const double *desolv;  // [8192]
     const double *eps;     // [8192]
