@@ -158,12 +158,12 @@ def copy_calibration(nbytes=1 << 30):
 
 def traffic_entry(shape):
     """HBM bytes per electrostatic launch on L from the committed counter
-    summary, when it was measured with this kernel shape."""
+    summary, when it was measured with the same kernel (points per lane)."""
     try:
         t = json.load(open(TRAFFIC_FILE))
     except (OSError, ValueError):
         return None
-    if tuple(t.get("shape", ())) != tuple(shape):
+    if t.get("pz") != shape[0]:
         return None
     return t
 
@@ -350,6 +350,7 @@ def main():
                          "pairs within 8 A and the affinity/desolvation maps in FP64)",
             "elec_pairs": L["elec_pairs"], "cutoff_pairs": L["cutoff_pairs"],
             "elec_kernel_shape": {"pz": L["shape"][0], "tile": L["shape"][1], "bx": L["shape"][2]},
+            "autotune": L["tune_report"].strip().splitlines(),
         },
         "seconds_per_map_set": round(comp["t"], 5),
         "plain_gpu": {"value": round(tot_pairs / plain["t"] / 1e9, 2), "unit": "Gpair-evals/s",

@@ -498,11 +498,11 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                 const int shift = 2 * ahb;
                 if (((a.addOnlyClasses >> ahb) & 1) && kTyped) {
                     // every map adds the table value for this atom (no H-bond
-                    // term, no covalent map): no per-map control flow.  Maps
-                    // m >= M accumulate values from the staged area that are
-                    // never stored.
+                    // term, no covalent map): no per-map control flow beyond
+                    // the job's map count
 #pragma unroll
                     for (int m = 0; m < MM; ++m) {
+                        if (m >= M) break;
                         e[m] += FG_ROW(m);
                         e[m] += ab[2 * m] * S + ab[2 * m + 1] * S;
                     }
