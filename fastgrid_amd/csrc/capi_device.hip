@@ -211,12 +211,12 @@ void launchCutoffM(fg_job *j, const fgk::CutoffArgs &a, dim3 grid, dim3 block)
     else launchCutoffKH<MM, false, TY>(j, a, grid, block);
 }
 
-// Candidate lists of the job (count, scans, fill, and the type sort when the
-// typed kernel runs).
+// Candidate lists of the job: fill, and the type sort when the typed kernel
+// runs.  The per-brick offsets come from the count pass made when the job was
+// created (they depend only on the atoms and the grid).
 int runLists(fg_job *j, bool affinity)
 {
     fgk::ListArgs la = listArgs(j);
-    if (int rc = runListCount(j, la)) return rc;
     hipLaunchKernelGGL(fgk::k_brick_lists<true>, dim3(j->nb), dim3(fgk::kListThreads), 0, j->ctx->stream, la);
     if (affinity && j->typed)
         hipLaunchKernelGGL(fgk::k_type_sort, dim3(j->nb), dim3(fgk::kListThreads), 0, j->ctx->stream, la);
@@ -234,6 +234,9 @@ int launchCutoff(fg_job *j, bool affinity, int doElecNear)
     } else if (j->M <= 4) {
         if (j->typed) launchCutoffM<4, true>(j, a, grid, block);
         else launchCutoffM<4, false>(j, a, grid, block);
+    } else if (j->M <= 6) {
+        if (j->typed) launchCutoffM<6, true>(j, a, grid, block);
+        else launchCutoffM<6, false>(j, a, grid, block);
     } else if (j->M <= 8) {
         if (j->typed) launchCutoffM<8, true>(j, a, grid, block);
         else launchCutoffM<8, false>(j, a, grid, block);
@@ -252,9 +255,14 @@ fgk::ElecArgs elecArgs(const fg_job *j, int pz, int tile, int bxShift)
     a.tableLen = j->tableLen;
     a.tile = tile;
     a.bxShift = bxShift;
-    const int bxl = 1 << bxShift, byl = fgk::kElecThreads >> bxShift;
-    a.nbx = (j->g.nx + bxl - 1) / bxl;
-    a.nby = (j->g.ny + byl - 1) / byl;
+    if (bxShift == 0) {
+        a.nbx = int((int64_t(j->g.nx) * j->g.ny + fgk::kElecThreads - 1) / fgk::kElecThreads);
+        a.nby = 1;
+    } else {
+        const int bxl = 1 << bxShift, byl = fgk::kElecThreads >> bxShift;
+        a.nbx = (j->g.nx + bxl - 1) / bxl;
+        a.nby = (j->g.ny + byl - 1) / byl;
+    }
     a.nbz = (j->g.nz + pz - 1) / pz;
     a.clampMax = j->tableLen - 1;
     a.floorBitsNeg4 = int32_t(0u - 4u * fgk::kFloorBits);
@@ -732,12 +740,12 @@ int ilog2(int64_t v)
 
 int tuneJobs(fg_ctx *ctx, fg_job *const *jobs, int n, bool comp)
 {
-    static const int kPz[2] = {4, 8}, kTiles[3] = {32, 64, 128}, kShifts[3] = {3, 4, 5};
+    static const int kPz[2] = {4, 8}, kTiles[3] = {32, 64, 128}, kShifts[4] = {3, 4, 5, 0};
     // size classes: (atom octave, point octave, best-padding brick width)
     struct Cls {
         int key[3];
         std::vector<int> members;
-        float t[2][3][3];
+        float t[2][3][4];
     };
     std::vector<Cls> cls;
     for (int i = 0; i < n; ++i) {
@@ -764,7 +772,7 @@ int tuneJobs(fg_ctx *ctx, fg_job *const *jobs, int n, bool comp)
         fg_job *rep = jobs[c.members[c.members.size() / 2]];
         for (int a = 0; a < 2; ++a)
             for (int b = 0; b < 3; ++b)
-                for (int d = 0; d < 3; ++d)
+                for (int d = 0; d < 4; ++d)
                     if (int rc = timeShape(rep, kPz[a], kTiles[b], kShifts[d], comp, c.t[a][b][d])) return rc;
         const double repWork = std::max(1.0, double(rep->N) * double(rep->P));
         double work = 0;
@@ -772,7 +780,7 @@ int tuneJobs(fg_ctx *ctx, fg_job *const *jobs, int n, bool comp)
         for (int a = 0; a < 2; ++a) {
             float best = 1e30f;
             for (int b = 0; b < 3; ++b)
-                for (int d = 0; d < 3; ++d) best = std::min(best, c.t[a][b][d]);
+                for (int d = 0; d < 4; ++d) best = std::min(best, c.t[a][b][d]);
             total[a] += best * work / repWork;
         }
     }
@@ -784,7 +792,7 @@ int tuneJobs(fg_ctx *ctx, fg_job *const *jobs, int n, bool comp)
     for (Cls &c : cls) {
         int bb = 0, bd = 0;
         for (int b = 0; b < 3; ++b)
-            for (int d = 0; d < 3; ++d)
+            for (int d = 0; d < 4; ++d)
                 if (c.t[pzi][b][d] < c.t[pzi][bb][bd]) {
                     bb = b;
                     bd = d;
@@ -799,7 +807,8 @@ int tuneJobs(fg_ctx *ctx, fg_job *const *jobs, int n, bool comp)
                       "class atoms~2^%d points~2^%d bx%d: %zu jobs, median %d atoms %dx%dx%d -> tile %d bx %d "
                       "(%.4f ms; default shape %.4f ms)\n",
                       c.key[0], c.key[1], 1 << c.key[2], c.members.size(), rep->N, rep->g.nx, rep->g.ny, rep->g.nz,
-                      kTiles[bb], 1 << kShifts[bd], c.t[pzi][bb][bd], c.t[pzi][1][1]);
+                      kTiles[bb], kShifts[bd] == 0 ? fgk::kElecThreads : 1 << kShifts[bd], c.t[pzi][bb][bd],
+                      c.t[pzi][1][1]);
         report += line;
     }
     ctx->tuneReport = report;
@@ -924,9 +933,7 @@ int enqueueBatch(fg_batch *b, hipEvent_t mid, bool launch = true)
     if (launch) hipLaunchKernelGGL(fgk::k_fill_i32, dim3(1), dim3(256), 0, s, b->dCounters, 2, 0);
     if (b->aff || b->near) {
         if (launch) {
-            hipLaunchKernelGGL(fgk::k_brick_lists_batch<false>, dim3(b->cutItems), dim3(fgk::kListThreads), 0, s,
-                               b->dList, b->dCutBase, n);
-            hipLaunchKernelGGL(fgk::k_scan_counts_batch, dim3(n), dim3(1024), 0, s, b->dList);
+            // list offsets are fixed at job creation: fill (and sort) only
             hipLaunchKernelGGL(fgk::k_brick_lists_batch<true>, dim3(b->cutItems), dim3(fgk::kListThreads), 0, s,
                                b->dList, b->dCutBase, n);
             if (b->typed)
@@ -936,6 +943,7 @@ int enqueueBatch(fg_batch *b, hipEvent_t mid, bool launch = true)
         int rc;
         if (!b->aff) rc = launchCutoffBatch<1, 0, false, false, false>(b, launch);
         else if (b->MM == 4) rc = launchCutoffBatchKH<4>(b, launch);
+        else if (b->MM == 6) rc = launchCutoffBatchKH<6>(b, launch);
         else if (b->MM == 8) rc = launchCutoffBatchKH<8>(b, launch);
         else rc = launchCutoffBatchKH<16>(b, launch);
         if (rc) return rc;
@@ -972,7 +980,7 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
         b->typed = b->typed && j->typed;
     }
     b->pz = work[1] > work[0] ? 8 : 4;
-    b->MM = maxM <= 4 ? 4 : maxM <= 8 ? 8 : 16;
+    b->MM = maxM <= 4 ? 4 : maxM <= 6 ? 6 : maxM <= 8 ? 8 : 16;
     if (b->aff && maxM == 0) b->aff = false;
     // one LDS table for all jobs when their dielectric tables agree
     {
@@ -1297,16 +1305,16 @@ int fg_job_get_shape(const fg_job *job, fg_shape *out)
     if (!job || !out) return fgc::fail(FG_E_ARG, "fg_job_get_shape: NULL argument");
     out->pz = job->pz;
     out->tile = job->tile;
-    out->bx = 1 << job->bxShift;
+    out->bx = job->bxShift == 0 ? fgk::kElecThreads : 1 << job->bxShift;
     return FG_OK;
 }
 
 int fg_job_set_shape(fg_job *job, const fg_shape *sh)
 {
     if (!job || !sh) return fgc::fail(FG_E_ARG, "fg_job_set_shape: NULL argument");
-    const int shift = sh->bx == 8 ? 3 : sh->bx == 16 ? 4 : sh->bx == 32 ? 5 : -1;
+    const int shift = sh->bx == 8 ? 3 : sh->bx == 16 ? 4 : sh->bx == 32 ? 5 : sh->bx == 256 ? 0 : -1;
     if ((sh->pz != 4 && sh->pz != 8) || (sh->tile != 32 && sh->tile != 64 && sh->tile != 128) || shift < 0)
-        return fgc::fail(FG_E_ARG, "fg_job_set_shape: pz must be 4 or 8, tile 32/64/128, bx 8/16/32");
+        return fgc::fail(FG_E_ARG, "fg_job_set_shape: pz must be 4 or 8, tile 32/64/128, bx 8/16/32/256");
     job->pz = sh->pz;
     job->tile = sh->tile;
     job->bxShift = shift;

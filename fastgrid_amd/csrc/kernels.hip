@@ -498,11 +498,11 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                 const int shift = 2 * ahb;
                 if (((a.addOnlyClasses >> ahb) & 1) && kTyped) {
                     // every map adds the table value for this atom (no H-bond
-                    // term, no covalent map): no per-map control flow beyond
-                    // the job's map count
+                    // term, no covalent map): no per-map control flow.  Maps
+                    // m >= M (the kernel is compiled for 4, 6, 8 or 16 maps)
+                    // accumulate values that are never stored.
 #pragma unroll
                     for (int m = 0; m < MM; ++m) {
-                        if (m >= M) break;
                         e[m] += FG_ROW(m);
                         e[m] += ab[2 * m] * S + ab[2 * m + 1] * S;
                     }
@@ -643,17 +643,37 @@ template <int PZ, bool kComp, bool kClamp>
 __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float *eps)
 {
     const int tid = threadIdx.x;
-    const int sh = a.bxShift, bxl = 1 << sh, byl = kElecThreads >> sh;
-    const int bx = b % a.nbx, by = (b / a.nbx) % a.nby, bz = b / (a.nbx * a.nby);
-    const int ix0 = bx * bxl, iy0 = by * byl, iz0 = bz * PZ;
-    const int ix = ix0 + (tid & (bxl - 1)), iy = iy0 + (tid >> sh);
+    const int sh = a.bxShift;
+    int ix, iy, ix0, iy0, bx1, by1, iz0;
+    if (sh == 0) {
+        // 256 consecutive columns of the x-y plane (x fastest): padding only
+        // at the end of each z-slab
+        const int c0 = (b % a.nbx) * kElecThreads, c = c0 + tid;
+        const int cl = min(c0 + kElecThreads, a.g.nx * a.g.ny) - 1;  // last column of the brick
+        iz0 = (b / a.nbx) * PZ;
+        ix = c % a.g.nx;
+        iy = c / a.g.nx;  // >= ny past the plane: the lane stores nothing
+        iy0 = c0 / a.g.nx;
+        by1 = cl / a.g.nx;
+        ix0 = iy0 == by1 ? c0 % a.g.nx : 0;
+        bx1 = iy0 == by1 ? cl % a.g.nx : a.g.nx - 1;
+    } else {
+        const int bxl = 1 << sh, byl = kElecThreads >> sh;
+        const int bx = b % a.nbx, by = (b / a.nbx) % a.nby, bz = b / (a.nbx * a.nby);
+        ix0 = bx * bxl;
+        iy0 = by * byl;
+        iz0 = bz * PZ;
+        ix = ix0 + (tid & (bxl - 1));
+        iy = iy0 + (tid >> sh);
+        bx1 = min(ix0 + bxl - 1, a.g.nx - 1);
+        by1 = min(iy0 + byl - 1, a.g.ny - 1);
+    }
     const float s100 = a.spacing100;
     const float gx = float(ix - a.g.dx2) * s100, gy = float(iy - a.g.dy2) * s100;
     float gz[PZ];
 #pragma unroll
     for (int k = 0; k < PZ; ++k) gz[k] = float(iz0 + k - a.g.dz2) * s100;
     // brick box (scaled), clamped to the grid
-    const int bx1 = min(ix0 + bxl - 1, a.g.nx - 1), by1 = min(iy0 + byl - 1, a.g.ny - 1);
     const int bz1 = min(iz0 + PZ - 1, a.g.nz - 1);
     const float blo[3] = {float(ix0 - a.g.dx2) * s100, float(iy0 - a.g.dy2) * s100, float(iz0 - a.g.dz2) * s100};
     const float bhi[3] = {float(bx1 - a.g.dx2) * s100, float(by1 - a.g.dy2) * s100, float(bz1 - a.g.dz2) * s100};
@@ -847,7 +867,8 @@ __global__ __launch_bounds__(kElecThreads) void k_elec_batch(const ElecArgs *job
     }
 }
 
-// Candidate lists of every job: one workgroup per (job, brick) item.
+// Candidate lists of every job (fill pass; the offsets were counted when the
+// jobs were created): one workgroup per (job, brick) item.
 template <bool kFill>
 __global__ __launch_bounds__(kListThreads) void k_brick_lists_batch(const ListArgs *jobs, const int32_t *base,
                                                                    int nJobs)
@@ -856,17 +877,6 @@ __global__ __launch_bounds__(kListThreads) void k_brick_lists_batch(const ListAr
     const int item = int(blockIdx.x);
     const int job = uniform(jobOfItem(base, nJobs, item));
     brickList<kFill>(jobs[job], item - base[job], S);
-}
-
-// Both scans of every job: one workgroup per job.
-__global__ __launch_bounds__(1024) void k_scan_counts_batch(const ListArgs *jobs)
-{
-    __shared__ int32_t part[1024];
-    const ListArgs &a = jobs[blockIdx.x];
-    const int nb = a.nbx * a.nby * a.nbz;
-    scanCounts(a.counts, nb, 0, const_cast<int32_t *>(a.start), part);
-    __syncthreads();
-    scanCounts(a.counts, nb, 1, const_cast<int32_t *>(a.hStart), part);
 }
 
 template <int MM, int KH, bool kAffinity, bool kPointAng, bool kTyped>

@@ -114,8 +114,9 @@ struct ElecArgs {
     int32_t nAtoms;          // atoms padded to a multiple of kSubTile
     int32_t tableLen;        // entries of eps32
     int32_t tile;            // atoms per tile (a multiple of kSubTile)
-    int32_t bxShift;         // lanes along x = 1 << bxShift (8, 16 or 32); 256 lanes per brick
-    int32_t nbx, nby, nbz;   // bricks per axis
+    int32_t bxShift;         // lanes along x = 1 << bxShift (8, 16 or 32); 0: the brick is 256
+                             // consecutive columns of the x-y plane (x fastest)
+    int32_t nbx, nby, nbz;   // bricks per axis (bxShift 0: column blocks per plane, 1, z-bricks)
     int32_t clampMax;        // last usable table index
     int32_t floorBitsNeg4;   // -4 * bit pattern of 2^23 (mod 2^32): the table's byte offset for biased bins
     int32_t pad;

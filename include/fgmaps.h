@@ -184,7 +184,8 @@ int fg_job_cutoff_pairs(fg_ctx *ctx, fg_job *job, int64_t *pairs);
 /* ---------------------------------------------------------------- */
 /* Shape of a job's electrostatic kernel: grid points per lane along z (4 or
  * 8), receptor atoms per tile (32, 64 or 128) and lanes of the 256-lane
- * brick along x (8, 16 or 32). */
+ * brick along x (8, 16 or 32; 256: the brick is 256 consecutive columns of
+ * the x-y plane, which pads only at the end of the plane). */
 typedef struct {
     int32_t pz, tile, bx;
 } fg_shape;
@@ -211,8 +212,8 @@ typedef struct fg_batch fg_batch;
 #define FG_BATCH_NO_GRAPH 1u /* launch the kernels directly on each run (split timing) */
 
 /* Prepares one run of jobs[0..n_jobs) (created on ctx; they must outlive the
- * batch): the candidate lists of every job (two kernels over all (job,
- * brick) items and one scan per job), then one persistent cutoff kernel and
+ * batch): the candidate lists of every job (fill and type sort over all
+ * (job, brick) items), then one persistent cutoff kernel and
  * one persistent electrostatic kernel, each taking (job, brick) items from a
  * device counter, largest jobs first.  The sequence is captured once into a
  * hipGraph.  precision: FG_PREC_COMPENSATED or FG_PREC_FP32.  Every job

@@ -135,7 +135,7 @@ def test_shapes_and_autotuner(ctx, pockets):
     by_tile = {}
     for pz in (4, 8):
         for tile in (32, 64, 128):
-            for bx in (8, 16, 32):
+            for bx in (8, 16, 32, 256):
                 job.shape = (pz, tile, bx)
                 assert job.shape == (pz, tile, bx)
                 job.run(fg.RUN_ALL, fg.PREC_COMPENSATED)
@@ -154,7 +154,7 @@ def test_shapes_and_autotuner(ctx, pockets):
     shapes = [j.shape for j in jobs]
     assert len({s[0] for s in shapes}) == 1
     for pz, tile, bx in shapes:
-        assert pz in (4, 8) and tile in (32, 64, 128) and bx in (8, 16, 32)
+        assert pz in (4, 8) and tile in (32, 64, 128) and bx in (8, 16, 32, 256)
     # tuned jobs still reproduce through the batch
     want = _single(jobs, fg.PREC_COMPENSATED)
     fg.batch_run(ctx, jobs)
