@@ -6,7 +6,7 @@ package is its Python binding; see fastgrid_amd.api.
 """
 from .api import (  # noqa: F401
     BATCH_NO_GRAPH, LIB, LIB_PATH, PREC_COMPENSATED, PREC_FP32, PREC_FP64, PREC_PLAIN, RUN_ALL, RUN_AFFINITY, RUN_ELEC,
-    SIGNATURES, Batch, Context, FgError, Job, JobArrays, Prepared, autotune, batch_run, device_count, round3,
+    RUN_FLOATING, SIGNATURES, Batch, Context, FgError, Job, JobArrays, Prepared, autotune, batch_run, device_count, round3,
     synth_write,
 )
 

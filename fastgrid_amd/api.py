@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libfgmaps.so")
 
 FG_OK = 0
 PREC_PLAIN, PREC_COMPENSATED, PREC_FP64, PREC_FP32 = 0, 1, 2, 3
-RUN_ELEC, RUN_AFFINITY, RUN_ALL = 1, 2, 3
+RUN_ELEC, RUN_AFFINITY, RUN_FLOATING, RUN_ALL = 1, 2, 4, 7
 BATCH_NO_GRAPH = 1
 TABLE_LEN = 8192
 
@@ -52,7 +52,9 @@ class FgMapParams(C.Structure):
 class FgJobDesc(C.Structure):
     _fields_ = [("grid", FgGrid), ("atoms", FgAtoms), ("geom", FgHbondGeom), ("tables", FgTables),
                 ("n_maps", C.c_int32), ("maps", C.POINTER(FgMapParams)), ("covalent", _dp),
-                ("solpar_q", C.c_double), ("hydrogen", C.c_int32), ("dist_dep_diel", C.c_int32)]
+                ("solpar_q", C.c_double), ("hydrogen", C.c_int32), ("dist_dep_diel", C.c_int32),
+                ("cov_on", C.c_int32), ("cov_point", C.c_double * 3), ("cov_hw_sq_inv", C.c_double),
+                ("cov_barrier", C.c_double), ("floating_grid", C.c_int32)]
 
 
 class FgTiming(C.Structure):
@@ -82,6 +84,8 @@ SIGNATURES = {
     "fg_destroy": (None, [C.c_void_p]),
     "fg_device_count": (C.c_int, []),
     "fg_elec_map": (C.c_int, [C.c_void_p, C.POINTER(FgGrid), C.POINTER(FgAtoms), _dp, C.c_int, C.c_int, _dp]),
+    "fg_covalent_map": (C.c_int, [C.c_void_p, C.POINTER(FgGrid), C.POINTER(C.c_double), C.c_double, C.c_double, _dp]),
+    "fg_floating_grid": (C.c_int, [C.c_void_p, C.POINTER(FgGrid), C.POINTER(FgAtoms), C.POINTER(C.c_float)]),
     "fg_affinity_d_maps": (C.c_int, [C.c_void_p, C.POINTER(FgGrid), C.POINTER(FgAtoms), C.POINTER(FgHbondGeom),
                                      C.POINTER(FgTables), C.POINTER(FgMapParams), C.c_int, C.c_double, C.c_int,
                                      C.POINTER(_dp), _dp]),
@@ -190,6 +194,12 @@ class JobArrays:
     solpar_q: float
     hydrogen: int
     dist_dep_diel: int
+    # covalent maps from their parameters when covalent is None
+    cov_on: int = 0
+    cov_point: tuple = (0.0, 0.0, 0.0)
+    cov_hw_sq_inv: float = 1.0
+    cov_barrier: float = 1000.0
+    floating_grid: int = 0
 
     @property
     def n_atoms(self) -> int:
@@ -237,6 +247,11 @@ class JobArrays:
         d.solpar_q = self.solpar_q
         d.hydrogen = self.hydrogen
         d.dist_dep_diel = self.dist_dep_diel
+        d.cov_on = int(self.cov_on)
+        d.cov_point = (C.c_double * 3)(*[float(v) for v in self.cov_point])
+        d.cov_hw_sq_inv = float(self.cov_hw_sq_inv)
+        d.cov_barrier = float(self.cov_barrier)
+        d.floating_grid = int(self.floating_grid)
         return d, keep
 
 
@@ -291,7 +306,9 @@ class Prepared:
             map_solpar=np.array([m.solpar_probe for m in mp], np.float64),
             map_vol=np.array([m.vol_probe for m in mp], np.float64),
             covalent=_arr(d.covalent, M * P, np.float64) if d.covalent else None,
-            solpar_q=d.solpar_q, hydrogen=d.hydrogen, dist_dep_diel=d.dist_dep_diel)
+            solpar_q=d.solpar_q, hydrogen=d.hydrogen, dist_dep_diel=d.dist_dep_diel,
+            cov_on=d.cov_on, cov_point=tuple(d.cov_point), cov_hw_sq_inv=d.cov_hw_sq_inv, cov_barrier=d.cov_barrier,
+            floating_grid=d.floating_grid)
 
     def write_outputs(self, maps: list[np.ndarray], program_name: str = "fastgrid4", threads: int = 0) -> None:
         keep = [np.ascontiguousarray(m, dtype=np.float64) for m in maps]
@@ -340,10 +357,32 @@ class Context:
                                _ptr(out, C.c_double)))
         return out
 
+    def covalent_map(self, arrays: JobArrays) -> np.ndarray:
+        """Initial values of the covalent maps (fg_covalent_map)."""
+        d, keep = arrays.desc()
+        out = np.zeros(arrays.P, np.float64)
+        pt = (C.c_double * 3)(*[float(v) for v in arrays.cov_point])
+        _check(LIB.fg_covalent_map(self._h, C.byref(d.grid), pt, float(arrays.cov_hw_sq_inv),
+                                   float(arrays.cov_barrier), _ptr(out, C.c_double)))
+        return out
+
+    def floating_grid(self, arrays: JobArrays) -> np.ndarray:
+        """The floating grid (fg_floating_grid), float32."""
+        d, keep = arrays.desc()
+        out = np.zeros(arrays.P, np.float32)
+        _check(LIB.fg_floating_grid(self._h, C.byref(d.grid), C.byref(d.atoms), _ptr(out, C.c_float)))
+        return out
+
     def affinity_d_maps(self, arrays: JobArrays) -> tuple[list[np.ndarray], np.ndarray]:
+        """Affinity and desolvation maps (fg_affinity_d_maps).  Covalent maps
+        start from arrays.covalent, else from fg_covalent_map."""
         d, keep = arrays.desc()
         P, M = arrays.P, arrays.n_maps
-        outs = [arrays.covalent[m * P:(m + 1) * P].copy() if arrays.covalent is not None else np.zeros(P)
+        cov = None
+        if arrays.covalent is None and arrays.cov_on:
+            cov = self.covalent_map(arrays)
+        outs = [arrays.covalent[m * P:(m + 1) * P].copy() if arrays.covalent is not None else
+                (cov.copy() if (cov is not None and arrays.map_is_covalent[m]) else np.zeros(P))
                 for m in range(M)]
         dmap = np.zeros(P)
         ptrs = (_dp * max(M, 1))(*[_ptr(o, C.c_double) for o in outs])
@@ -381,10 +420,11 @@ class Job:
         _check(LIB.fg_job_run(self.ctx.handle, self._h, what, precision))
 
     def fetch(self) -> list[np.ndarray]:
-        """[affinity maps..., e, d], unrounded FP64."""
+        """[affinity maps..., e, d] (+ the floating grid when the job has one), unrounded FP64."""
         P, M = self.arrays.P, self.arrays.n_maps
-        outs = [np.empty(P, np.float64) for _ in range(M + 2)]
-        ptrs = (_dp * (M + 2))(*[_ptr(o, C.c_double) for o in outs])
+        n = M + 2 + (1 if self.arrays.floating_grid else 0)
+        outs = [np.empty(P, np.float64) for _ in range(n)]
+        ptrs = (_dp * n)(*[_ptr(o, C.c_double) for o in outs])
         _check(LIB.fg_job_fetch(self.ctx.handle, self._h, ptrs))
         return outs
 

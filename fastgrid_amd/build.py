@@ -23,7 +23,7 @@ CXX = os.environ.get("CXX", "g++")
 
 HOST_SOURCES = ["params.cpp", "loader.cpp", "writers.cpp", "synth.cpp", "capi_host.cpp"]
 DEVICE_SOURCES = ["capi_device.hip"]
-HEADERS = ["host.hpp", "capi_common.hpp", "kernels.hpp", "kernels.hip"]
+HEADERS = ["host.hpp", "capi_common.hpp", "kernels.hpp", "kernels.hip", "kernels_plain.hip"]
 
 CXXFLAGS = ["-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-I" + INCLUDE]
 HIPFLAGS = [

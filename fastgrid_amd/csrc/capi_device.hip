@@ -94,6 +94,11 @@ struct fg_job {
     float *dPair32 = nullptr, *dDesolv32 = nullptr, *dAcc32 = nullptr;
     fgk::PlainMap pmp[16] = {};
     float solparQ = 0;
+    // covalent maps from their parameters, floating grid
+    int32_t covOn = 0;
+    double covPoint[3] = {0, 0, 0}, covHwSqInv = 1, covBarrier = 1000;
+    bool floating = false;
+    float *dFmap = nullptr;
     // outputs: [M affinity][e][d]
     double *dMaps = nullptr;
     std::vector<void *> owned;
@@ -191,6 +196,12 @@ fgk::CutoffArgs cutoffArgs(const fg_job *j, int doElecNear)
     a.eps = j->dEps;
     a.maps = j->dMaps;
     a.covalent = j->dCovalent;
+    a.covOn = j->covOn;
+    for (int k = 0; k < 3; ++k) a.covPoint[k] = j->covPoint[k];
+    a.covHwSqInv = j->covHwSqInv;
+    a.covBarrier = j->covBarrier;
+    a.N = j->N;
+    a.fmap = j->floating ? j->dFmap : nullptr;
     for (int m = 0; m < j->M && m < 16; ++m) a.mp[m] = j->mp[size_t(m)];
     return a;
 }
@@ -220,6 +231,16 @@ int runLists(fg_job *j, bool affinity)
     hipLaunchKernelGGL(fgk::k_brick_lists<true>, dim3(j->nb), dim3(fgk::kListThreads), 0, j->ctx->stream, la);
     if (affinity && j->typed)
         hipLaunchKernelGGL(fgk::k_type_sort, dim3(j->nb), dim3(fgk::kListThreads), 0, j->ctx->stream, la);
+    FG_HIP(hipGetLastError());
+    return FG_OK;
+}
+
+// Floating grid from the (unsorted) candidate lists of this run.
+int launchFmap(fg_job *j)
+{
+    fgk::CutoffArgs a = cutoffArgs(j, 0);
+    a.brickList = j->dList;
+    hipLaunchKernelGGL(fgk::k_fmap, dim3(unsigned(j->nb)), dim3(fgk::kCutThreads), 0, j->ctx->stream, a);
     FG_HIP(hipGetLastError());
     return FG_OK;
 }
@@ -349,7 +370,7 @@ int buildElecTiles(fg_job *j, const fg_job_desc &d, double gridMaxAbs)
             c[0] = A.x[i];
             c[1] = A.y[i];
             c[2] = A.z[i];
-            w = A.w[i];
+            w = A.w ? A.w[i] : 0.0;  // no weights: a floating-grid-only job
         } else {
             // padding: zero charge at the centre of a grid cell next to the
             // last atom, never on a grid point, inside the table's range
@@ -644,6 +665,10 @@ int runPlainAffinity(fg_job *j)
     a.desolv = j->dDesolv32;
     a.maps = j->dMaps;
     a.covalent = j->dCovalent;
+    a.covOn = j->covOn;
+    for (int k = 0; k < 3; ++k) a.covPoint[k] = j->covPoint[k];
+    a.covHwSqInv = j->covHwSqInv;
+    a.covBarrier = j->covBarrier;
     for (int m = 0; m < j->M && m < 16; ++m) a.mp[m] = j->pmp[m];
     const dim3 grid(unsigned((j->P + fgk::kPlainCutThreads - 1) / fgk::kPlainCutThreads)), block(fgk::kPlainCutThreads);
     hipStream_t s = j->ctx->stream;
@@ -823,7 +848,7 @@ int tuneJobs(fg_ctx *ctx, fg_job *const *jobs, int n, bool comp)
 struct fg_batch {
     fg_ctx *ctx = nullptr;
     std::vector<fg_job *> jobs;  // largest first
-    bool aff = false, near = false, elec = false, comp = true;
+    bool aff = false, near = false, elec = false, comp = true, fmap = false;
     int MM = 4, KH = 0, pz = 4;
     bool pointAng = false, clamp = false, typed = true;
     int sharedTableLen = 0, maxTableLen = 0;
@@ -931,7 +956,7 @@ int enqueueBatch(fg_batch *b, hipEvent_t mid, bool launch = true)
     hipStream_t s = b->ctx->stream;
     const int n = int(b->jobs.size());
     if (launch) hipLaunchKernelGGL(fgk::k_fill_i32, dim3(1), dim3(256), 0, s, b->dCounters, 2, 0);
-    if (b->aff || b->near) {
+    if (b->aff || b->near || b->fmap) {
         if (launch) {
             // list offsets are fixed at job creation: fill (and sort) only
             hipLaunchKernelGGL(fgk::k_brick_lists_batch<true>, dim3(b->cutItems), dim3(fgk::kListThreads), 0, s,
@@ -940,8 +965,12 @@ int enqueueBatch(fg_batch *b, hipEvent_t mid, bool launch = true)
                 hipLaunchKernelGGL(fgk::k_type_sort_batch, dim3(b->cutItems), dim3(fgk::kListThreads), 0, s, b->dList,
                                    b->dCutBase, n);
         }
-        int rc;
-        if (!b->aff) rc = launchCutoffBatch<1, 0, false, false, false>(b, launch);
+        if (b->fmap && launch)
+            hipLaunchKernelGGL(fgk::k_fmap_batch, dim3(b->cutItems), dim3(fgk::kCutThreads), 0, s, b->dCut, b->dCutBase,
+                               n);
+        int rc = FG_OK;
+        if (!b->aff && !b->near) rc = FG_OK;
+        else if (!b->aff) rc = launchCutoffBatch<1, 0, false, false, false>(b, launch);
         else if (b->MM == 4) rc = launchCutoffBatchKH<4>(b, launch);
         else if (b->MM == 6) rc = launchCutoffBatchKH<6>(b, launch);
         else if (b->MM == 8) rc = launchCutoffBatchKH<8>(b, launch);
@@ -967,6 +996,7 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
     b->elec = (what & FG_RUN_ELEC) != 0;
     b->near = b->elec && comp;
     b->aff = (what & FG_RUN_AFFINITY) != 0;
+    for (fg_job *j : b->jobs) b->fmap = b->fmap || ((what & FG_RUN_FLOATING) && j->floating);
     // one pz for the batch: the one carrying the most work
     double work[2] = {0, 0};
     int maxM = 0;
@@ -1105,6 +1135,11 @@ int fg_job_create(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
     j->T = d->tables.n_types;
     j->distDepDiel = d->dist_dep_diel;
     j->hydrogen = d->hydrogen;
+    j->covOn = d->covalent ? 0 : d->cov_on;
+    for (int k = 0; k < 3; ++k) j->covPoint[k] = d->cov_point[k];
+    j->covHwSqInv = d->cov_hw_sq_inv;
+    j->covBarrier = d->cov_barrier;
+    j->floating = d->floating_grid != 0;
     const fg_atoms &A = d->atoms;
     j->hasGeom = A.charge && A.vol && A.solpar && A.type && A.hbond && d->geom.rvector && d->geom.rvector2 &&
                  d->geom.rexp && d->geom.disorder && d->tables.pair && d->tables.desolv && (j->M == 0 || d->maps);
@@ -1182,6 +1217,7 @@ int fg_job_create(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
     if (!rc) rc = buildElecTiles(j, *d, gridMaxAbs);
     if (!rc) rc = buildPlain(j, *d);
     if (!rc) rc = devAlloc(j, j->dMaps, size_t(j->M + 2) * size_t(j->P));
+    if (!rc && j->floating) rc = devAlloc(j, j->dFmap, size_t(j->P));
     if (!rc) {
         hipError_t e = hipMemsetAsync(j->dMaps, 0, size_t(j->M + 2) * size_t(j->P) * sizeof(double), ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -1212,12 +1248,15 @@ int fg_job_run(fg_ctx *ctx, fg_job *j, unsigned what, int precision)
     hipStream_t s = ctx->stream;
     const bool elec = what & FG_RUN_ELEC, aff = (what & FG_RUN_AFFINITY) != 0;
     const bool near = elec && precision == FG_PREC_COMPENSATED;
+    const bool fmap = (what & FG_RUN_FLOATING) && j->floating;
     FG_HIP(hipEventRecord(ctx->ev[0], s));
     if (precision == FG_PREC_PLAIN) {
-        if (aff && j->hasGeom) {
+        if ((aff && j->hasGeom) || fmap)
             if (int rc = runLists(j, false)) return rc;
+        if (aff && j->hasGeom)
             if (int rc = runPlainAffinity(j)) return rc;
-        }
+        if (fmap)
+            if (int rc = launchFmap(j)) return rc;
         FG_HIP(hipEventRecord(ctx->ev[1], s));
         if (elec)
             if (int rc = runPlainElec(j)) return rc;
@@ -1226,10 +1265,13 @@ int fg_job_run(fg_ctx *ctx, fg_job *j, unsigned what, int precision)
         ctx->splitTiming = true;
         return FG_OK;
     }
-    if (aff || near) {
+    if (aff || near || fmap) {
         const bool affMaps = aff && j->hasGeom;
         if (int rc = runLists(j, affMaps)) return rc;
-        if (int rc = launchCutoff(j, affMaps, near ? 1 : 0)) return rc;
+        if (aff || near)
+            if (int rc = launchCutoff(j, affMaps, near ? 1 : 0)) return rc;
+        if (fmap)
+            if (int rc = launchFmap(j)) return rc;
     }
     FG_HIP(hipEventRecord(ctx->ev[1], s));
     if (elec)
@@ -1272,7 +1314,13 @@ int fg_job_fetch(fg_ctx *ctx, fg_job *j, double *const *out)
         if (out[m])
             FG_HIP(hipMemcpyAsync(out[m], j->dMaps + int64_t(m) * j->P, size_t(j->P) * sizeof(double),
                                   hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<float> fm;
+    if (j->floating && out[j->M + 2]) {
+        fm.resize(size_t(j->P));
+        FG_HIP(hipMemcpyAsync(fm.data(), j->dFmap, fm.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    }
     FG_HIP(hipStreamSynchronize(ctx->stream));
+    for (size_t p = 0; p < fm.size(); ++p) out[j->M + 2][p] = double(fm[p]);
     return FG_OK;
 }
 
@@ -1408,6 +1456,54 @@ int fg_elec_map(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, const d
     outs[0] = e_out;
     int rc = fg_job_run(ctx, j, FG_RUN_ELEC, precision);
     if (!rc) rc = fg_job_fetch(ctx, j, outs.data());
+    fg_job_destroy(j);
+    return rc;
+}
+
+int fg_covalent_map(fg_ctx *ctx, const fg_grid *grid, const double cov_point[3], double hw_sq_inv, double barrier,
+                    double *out)
+{
+    if (!ctx || !grid || !cov_point || !out) return fgc::fail(FG_E_ARG, "fg_covalent_map: NULL argument");
+    const fg_grid &g = *grid;
+    if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0 || !(g.spacing > 0)) return fgc::fail(FG_E_ARG, "fg_covalent_map: invalid grid");
+    FG_HIP(hipSetDevice(ctx->device));
+    fgk::CutoffArgs a{};
+    a.g = fgk::GridGeom{g.nx, g.ny, g.nz, g.div2x, g.div2y, g.div2z, g.spacing};
+    a.P = int64_t(g.nx) * g.ny * g.nz;
+    a.covOn = 1;
+    for (int k = 0; k < 3; ++k) a.covPoint[k] = cov_point[k];
+    a.covHwSqInv = hw_sq_inv;
+    a.covBarrier = barrier;
+    double *d = nullptr;
+    FG_HIP(hipMalloc(&d, size_t(a.P) * sizeof(double)));
+    hipLaunchKernelGGL(fgk::k_covalent, dim3(unsigned((a.P + 255) / 256)), dim3(256), 0, ctx->stream, a, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, size_t(a.P) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fgc::fail(FG_E_DEVICE, std::string("fg_covalent_map: ") + hipGetErrorString(e));
+    return FG_OK;
+}
+
+int fg_floating_grid(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, float *out)
+{
+    if (!ctx || !grid || !atoms || !out) return fgc::fail(FG_E_ARG, "fg_floating_grid: NULL argument");
+    fg_job_desc d{};
+    d.grid = *grid;
+    d.atoms.n = atoms->n;
+    d.atoms.x = atoms->x;
+    d.atoms.y = atoms->y;
+    d.atoms.z = atoms->z;
+    d.hydrogen = -1;
+    d.floating_grid = 1;
+    fg_job *j = nullptr;
+    if (int rc = fg_job_create(ctx, &d, &j)) return rc;
+    int rc = fg_job_run(ctx, j, FG_RUN_FLOATING, FG_PREC_COMPENSATED);
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(out, j->dFmap, size_t(j->P) * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = fgc::fail(FG_E_DEVICE, std::string("fg_floating_grid: ") + hipGetErrorString(e));
+    }
     fg_job_destroy(j);
     return rc;
 }

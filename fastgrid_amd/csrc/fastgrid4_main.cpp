@@ -171,8 +171,9 @@ int main(int argc, char **argv)
     fg_prep_desc(prep, &desc);
     const int M = desc.n_maps;
     const int64_t P = fg_prep_num_points(prep);
+    const int nOut = M + 2 + (desc.floating_grid ? 1 : 0);  // maps incl. the floating grid
     std::fprintf(log, "Beginning grid calculations.\n\nCalculating %d grids over %lld elements, around %d receptor atoms.\n\n",
-                 M + 2, (long long)P, desc.atoms.n);
+                 nOut, (long long)P, desc.atoms.n);
     fg_job *job = nullptr;
     if (fg_job_create(ctx, &desc, &job) != FG_OK) return fatal(0xbad, fg_last_error());
     const auto t2 = std::chrono::steady_clock::now();
@@ -180,8 +181,8 @@ int main(int argc, char **argv)
         return fatal(0xbad, fg_last_error());
     fg_timing tm{};
     fg_last_timing(ctx, &tm);
-    std::vector<std::vector<double>> host(size_t(M) + 2, std::vector<double>(size_t(P)));
-    std::vector<double *> ptrs(size_t(M) + 2);
+    std::vector<std::vector<double>> host(static_cast<size_t>(nOut), std::vector<double>(static_cast<size_t>(P)));
+    std::vector<double *> ptrs(static_cast<size_t>(nOut));
     for (size_t m = 0; m < host.size(); ++m) ptrs[m] = host[m].data();
     if (fg_job_fetch(ctx, job, ptrs.data()) != FG_OK) return fatal(0xbad, fg_last_error());
     fg_job_destroy(job);

@@ -81,7 +81,6 @@ struct Prepared {
     std::vector<double> rvec, rvec2;        // [N][3]
     std::vector<int32_t> rexp;
     std::vector<uint8_t> disorder;
-    std::vector<double> covalent;           // [numAtomMaps][P] when any map is covalent
     std::vector<fg_map_params> mapParams;
 
     double solparQ = .01097, invDielCal = 0, rSmooth = 0;

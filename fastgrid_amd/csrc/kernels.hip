@@ -317,6 +317,15 @@ __device__ __forceinline__ int binGuarded(double r, bool &edge)
     return k < kTableLen - 1 ? k : kTableLen - 1;
 }
 
+// Initial value of a covalent map at (gx, gy, gz) (FG/main.cpp:39-46).
+__device__ __forceinline__ double covalentValue(const CutoffArgs &a, double gx, double gy, double gz)
+{
+    const double dx = a.covPoint[0] - gx, dy = a.covPoint[1] - gy, dz = a.covPoint[2] - gz;
+    double r = (dx * dx + dy * dy + dz * dz) * a.covHwSqInv;
+    if (r < 1.0e-12) r = 1.0e-12;  // APPROX_ZERO^2
+    return a.covBarrier * (1 - exp(-0.69314718055994529 * r));
+}
+
 constexpr int kChunk = 32;     // candidate atoms staged in LDS per round
 constexpr int kRowBins = 801;  // table bins of pairs with r <= 8 A: floor(100 r) <= 800
 constexpr int kMaxTypes = 32;  // receptor types the type-sorted lists support
@@ -394,10 +403,12 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
 
     double e[MM], hmin[KH > 0 ? KH : 1], hmax[KH > 0 ? KH : 1];
     unsigned flags = 0;
+    const double cov = (kAffinity && a.covOn && !a.covalent) ? covalentValue(a, gx, gy, gz) : 0.0;
 #pragma unroll
     for (int m = 0; m < MM; ++m)
-        e[m] = (kAffinity && m < M && a.covalent && a.mp[m].isCovalent) ? a.covalent[int64_t(a.mp[m].out) * a.P + p]
-                                                                        : 0.0;
+        e[m] = (kAffinity && m < M && a.mp[m].isCovalent)
+                   ? (a.covalent ? a.covalent[int64_t(a.mp[m].out) * a.P + p] : cov)
+                   : 0.0;
 #pragma unroll
     for (int k = 0; k < (KH > 0 ? KH : 1); ++k) {
         hmin[k] = 999999;
@@ -566,6 +577,56 @@ __global__ __launch_bounds__(kCutThreads) void k_cutoff(CutoffArgs a)
     const int full = (nb / 8) * 8, g = int(blockIdx.x);
     const int b = g < full ? (g % 8) * (full / 8) + g / 8 : g;
     cutoffBrick<MM, KH, kAffinity, kPointAng, kTyped>(a, b, L);
+}
+
+// ------------------------------------------------------------------------
+// Floating grid (FG/main.cpp:54-79): float(sqrt(min over atoms of |a - p|^2)).
+// The nearest atom of a point within 8 A is in its brick's candidate list,
+// so the list decides every such point exactly; a wave with a point farther
+// than 8 A from every listed atom scans all atoms.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ void fmapBrick(const CutoffArgs &a, int b)
+{
+    const int N = a.N;
+    float *fmap = a.fmap;
+    const int bx = b % a.nbx, by = (b / a.nbx) % a.nby, bz = b / (a.nbx * a.nby);
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const int ix = bx * kCutBX + (w & 1) * 4 + (l & 3);
+    const int iy = by * kCutBY + (w >> 1) * 4 + ((l >> 2) & 3);
+    const int iz = bz * kCutBZ + (l >> 4);
+    const bool valid = ix < a.g.nx && iy < a.g.ny && iz < a.g.nz;
+    const double gx = (ix - a.g.dx2) * a.g.spacing;
+    const double gy = (iy - a.g.dy2) * a.g.spacing;
+    const double gz = (iz - a.g.dz2) * a.g.spacing;
+    double m2 = 1.0e24;  // BIG^2
+    for (int c = a.brickStart[b]; c < a.brickStart[b + 1]; ++c) {
+        const CutAtom &A = a.atoms[uniform(a.brickList[c])];
+        const double dx = A.x - gx, dy = A.y - gy, dz = A.z - gz;
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < m2) m2 = d2;
+    }
+    if (__ballot(valid && m2 > 64.0))
+        for (int ia = 0; ia < N; ++ia) {
+            const CutAtom &A = a.atoms[ia];
+            const double dx = A.x - gx, dy = A.y - gy, dz = A.z - gz;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 < m2) m2 = d2;
+        }
+    if (valid) fmap[(int64_t(iz) * a.g.ny + iy) * a.g.nx + ix] = float(sqrt(m2));
+}
+
+// Covalent map values alone (fg_covalent_map), one point per lane.
+__global__ __launch_bounds__(256) void k_covalent(CutoffArgs a, double *out)
+{
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (p >= a.P) return;
+    const int ix = int(p % a.g.nx), iy = int((p / a.g.nx) % a.g.ny), iz = int(p / (int64_t(a.g.nx) * a.g.ny));
+    out[p] = covalentValue(a, (ix - a.g.dx2) * a.g.spacing, (iy - a.g.dy2) * a.g.spacing, (iz - a.g.dz2) * a.g.spacing);
+}
+
+__global__ __launch_bounds__(kCutThreads) void k_fmap(CutoffArgs a)
+{
+    fmapBrick(a, int(blockIdx.x));
 }
 
 // Stable counting sort of brick b's candidate list by receptor type: list ->
@@ -892,6 +953,14 @@ __global__ __launch_bounds__(kCutThreads) void k_cutoff_batch(const CutoffArgs *
         const int job = uniform(jobOfItem(base, nJobs, item));
         cutoffBrick<MM, KH, kAffinity, kPointAng, kTyped>(jobs[job], item - base[job], L);
     }
+}
+
+// Floating grids of every job that has one: one workgroup per (job, brick) item.
+__global__ __launch_bounds__(kCutThreads) void k_fmap_batch(const CutoffArgs *jobs, const int32_t *base, int nJobs)
+{
+    const int item = int(blockIdx.x);
+    const int job = uniform(jobOfItem(base, nJobs, item));
+    if (jobs[job].fmap) fmapBrick(jobs[job], item - base[job]);
 }
 
 // Type sort of every job's lists: one workgroup per (job, brick) item.

@@ -106,6 +106,11 @@ struct CutoffArgs {
     const double *eps;     // [8192]
     double *maps;          // [M+2][P]: affinity maps, e (near part), d
     const double *covalent;  // [M][P] or null
+    // covalent maps from their parameters (FG/main.cpp:32-52) when covalent is null
+    int32_t covOn;
+    int32_t N;                // receptor atoms (the floating grid's full scan)
+    float *fmap;              // floating grid [P], or null
+    double covPoint[3], covHwSqInv, covBarrier;
     MapParam mp[16];
 };
 

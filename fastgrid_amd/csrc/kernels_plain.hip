@@ -108,6 +108,8 @@ struct PlainCutArgs {
     const float *desolv;   // [8192]
     double *maps;          // [M+2][P]
     const double *covalent;
+    int32_t covOn, pad2;   // covalent maps from their parameters when covalent is null
+    double covPoint[3], covHwSqInv, covBarrier;
     PlainMap mp[16];
 };
 
@@ -197,7 +199,19 @@ __global__ __launch_bounds__(kPlainCutThreads) void k_plain_cutoff(PlainCutArgs 
     for (int m = 0; m < MM; ++m) {
         if (m >= M) continue;
         double v = double(e[m] + (flag[m] ? hmin[m] + hmax[m] : 0.f));
-        if (a.mp[m].isCovalent) v = a.covalent ? a.covalent[int64_t(m) * a.P + p] : 0.0;
+        if (a.mp[m].isCovalent) {
+            if (a.covalent) {
+                v = a.covalent[int64_t(m) * a.P + p];
+            } else {
+                // FG/main.cpp:39-46
+                const double gxd = (ix - a.g.dx2) * a.g.spacing, gyd = (iy - a.g.dy2) * a.g.spacing,
+                             gzd = (iz - a.g.dz2) * a.g.spacing;
+                const double dx = a.covPoint[0] - gxd, dy = a.covPoint[1] - gyd, dz = a.covPoint[2] - gzd;
+                double r = (dx * dx + dy * dy + dz * dz) * a.covHwSqInv;
+                if (r < 1.0e-12) r = 1.0e-12;
+                v = a.covOn ? a.covBarrier * (1 - exp(-0.69314718055994529 * r)) : 0.0;
+            }
+        }
         a.maps[int64_t(m) * a.P + p] = v;
     }
     a.maps[int64_t(M + 1) * a.P + p] = double(dmap);

@@ -584,29 +584,6 @@ void bondVectors(Prepared &p, ParamLibrary &lib)
     (void)load;
 }
 
-// Covalent maps: covBarrier * (1 - exp(ln 0.5 * r^2/hw^2)) (FG/main.cpp:32-52).
-void covalentMaps(Prepared &p)
-{
-    const size_t M = size_t(p.numAtomMaps), P = size_t(p.P);
-    bool any = false;
-    for (size_t m = 0; m < M; ++m) any = any || p.maps[m].isCovalent;
-    if (!any) return;
-    p.covalent.assign(M * P, 0.0);
-    for (size_t m = 0; m < M; ++m) {
-        if (!p.maps[m].isCovalent) continue;
-        for (int z = 0; z < p.n[2]; ++z)
-            for (int y = 0; y < p.n[1]; ++y)
-                for (int x = 0; x < p.n[0]; ++x) {
-                    Vec3 g{(x - p.div2[0]) * p.spacing, (y - p.div2[1]) * p.spacing, (z - p.div2[2]) * p.spacing};
-                    Vec3 cp{p.covPoint[0], p.covPoint[1], p.covPoint[2]};
-                    double r = norm2(cp - g) * p.covHalfWidthSqInv;
-                    if (r < kApproxZero * kApproxZero) r = kApproxZero * kApproxZero;
-                    size_t idx = (size_t(z) * size_t(p.n[1]) + size_t(y)) * size_t(p.n[0]) + size_t(x);
-                    p.covalent[m * P + idx] = p.covBarrier * (1 - std::exp(-0.69314718055994529 * r));
-                }
-    }
-}
-
 }  // namespace
 
 int precompute(Prepared &p, ParamLibrary &lib, std::string &err)
@@ -632,7 +609,6 @@ int precompute(Prepared &p, ParamLibrary &lib, std::string &err)
     if (int rc = pairTables(p, err)) return rc;
     bondVectors(p, lib);
     p.hydrogen = lib.recIndexOf("HD");
-    covalentMaps(p);
     p.mapParams.resize(size_t(p.numAtomMaps));
     for (size_t m = 0; m < size_t(p.numAtomMaps); ++m) {
         const MapDef &d = p.maps[m];
@@ -659,7 +635,13 @@ fg_job_desc Prepared::desc() const
     d.tables = fg_tables{tableTypes, int32_t(numAtomMaps), pair.data(), desolv.data(), eps.data()};
     d.n_maps = numAtomMaps;
     d.maps = mapParams.data();
-    d.covalent = covalent.empty() ? nullptr : covalent.data();
+    // covalent maps are computed on the device from their parameters
+    d.covalent = nullptr;
+    for (int m = 0; m < numAtomMaps; ++m) d.cov_on = d.cov_on || maps[size_t(m)].isCovalent;
+    for (int k = 0; k < 3; ++k) d.cov_point[k] = covPoint[size_t(k)];
+    d.cov_hw_sq_inv = covHalfWidthSqInv;
+    d.cov_barrier = covBarrier;
+    d.floating_grid = fmapPath.empty() ? 0 : 1;
     d.solpar_q = solparQ;
     d.hydrogen = hydrogen;
     d.dist_dep_diel = distDepDiel ? 1 : 0;

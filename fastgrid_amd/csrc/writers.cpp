@@ -66,12 +66,25 @@ void formatRange(const double *v, int64_t lo, int64_t hi, std::string &buf, doub
     mx = b;
 }
 
+// Floating-grid values [lo, hi) (floats held as doubles), printed as the
+// reference prints them: "%.3f" of the value, no rounding step, no "0" form.
+void formatFloatRange(const double *v, int64_t lo, int64_t hi, std::string &buf)
+{
+    buf.resize(size_t(hi - lo) * 32);
+    char *p = &buf[0];
+    for (int64_t i = lo; i < hi; ++i) p += std::snprintf(p, 32, "%.3f\n", v[i]);
+    buf.resize(size_t(p - &buf[0]));
+}
+
 }  // namespace
 
 int writeOutputs(Prepared &p, double *const *maps, const std::string &programName, int threads,
                  std::string &err)
 {
     const int nm = int(p.maps.size());
+    // the floating grid is one more variable of the field (FG/Utils.cpp:51-81)
+    const bool fmap = !p.fmapPath.empty();
+    const int nmF = nm + (fmap ? 1 : 0);
     // AVS field file
     FILE *f = std::fopen(p.fldPath.c_str(), "w");
     if (!f) {
@@ -89,18 +102,20 @@ int writeOutputs(Prepared &p, double *const *maps, const std::string &programNam
     std::fprintf(f, "dim2=%d\t\t\t# number of y-elements\n", p.n[1]);
     std::fprintf(f, "dim3=%d\t\t\t# number of z-elements\n", p.n[2]);
     std::fprintf(f, "nspace=3\t\t# number of physical coordinates per point\n");
-    std::fprintf(f, "veclen=%d\t\t# number of affinity values at each point\n", nm);
+    std::fprintf(f, "veclen=%d\t\t# number of affinity values at each point\n", nmF);
     std::fprintf(f, "data=float\t\t# data type (byte, integer, float, double)\n");
     std::fprintf(f, "field=uniform\t\t# field type (uniform, rectilinear, irregular)\n");
     for (int i = 0; i < 3; ++i)
         std::fprintf(f, "coord %d file=%s filetype=ascii offset=%d\n", i + 1, p.xyzPath.c_str(), i * 2);
     for (int i = 0; i < p.numAtomMaps; ++i)
         std::fprintf(f, "label=%s-affinity\t# component label for variable %d\n", p.maps[size_t(i)].type.c_str(), i + 1);
-    std::fprintf(f, "label=Electrostatics\t# component label for variable %d\n", nm - 2);
-    std::fprintf(f, "label=Desolvation\t# component label for variable %d\n", nm - 1);
+    std::fprintf(f, "label=Electrostatics\t# component label for variable %d\n", nmF - 2);
+    std::fprintf(f, "label=Desolvation\t# component label for variable %d\n", nmF - 1);
+    if (fmap) std::fprintf(f, "label=Floating_Grid\t# component label for variable %d\n", nmF);
     std::fprintf(f, "#\n# location of affinity grid files and how to read them\n#\n");
     for (int i = 0; i < nm; ++i)
         std::fprintf(f, "variable %d file=%s filetype=ascii skip=6\n", i + 1, p.maps[size_t(i)].filename.c_str());
+    if (fmap) std::fprintf(f, "variable %d file=%s filetype=ascii skip=6\n", nmF, p.fmapPath.c_str());
     std::fclose(f);
 
     char hdr[2048];
@@ -145,6 +160,28 @@ int writeOutputs(Prepared &p, double *const *maps, const std::string &programNam
         }
         p.mapMin[size_t(m)] = a;
         p.mapMax[size_t(m)] = b;
+    }
+    if (fmap && maps[nm]) {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; ++t) {
+            int64_t lo = P * t / threads, hi = P * (t + 1) / threads;
+            pool.emplace_back(formatFloatRange, maps[nm], lo, hi, std::ref(chunks[size_t(t)]));
+        }
+        for (auto &th : pool) th.join();
+        FILE *mf = std::fopen(p.fmapPath.c_str(), "w");
+        if (!mf) {
+            err = "can't open grid map \"" + p.fmapPath + "\" for writing.";
+            return -2;
+        }
+        bool ok = std::fwrite(hdr, size_t(hl), 1, mf) == 1;
+        for (int t = 0; t < threads; ++t)
+            if (!chunks[size_t(t)].empty())
+                ok = ok && std::fwrite(chunks[size_t(t)].data(), chunks[size_t(t)].size(), 1, mf) == 1;
+        ok = (std::fclose(mf) == 0) && ok;
+        if (!ok) {
+            err = "Not enough disk space.";
+            return -2;
+        }
     }
     return 0;
 }

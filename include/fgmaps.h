@@ -62,7 +62,8 @@ enum {
 /* What a job run computes. */
 #define FG_RUN_ELEC 1u
 #define FG_RUN_AFFINITY 2u /* affinity maps + desolvation map */
-#define FG_RUN_ALL 3u
+#define FG_RUN_FLOATING 4u /* the floating grid (jobs created with floating_grid = 1) */
+#define FG_RUN_ALL 7u
 
 typedef struct fg_ctx fg_ctx;
 typedef struct fg_job fg_job;
@@ -122,6 +123,16 @@ typedef struct {
     double solpar_q;                    /* solparQ */
     int32_t hydrogen;                   /* receptor type index of "HD", or -1 */
     int32_t dist_dep_diel;              /* 1: binned eps table, 0: constant dielectric */
+    /* Covalent maps computed on the device (FG/main.cpp:32-52), used when
+     * covalent is NULL and cov_on is 1: every covalent map starts from
+     * cov_barrier * (1 - exp(ln(1/2) * max(r, 1e-12))) with
+     * r = |cov_point - p|^2 * cov_hw_sq_inv (cov_point grid-centred). */
+    int32_t cov_on;
+    double cov_point[3];
+    double cov_hw_sq_inv, cov_barrier;
+    /* 1: the job also computes the floating grid (GPF keyword fmap,
+     * FG/main.cpp:54-79): float(sqrt(min over atoms of |atom - p|^2)). */
+    int32_t floating_grid;
 } fg_job_desc;
 
 /* ---------------------------------------------------------------- */
@@ -160,6 +171,17 @@ int fg_affinity_d_maps(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms,
                        const fg_map_params *maps, int n_maps, double solpar_q, int hydrogen,
                        double *const *maps_out, double *d_out);
 
+/* Initial values of a covalent map.  Replaces initCovalentMaps
+ * (FG/main.cpp:32-52): out[p] = barrier * (1 - exp(ln(1/2) * max(r, 1e-12))),
+ * r = |cov_point - p|^2 * hw_sq_inv, cov_point grid-centred. */
+int fg_covalent_map(fg_ctx *ctx, const fg_grid *grid, const double cov_point[3], double hw_sq_inv,
+                    double barrier, double *out);
+
+/* Floating grid.  Replaces calculateFloatingGrid (FG/main.cpp:54-79):
+ * out[p] = float(sqrt(min over atoms of |atom - p|^2)); only atoms->n and
+ * atoms->x/y/z are read. */
+int fg_floating_grid(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, float *out);
+
 /* ---------------------------------------------------------------- */
 /* Device-resident jobs (no reference counterpart: they split the   */
 /* one-shot calls into upload / compute / fetch so inputs and maps  */
@@ -172,7 +194,9 @@ int fg_job_run(fg_ctx *ctx, fg_job *job, unsigned what, int precision);
 /* Wait for everything enqueued on the context. */
 int fg_sync(fg_ctx *ctx);
 /* Copy maps back: out[m] for the n_maps affinity maps, out[n_maps] = e,
- * out[n_maps+1] = d (P doubles each; NULL entries are skipped). */
+ * out[n_maps+1] = d, and out[n_maps+2] = the floating grid when the job has
+ * one (P doubles each, the floating grid's values are floats; NULL entries
+ * are skipped). */
 int fg_job_fetch(fg_ctx *ctx, fg_job *job, double *const *out);
 /* Number of (point, atom) pairs with r <= 8 A (computed on the device). */
 int fg_job_cutoff_pairs(fg_ctx *ctx, fg_job *job, int64_t *pairs);
@@ -265,7 +289,9 @@ const char *fg_prep_gpf_echo(const fg_prep *prep);
 int fg_prep_summary(const fg_prep *prep, int m, double *min, double *max);
 
 /* .fld + .map files (FG/GridMap.cpp:127-212, FG/Utils.cpp:40-84).  maps[m]
- * for m < n_maps, maps[n_maps] = e, maps[n_maps+1] = d, unrounded.  Also
+ * for m < n_maps, maps[n_maps] = e, maps[n_maps+1] = d, unrounded, and
+ * maps[n_maps+2] = the floating grid when the GPF names one (fmap; printed
+ * with "%.3f" as the reference prints its float values).  Also
  * records each map's min/max in the prep for fg_prep_summary. */
 int fg_write_outputs(fg_prep *prep, double *const *maps, const char *program_name, int threads);
 

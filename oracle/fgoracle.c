@@ -1047,24 +1047,49 @@ long long or_count_cutoff_pairs(const int n[3], const int div2[3], double spacin
 }
 
 /* FG/main.cpp:32-52 */
+void or_covalent_map(const int n[3], const int div2[3], double spacing, const double covPoint[3],
+                     double covHalfWidthSqInv, double covBarrier, double *out)
+{
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int iz = 0; iz < n[2]; iz++)
+        for (int iy = 0; iy < n[1]; iy++)
+            for (int ix = 0; ix < n[0]; ix++) {
+                double g[3] = {(ix - div2[0]) * spacing, (iy - div2[1]) * spacing, (iz - div2[2]) * spacing};
+                double d[3];
+                v_sub(d, covPoint, g);
+                double r = v_msq(d) * covHalfWidthSqInv;
+                if (r < APPROX_ZERO * APPROX_ZERO) r = APPROX_ZERO * APPROX_ZERO;
+                out[((size_t)iz * n[1] + iy) * n[0] + ix] = covBarrier * (1 - exp(-0.69314718055994529 * r));
+            }
+}
+
 static void covalent_maps(or_input *in)
 {
-    for (int m = 0; m < in->numAtomMaps; m++) {
-        if (!in->maps[m].isCovalent) continue;
+    for (int m = 0; m < in->numAtomMaps; m++)
+        if (in->maps[m].isCovalent)
+            or_covalent_map(in->n, in->div2, in->spacing, in->covPoint, in->covHalfWidthSqInv, in->covBarrier,
+                            in->maps[m].energies);
+}
+
+/* FG/main.cpp:54-79 */
+void or_floating_grid(const int n[3], const int div2[3], double spacing, int N, const double *x, const double *y,
+                      const double *z, float *out, int threads)
+{
+    if (threads > 0) omp_set_num_threads(threads);
 #pragma omp parallel for schedule(dynamic, 1)
-        for (int iz = 0; iz < in->n[2]; iz++)
-            for (int iy = 0; iy < in->n[1]; iy++)
-                for (int ix = 0; ix < in->n[0]; ix++) {
-                    double g[3] = {(ix - in->div2[0]) * in->spacing, (iy - in->div2[1]) * in->spacing,
-                                   (iz - in->div2[2]) * in->spacing};
-                    double d[3];
-                    v_sub(d, in->covPoint, g);
-                    double r = v_msq(d) * in->covHalfWidthSqInv;
-                    if (r < APPROX_ZERO * APPROX_ZERO) r = APPROX_ZERO * APPROX_ZERO;
-                    in->maps[m].energies[((size_t)iz * in->n[1] + iy) * in->n[0] + ix] =
-                        in->covBarrier * (1 - exp(-0.69314718055994529 * r));
+    for (int iz = 0; iz < n[2]; iz++)
+        for (int iy = 0; iy < n[1]; iy++)
+            for (int ix = 0; ix < n[0]; ix++) {
+                double g[3] = {(ix - div2[0]) * spacing, (iy - div2[1]) * spacing, (iz - div2[2]) * spacing};
+                double minDistSq = 1.0E12 * 1.0E12;
+                for (int ia = 0; ia < N; ia++) {
+                    double a[3] = {x[ia], y[ia], z[ia]}, d[3];
+                    v_sub(d, a, g);
+                    double distSq = v_msq(d);
+                    if (distSq < minDistSq) minDistSq = distSq;
                 }
-    }
+                out[((size_t)iz * n[1] + iy) * n[0] + ix] = (float)sqrt(minDistSq);
+            }
 }
 
 void or_compute(or_input *in, int mode, int threads)
@@ -1091,6 +1116,10 @@ void or_compute(or_input *in, int mode, int threads)
             if (in->maps[m].energies) memcpy(in->maps[m].energies, &maps[(size_t)m * P], sizeof(double) * P);
         free(maps);
     }
+    if ((mode & 4) && in->fmap[0]) {
+        if (!in->fmapValues) in->fmapValues = malloc(sizeof(float) * (size_t)P);
+        or_floating_grid(in->n, in->div2, in->spacing, in->N, in->x, in->y, in->z, in->fmapValues, threads);
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -1107,6 +1136,8 @@ int or_write_outputs(or_input *in, const char *programName)
     FILE *f = fopen(in->fld, "w");
     if (!f) return -2;
     int nm = in->numMaps;
+    /* the floating grid is one more variable of the field (FG/Utils.cpp:51-81) */
+    int fm = in->fmap[0] != 0, nmF = nm + fm;
     fprintf(f, "# AVS field file\n#\n# AutoDock Atomic Affinity and Electrostatic Grids\n#\n# Created by %s.\n#\n", programName);
     fprintf(f, "#SPACING %.3f\n", (double)(float)in->spacing);
     fprintf(f, "#NELEMENTS %d %d %d\n", in->n[0] - 1, in->n[1] - 1, in->n[2] - 1);
@@ -1117,17 +1148,19 @@ int or_write_outputs(or_input *in, const char *programName)
     fprintf(f, "dim2=%d\t\t\t# number of y-elements\n", in->n[1]);
     fprintf(f, "dim3=%d\t\t\t# number of z-elements\n", in->n[2]);
     fprintf(f, "nspace=3\t\t# number of physical coordinates per point\n");
-    fprintf(f, "veclen=%d\t\t# number of affinity values at each point\n", nm);
+    fprintf(f, "veclen=%d\t\t# number of affinity values at each point\n", nmF);
     fprintf(f, "data=float\t\t# data type (byte, integer, float, double)\n");
     fprintf(f, "field=uniform\t\t# field type (uniform, rectilinear, irregular)\n");
     for (int i = 0; i < 3; i++) fprintf(f, "coord %d file=%s filetype=ascii offset=%d\n", i + 1, in->xyz, i * 2);
     for (int i = 0; i < in->numAtomMaps; i++)
         fprintf(f, "label=%s-affinity\t# component label for variable %d\n", in->maps[i].type, i + 1);
-    fprintf(f, "label=Electrostatics\t# component label for variable %d\n", nm - 2);
-    fprintf(f, "label=Desolvation\t# component label for variable %d\n", nm - 1);
+    fprintf(f, "label=Electrostatics\t# component label for variable %d\n", nmF - 2);
+    fprintf(f, "label=Desolvation\t# component label for variable %d\n", nmF - 1);
+    if (fm) fprintf(f, "label=Floating_Grid\t# component label for variable %d\n", nmF);
     fprintf(f, "#\n# location of affinity grid files and how to read them\n#\n");
     for (int i = 0; i < nm; i++)
         fprintf(f, "variable %d file=%s filetype=ascii skip=6\n", i + 1, in->maps[i].filename);
+    if (fm) fprintf(f, "variable %d file=%s filetype=ascii skip=6\n", nmF, in->fmap);
     fclose(f);
     for (int i = 0; i < nm; i++) {
         if (!in->maps[i].filename[0]) continue;
@@ -1141,11 +1174,21 @@ int or_write_outputs(or_input *in, const char *programName)
         }
         fclose(mf);
     }
+    if (fm && in->fmapValues) {
+        /* FG/GridMap.cpp:160-181: "%.3f" of each float */
+        FILE *mf = fopen(in->fmap, "w");
+        if (!mf) return -2;
+        fwrite(hdr, (size_t)hl, 1, mf);
+        for (int j = 0; j < in->P; j++) fprintf(mf, "%.3f\n", in->fmapValues[j]);
+        fclose(mf);
+    }
     return 0;
 }
 
 void or_free(or_input *in)
 {
+    free(in->fmapValues);
+    in->fmapValues = NULL;
     free(in->x); free(in->y); free(in->z); free(in->w); free(in->charge); free(in->vol); free(in->solpar);
     free(in->type); free(in->hbond); free(in->rvec); free(in->rvec2); free(in->rexp); free(in->disorder);
     free(in->pair);

@@ -67,6 +67,7 @@ typedef struct {
     /* files */
     char gpfName[OR_NAME], fld[OR_NAME], xyz[OR_NAME], receptor[OR_NAME];
     char fmap[OR_NAME], paramFile[OR_NAME];
+    float *fmapValues;     /* floating grid [P] when fmap is set */
     /* grid */
     int n[3], div2[3], P;
     double spacing, center[3];
@@ -99,7 +100,8 @@ typedef struct {
 /* Returns 0 on success, else the reference's exit code (ExitProgram) and err. */
 int or_load(or_input *in, const char *gpf, int v4, int agParamOrder, FILE *log);
 int or_precompute(or_input *in);
-/* mode bit 0: e-map, bit 1: affinity + d maps.  threads <= 0: OpenMP default. */
+/* mode bit 0: e-map, bit 1: affinity + d maps, bit 2: floating grid (when the
+ * GPF names one).  threads <= 0: OpenMP default. */
 void or_compute(or_input *in, int mode, int threads);
 int or_write_outputs(or_input *in, const char *programName);
 void or_free(or_input *in);
@@ -125,6 +127,15 @@ void or_affinity_maps(const int n[3], const int div2[3], double spacing,
                       const int *mapHbond, const int *mapIsHBonder, const int *mapIsCovalent,
                       const double *mapSolpar, const double *mapVol,
                       double solparQ, int hydrogen, double *maps, double *d, int round3, int threads);
+
+/* FG/main.cpp:54-79: out[p] = float(sqrt(min over atoms of |a - p|^2)),
+ * starting from BIG^2 = 1e24 (autodock/autocomm.h:75). */
+void or_floating_grid(const int n[3], const int div2[3], double spacing, int N, const double *x, const double *y,
+                      const double *z, float *out, int threads);
+
+/* FG/main.cpp:32-52: one covalent map's values. */
+void or_covalent_map(const int n[3], const int div2[3], double spacing, const double covPoint[3],
+                     double covHalfWidthSqInv, double covBarrier, double *out);
 
 /* Tables (FG/autogrid.h:106-112, FG/PairwiseInteractionEnergies.cpp:59-115,
  * FG/DesolvExpFunc.cpp:27-43). */

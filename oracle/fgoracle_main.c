@@ -38,7 +38,7 @@ int main(int argc, char **argv)
     double t1 = omp_get_wtime();
     or_compute(&in, 1, threads);
     double t2 = omp_get_wtime();
-    or_compute(&in, 2, threads);
+    or_compute(&in, 2 | 4, threads);
     double t3 = omp_get_wtime();
     rc = or_write_outputs(&in, argv[0]);
     double t4 = omp_get_wtime();

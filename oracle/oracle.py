@@ -52,6 +52,8 @@ def _load() -> C.CDLL:
                                      C.c_double, C.c_int, _dp, _dp, C.c_int, C.c_int]
     lib.or_count_cutoff_pairs.restype = C.c_longlong
     lib.or_count_cutoff_pairs.argtypes = [_ip, _ip, C.c_double, C.c_int, _dp, _dp, _dp, C.c_int]
+    lib.or_floating_grid.argtypes = [_ip, _ip, C.c_double, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_float), C.c_int]
+    lib.or_covalent_map.argtypes = [_ip, _ip, C.c_double, _dp, C.c_double, C.c_double, _dp]
     lib.or_ddd_table.argtypes = [_dp]
     lib.or_desolv_table.argtypes = [C.c_double, _dp]
     return lib
@@ -144,6 +146,11 @@ def affinity_maps(a, round3: bool = False, threads: int = 0):
     maps = np.zeros(max(M, 1) * P)
     if a.covalent is not None:
         maps[:M * P] = a.covalent
+    elif getattr(a, "cov_on", 0):
+        cov = covalent_map(a)
+        for m in range(M):
+            if a.map_is_covalent[m]:
+                maps[m * P:(m + 1) * P] = cov
     d = np.zeros(P)
     f = lambda v: np.ascontiguousarray(v, np.float64)
     i = lambda v: np.ascontiguousarray(v, np.int32)
@@ -159,6 +166,24 @@ def affinity_maps(a, round3: bool = False, threads: int = 0):
                          _p(mhb, C.c_int), _p(mis, C.c_int), _p(mcov, C.c_int), _p(msol), _p(mvol),
                          float(a.solpar_q), int(a.hydrogen), _p(maps), _p(d), int(round3), threads)
     return [maps[m * P:(m + 1) * P].copy() for m in range(M)], d
+
+
+def covalent_map(a) -> np.ndarray:
+    """Values of a covalent map from JobArrays-like parameters (FG/main.cpp:32-52)."""
+    out = np.zeros(a.P)
+    pt = np.ascontiguousarray(a.cov_point, np.float64)
+    LIB.or_covalent_map(_i3(a.n), _i3(a.div2), a.spacing, _p(pt), float(a.cov_hw_sq_inv), float(a.cov_barrier),
+                        _p(out))
+    return out
+
+
+def floating_grid(a, threads: int = 0) -> np.ndarray:
+    """Floating grid (FG/main.cpp:54-79), float32."""
+    out = np.zeros(a.P, np.float32)
+    x, y, z = (np.ascontiguousarray(v, np.float64) for v in (a.x, a.y, a.z))
+    LIB.or_floating_grid(_i3(a.n), _i3(a.div2), a.spacing, a.n_atoms, _p(x), _p(y), _p(z),
+                         out.ctypes.data_as(C.POINTER(C.c_float)), threads)
+    return out
 
 
 def cutoff_pairs(a, threads: int = 0) -> int:

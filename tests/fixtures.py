@@ -96,6 +96,15 @@ def variant(tmp, name):
     return run, gpf, meta["names"], meta["args"]
 
 
+def fld_lines(path):
+    """An AVS field file's lines without the "Created by <program>" line."""
+    return [l for l in open(path).read().split("\n") if not l.startswith("# Created by")]
+
+
+def variant_fld(name):
+    return open(os.path.join(GOLDEN, "variants", f"{name}.fld")).read().split("\n")
+
+
 VARIANTS = sorted(json.load(open(os.path.join(GOLDEN, "variants", "variants.json"))))
 
 

@@ -108,6 +108,8 @@ VARIANTS = {
                        ("map colagen2.OA.map", "map colagen2.OA.map\nmap colagen2.Z.map\ncovalentmap 1.7 1000.0 64.5 -16.0 37.2")],
                       ["qasp 0.01234"], ["--v4"]),
     "odd_grid_ad41": ([("npts 40 40 40", "npts 21 34 9"), ("spacing 1 ", "spacing 0.4 ")], [], []),
+    # floating grid: the receptor's atoms reach past the grid on every side
+    "floating_grid": ([("npts 40 40 40", "npts 26 30 22")], ["fmap colagen2.fmap.map"], ["--v4"]),
 }
 
 
@@ -133,7 +135,13 @@ def variants():
             open(os.path.join(d, "colagen2.gpf"), "w").write(text)
             run_ref("../colagen2.gpf", run, args)
             names = [l.split()[1].split(".")[1] for l in text.split("\n") if l.startswith("map ")] + ["e", "d"]
+            if any(l.startswith("fmap ") for l in extra):
+                names.append("fmap")
             save_maps(os.path.join(out, f"{name}.npz"), [os.path.join(run, f"colagen2.{n}.map") for n in names], names)
+            # the AVS field file without its "Created by <program>" line
+            fld = [l for l in open(os.path.join(run, "colagen2.maps.fld")).read().split("\n")
+                   if not l.startswith("# Created by")]
+            open(os.path.join(out, f"{name}.fld"), "w").write("\n".join(fld))
             meta[name] = {"args": args, "names": names, "edits": edits, "extra": extra}
     json.dump(meta, open(os.path.join(out, "variants.json"), "w"), indent=1)
 

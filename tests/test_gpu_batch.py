@@ -27,8 +27,11 @@ POCKETS = [
 
 @pytest.fixture(scope="module")
 def pockets(tmp_path_factory):
+    import dataclasses
     tmp = str(tmp_path_factory.mktemp("pockets"))
-    return [fx.pocket(tmp, f"p{i}", seed, n, npts, lig) for i, (seed, n, npts, lig) in enumerate(POCKETS)]
+    arrays = [fx.pocket(tmp, f"p{i}", seed, n, npts, lig) for i, (seed, n, npts, lig) in enumerate(POCKETS)]
+    # two pockets also compute their floating grid
+    return [dataclasses.replace(a, floating_grid=1) if i in (1, 3) else a for i, a in enumerate(arrays)]
 
 
 def _jobs(ctx, arrays):
@@ -118,6 +121,8 @@ def test_batch_against_oracle(ctx, pockets):
             assert np.array_equal(fx.milli(out[m]), fx.milli(om[m])), m
         assert np.array_equal(fx.milli(out[M + 1]), fx.milli(od))
         assert np.abs(out[M] - oe).max() < 3e-4
+        if a.floating_grid:
+            assert np.array_equal(out[M + 2].astype(np.float32), orc.floating_grid(a))
     for j in jobs:
         j.close()
 

@@ -67,6 +67,7 @@ def test_cli_gpf_variant_goldens(tmp_path, name):
     run, gpf, names, args = fx.variant(str(tmp_path), name)
     fastgrid4(run, gpf, *args)
     compare_files(run, "colagen2", fx.goldens("variants", f"{name}.npz"))
+    assert fx.fld_lines(os.path.join(run, "colagen2.maps.fld")) == fx.variant_fld(name)
 
 
 @pytest.mark.parametrize("name", ["micro", "S"])
@@ -161,6 +162,25 @@ def test_kernels_vs_oracle_gpf_variants(ctx, tmp_path, variant):
     _edit_gpf(d, "colagen2.gpf", d, edits, extra)
     a = fx.load_arrays(run, gpf, v4=True)
     _kernel_check(ctx, a)
+
+
+def test_floating_grid_and_covalent_map_one_shots(ctx, tmp_path):
+    """fg_floating_grid and fg_covalent_map against the oracle: the floating
+    grid bit for bit (FP64 minimum, one sqrt, one cast to float), on a grid
+    around the receptor and on one farther than 8 A from every atom (the
+    full-scan path); the covalent map to FP64 rounding of exp."""
+    from oracle import oracle as orc
+    run, gpf, names, args = fx.variant(str(tmp_path), "covalent_qasp")
+    a = fx.load_arrays(run, gpf, v4=True)
+    assert a.cov_on and a.covalent is None
+    assert np.array_equal(ctx.floating_grid(a), orc.floating_grid(a))
+    assert np.allclose(ctx.covalent_map(a), orc.covalent_map(a), rtol=1e-13, atol=1e-12)
+    import copy
+    far = copy.copy(a)
+    far.div2 = (a.div2[0] - 400, a.div2[1], a.div2[2])  # the grid moved 150 A along x
+    fm = ctx.floating_grid(far)
+    assert fm.min() > 8.0
+    assert np.array_equal(fm, orc.floating_grid(far))
 
 
 def test_atom_on_grid_point(ctx, tmp_path):

@@ -106,9 +106,10 @@ def test_pair_tables_match_golden_log(tmp_path):
 @pytest.mark.parametrize("name", fx.VARIANTS)
 def test_gpf_variants_match_reference_binary(tmp_path, name):
     """Constant dielectric, disorder_h, covalent map + qasp, odd grid with
-    AD4.1 defaults: the reference binary's maps."""
+    AD4.1 defaults, floating grid: the reference binary's maps and .fld."""
     run, gpf, names, args = fx.variant(str(tmp_path), name)
     got = run_oracle(run, gpf, names, "colagen2", v4="--v4" in args)
     gold = fx.goldens("variants", f"{name}.npz")
     for n in names:
         assert np.array_equal(got[n], gold[n]), n
+    assert fx.fld_lines(os.path.join(run, "colagen2.maps.fld")) == fx.variant_fld(name)
