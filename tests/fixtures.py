@@ -119,3 +119,14 @@ def pocket(tmp, name, seed, n_atoms, npts, ligand_types="A C HD N OA SA"):
     radius = (3.0 * n_atoms / (4.0 * np.pi * POCKET_DENSITY)) ** (1.0 / 3.0)
     fg.synth_write(tmp, name, seed, n_atoms, "sphere", radius, npts, ligand_types)
     return load_arrays(tmp, name + ".gpf")
+
+
+def value_digest(path):
+    """sha256 of a map file's value lines (after the 6 header lines), line
+    ends normalised: byte-level parity of the writer."""
+    text = open(path, newline="").read().replace("\r\n", "\n")
+    return hashlib.sha256("\n".join(text.split("\n")[6:]).encode()).hexdigest()
+
+
+def text_digests(name, which):
+    return json.load(open(os.path.join(GOLDEN, name, which.replace(".npz", ".text_sha256.json"))))

@@ -50,14 +50,24 @@ def read_map(path):
     return header, np.rint(vals * 1000).astype(np.int32)
 
 
+def value_digest(path):
+    text = open(path, newline="").read().replace("\r\n", "\n")
+    return hashlib.sha256("\n".join(text.split("\n")[6:]).encode()).hexdigest()
+
+
 def save_maps(path, files, names):
     arrays = {}
     headers = {}
+    digests = {}
     for name, f in zip(names, files):
         h, v = read_map(f)
         arrays[name] = v
         headers[name] = h
+        # the value lines byte for byte (the header names paths; line ends
+        # normalised, the shipped win32 goldens use CRLF)
+        digests[name] = value_digest(f)
     np.savez_compressed(path, **arrays)
+    json.dump(digests, open(path.replace(".npz", ".text_sha256.json"), "w"), indent=1)
     return headers
 
 

@@ -27,19 +27,25 @@ def fastgrid4(run, gpf, *args):
     return p
 
 
-def compare_files(run, stem, gold, exact_e=False):
+def compare_files(run, stem, gold, exact_e=False, digests=None):
+    """Values against the goldens; with digests, every map whose values are
+    identical must also be byte-identical text."""
     for n, want in gold.items():
-        _, got = fx.read_map(os.path.join(run, f"{stem}.{n}.map"))
+        path = os.path.join(run, f"{stem}.{n}.map")
+        _, got = fx.read_map(path)
         if n == "e" and not exact_e:
             assert np.abs(got - want).max() <= 1, (n, np.abs(got - want).max())
         else:
             assert np.array_equal(got, want), (n, int((got != want).sum()))
+        if digests is not None and np.array_equal(got, want):
+            assert fx.value_digest(path) == digests[n], n
 
 
 def test_cli_colagen2_goldens(tmp_path):
     run, gpf = fx.colagen2(str(tmp_path))
     fastgrid4(run, gpf, "--v4")
-    compare_files(run, "colagen2", fx.goldens("colagen2", "goldens_autogrid400.npz"))
+    compare_files(run, "colagen2", fx.goldens("colagen2", "goldens_autogrid400.npz"),
+                  digests=fx.text_digests("colagen2", "goldens_autogrid400.npz"))
     assert open(os.path.join(run, "colagen2.maps.xyz")).read() == \
         open(os.path.join(fx.GOLDEN, "colagen2", "colagen2.maps.xyz")).read()
 
@@ -66,7 +72,8 @@ def test_cli_pgp_fastgrid_and_autogrid_order(tmp_path):
 def test_cli_gpf_variant_goldens(tmp_path, name):
     run, gpf, names, args = fx.variant(str(tmp_path), name)
     fastgrid4(run, gpf, *args)
-    compare_files(run, "colagen2", fx.goldens("variants", f"{name}.npz"))
+    compare_files(run, "colagen2", fx.goldens("variants", f"{name}.npz"),
+                  digests=fx.text_digests("variants", f"{name}.npz"))
     assert fx.fld_lines(os.path.join(run, "colagen2.maps.fld")) == fx.variant_fld(name)
 
 
@@ -74,7 +81,8 @@ def test_cli_gpf_variant_goldens(tmp_path, name):
 def test_cli_synthetic_goldens(tmp_path, name):
     run, gpf, names = fx.synth(str(tmp_path), name)
     fastgrid4(run, gpf)
-    compare_files(run, name, fx.goldens("synth", f"{name}_goldens_fastgrid.npz"))
+    compare_files(run, name, fx.goldens("synth", f"{name}_goldens_fastgrid.npz"),
+                  digests=fx.text_digests("synth", f"{name}_goldens_fastgrid.npz"))
 
 
 def plain_errors(out, om, od, oe):

@@ -24,8 +24,10 @@ def test_colagen2_reproduces_shipped_goldens(tmp_path):
     run, gpf = fx.colagen2(str(tmp_path))
     gold = fx.goldens("colagen2", "goldens_autogrid400.npz")
     got = run_oracle(run, gpf, list(gold), "colagen2", v4=True)
+    dig = fx.text_digests("colagen2", "goldens_autogrid400.npz")
     for n in gold:
         assert np.array_equal(got[n], gold[n]), n
+        assert fx.value_digest(os.path.join(run, f"colagen2.{n}.map")) == dig[n], n
     assert open(os.path.join(run, "colagen2.maps.xyz")).read() == \
         open(os.path.join(fx.GOLDEN, "colagen2", "colagen2.maps.xyz")).read()
 
@@ -65,8 +67,10 @@ def test_synthetic_receptors_match_reference_binary(tmp_path, name):
     run, gpf, names = fx.synth(str(tmp_path), name)
     gold = fx.goldens("synth", f"{name}_goldens_fastgrid.npz")
     got = run_oracle(run, gpf, names, name)
+    dig = fx.text_digests("synth", f"{name}_goldens_fastgrid.npz")
     for n in names:
         assert np.array_equal(got[n], gold[n]), n
+        assert fx.value_digest(os.path.join(tmp_path, f"{name}.{n}.map")) == dig[n], n
 
 
 def test_dielectric_table_matches_golden_log():
@@ -110,6 +114,8 @@ def test_gpf_variants_match_reference_binary(tmp_path, name):
     run, gpf, names, args = fx.variant(str(tmp_path), name)
     got = run_oracle(run, gpf, names, "colagen2", v4="--v4" in args)
     gold = fx.goldens("variants", f"{name}.npz")
+    dig = fx.text_digests("variants", f"{name}.npz")
     for n in names:
         assert np.array_equal(got[n], gold[n]), n
+        assert fx.value_digest(os.path.join(run, f"colagen2.{n}.map")) == dig[n], n
     assert fx.fld_lines(os.path.join(run, "colagen2.maps.fld")) == fx.variant_fld(name)
