@@ -158,12 +158,13 @@ def copy_calibration(nbytes=1 << 30):
 
 def traffic_entry(shape):
     """HBM bytes per electrostatic launch on L from the committed counter
-    summary, when it was measured with the same kernel (points per lane)."""
+    summary, when it was measured with the same kernel shape
+    (pz, tile, bx, px)."""
     try:
         t = json.load(open(TRAFFIC_FILE))
     except (OSError, ValueError):
         return None
-    if t.get("pz") != shape[0]:
+    if list(t.get("shape", [])) != list(shape):
         return None
     return t
 
@@ -288,7 +289,7 @@ def run_workload(fg, ctx, w, steps, warmup, tune=True):
     log(f"fp64 form: elec {t64.elec_ms:.1f} ms")
     report = fg.autotune(ctx, [job]) if tune else ""
     shape = job.shape
-    log("tuned shape (pz, tile, bx):", shape)
+    log("tuned shape (pz, tile, bx, px):", shape)
     names = prep.map_types()[:M] + ["e", "d"]
     res = {"N": N, "P": P, "M": M, "elec_pairs": elec_pairs, "cutoff_pairs": pcut, "fp64_elec_ms": t64.elec_ms,
            "shape": shape, "tune_report": report}
@@ -349,7 +350,8 @@ def main():
             "precision": "compensated (FP32 far-field pairs, two-float cross-tile sums, FP64 final sum; "
                          "pairs within 8 A and the affinity/desolvation maps in FP64)",
             "elec_pairs": L["elec_pairs"], "cutoff_pairs": L["cutoff_pairs"],
-            "elec_kernel_shape": {"pz": L["shape"][0], "tile": L["shape"][1], "bx": L["shape"][2]},
+            "elec_kernel_shape": {"pz": L["shape"][0], "tile": L["shape"][1], "bx": L["shape"][2],
+                                  "px": L["shape"][3]},
             "autotune": L["tune_report"].strip().splitlines(),
         },
         "seconds_per_map_set": round(comp["t"], 5),

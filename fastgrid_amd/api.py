@@ -62,7 +62,7 @@ class FgTiming(C.Structure):
 
 
 class FgShape(C.Structure):
-    _fields_ = [("pz", C.c_int32), ("tile", C.c_int32), ("bx", C.c_int32)]
+    _fields_ = [("pz", C.c_int32), ("tile", C.c_int32), ("bx", C.c_int32), ("px", C.c_int32)]
 
 
 class FgLoadOpts(C.Structure):
@@ -434,15 +434,18 @@ class Job:
         return int(v.value)
 
     @property
-    def shape(self) -> tuple[int, int, int]:
-        """(points per lane along z, atoms per tile, brick lanes along x)."""
+    def shape(self) -> tuple[int, int, int, int]:
+        """(points per lane along z, atoms per tile, brick lanes along x,
+        points per lane along x)."""
         sh = FgShape()
         _check(LIB.fg_job_get_shape(self._h, C.byref(sh)))
-        return sh.pz, sh.tile, sh.bx
+        return sh.pz, sh.tile, sh.bx, sh.px
 
     @shape.setter
-    def shape(self, v: tuple[int, int, int]) -> None:
-        sh = FgShape(*[int(x) for x in v])
+    def shape(self, v: tuple[int, ...]) -> None:
+        """(pz, tile, bx) or (pz, tile, bx, px); px defaults to 1."""
+        v = tuple(int(x) for x in v)
+        sh = FgShape(*(v if len(v) == 4 else v + (1,)))
         _check(LIB.fg_job_set_shape(self._h, C.byref(sh)))
 
 

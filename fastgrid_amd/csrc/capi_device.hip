@@ -84,9 +84,9 @@ struct fg_job {
     float *dSubBox = nullptr, *dEps32 = nullptr;
     int nAtoms32 = 0, tableLen = fgk::kTableLen;
     bool clamp = true;
-    // k_elec shape: points per lane along z, atoms per tile, lanes along x
-    // (defaults; the autotuner may change them)
-    int pz = 4, tile = 64, bxShift = 4;
+    // k_elec shape: points per lane along x and z, atoms per tile, lanes
+    // along x (defaults; the autotuner may change them)
+    int px = 1, pz = 4, tile = 64, bxShift = 4;
     float farCut2 = 0, r2Lo = 0, r2Hi = 0, spacing100 = 0;
     // plain form (FP32 throughout)
     float4 *dPlainQ = nullptr;          // (x, y, z, w) per atom, index order
@@ -268,7 +268,7 @@ int launchCutoff(fg_job *j, bool affinity, int doElecNear)
     return FG_OK;
 }
 
-fgk::ElecArgs elecArgs(const fg_job *j, int pz, int tile, int bxShift)
+fgk::ElecArgs elecArgs(const fg_job *j, int px, int pz, int tile, int bxShift)
 {
     fgk::ElecArgs a{};
     a.g = j->g;
@@ -280,8 +280,8 @@ fgk::ElecArgs elecArgs(const fg_job *j, int pz, int tile, int bxShift)
         a.nbx = int((int64_t(j->g.nx) * j->g.ny + fgk::kElecThreads - 1) / fgk::kElecThreads);
         a.nby = 1;
     } else {
-        const int bxl = 1 << bxShift, byl = fgk::kElecThreads >> bxShift;
-        a.nbx = (j->g.nx + bxl - 1) / bxl;
+        const int bw = (1 << bxShift) * px, byl = fgk::kElecThreads >> bxShift;
+        a.nbx = (j->g.nx + bw - 1) / bw;
         a.nby = (j->g.ny + byl - 1) / byl;
     }
     a.nbz = (j->g.nz + pz - 1) / pz;
@@ -300,16 +300,30 @@ fgk::ElecArgs elecArgs(const fg_job *j, int pz, int tile, int bxShift)
     return a;
 }
 
-template <int PZ>
-void launchElecPZ(fg_job *j, const fgk::ElecArgs &a, bool comp, size_t lds)
+template <int PX, int PZ>
+void launchElecV(fg_job *j, const fgk::ElecArgs &a, bool comp, size_t lds)
 {
     dim3 grid(unsigned(a.nbx * a.nby * a.nbz));
     dim3 block(fgk::kElecThreads);
     hipStream_t s = j->ctx->stream;
-    if (comp && j->clamp) hipLaunchKernelGGL((fgk::k_elec<PZ, true, true>), grid, block, lds, s, a);
-    else if (comp) hipLaunchKernelGGL((fgk::k_elec<PZ, true, false>), grid, block, lds, s, a);
-    else if (j->clamp) hipLaunchKernelGGL((fgk::k_elec<PZ, false, true>), grid, block, lds, s, a);
-    else hipLaunchKernelGGL((fgk::k_elec<PZ, false, false>), grid, block, lds, s, a);
+    if (comp && j->clamp) hipLaunchKernelGGL((fgk::k_elec<PX, PZ, true, true>), grid, block, lds, s, a);
+    else if (comp) hipLaunchKernelGGL((fgk::k_elec<PX, PZ, true, false>), grid, block, lds, s, a);
+    else if (j->clamp) hipLaunchKernelGGL((fgk::k_elec<PX, PZ, false, true>), grid, block, lds, s, a);
+    else hipLaunchKernelGGL((fgk::k_elec<PX, PZ, false, false>), grid, block, lds, s, a);
+}
+
+// k_elec of shape (px, pz, tile, bxShift); px 2 needs a 2D brick (bxShift > 0).
+void launchElecShape(fg_job *j, int px, int pz, int tile, int bxShift, bool comp)
+{
+    const fgk::ElecArgs a = elecArgs(j, px, pz, tile, bxShift);
+    const size_t lds = size_t(j->tableLen) * sizeof(float);
+    if (px == 2) {
+        if (pz == 8) launchElecV<2, 8>(j, a, comp, lds);
+        else launchElecV<2, 4>(j, a, comp, lds);
+    } else {
+        if (pz == 8) launchElecV<1, 8>(j, a, comp, lds);
+        else launchElecV<1, 4>(j, a, comp, lds);
+    }
 }
 
 int launchElec(fg_job *j, int precision)
@@ -322,11 +336,7 @@ int launchElec(fg_job *j, int precision)
         FG_HIP(hipGetLastError());
         return FG_OK;
     }
-    const fgk::ElecArgs a = elecArgs(j, j->pz, j->tile, j->bxShift);
-    size_t lds = size_t(j->tableLen) * sizeof(float);
-    const bool comp = precision == FG_PREC_COMPENSATED;
-    if (j->pz == 8) launchElecPZ<8>(j, a, comp, lds);
-    else launchElecPZ<4>(j, a, comp, lds);
+    launchElecShape(j, j->px, j->pz, j->tile, j->bxShift, precision == FG_PREC_COMPENSATED);
     FG_HIP(hipGetLastError());
     return FG_OK;
 }
@@ -719,21 +729,11 @@ void destroyJob(fg_job *j)
 // ------------------------------------------------------------------------
 // Autotuner of the electrostatic kernel shape
 // ------------------------------------------------------------------------
-template <int PZ>
-void launchElecShape(fg_job *j, int tile, int bxShift, bool comp)
-{
-    const fgk::ElecArgs a = elecArgs(j, PZ, tile, bxShift);
-    launchElecPZ<PZ>(j, a, comp, size_t(j->tableLen) * sizeof(float));
-}
-
-// Device time of one electrostatic launch of shape (pz, tile, bxShift), ms.
-int timeShape(fg_job *j, int pz, int tile, int bxShift, bool comp, float &ms)
+// Device time of one electrostatic launch of shape (px, pz, tile, bxShift), ms.
+int timeShape(fg_job *j, int px, int pz, int tile, int bxShift, bool comp, float &ms)
 {
     fg_ctx *ctx = j->ctx;
-    auto go = [&]() {
-        if (pz == 8) launchElecShape<8>(j, tile, bxShift, comp);
-        else launchElecShape<4>(j, tile, bxShift, comp);
-    };
+    auto go = [&]() { launchElecShape(j, px, pz, tile, bxShift, comp); };
     go();  // warm-up
     FG_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
     go();
@@ -765,12 +765,15 @@ int ilog2(int64_t v)
 
 int tuneJobs(fg_ctx *ctx, fg_job *const *jobs, int n, bool comp)
 {
-    static const int kPz[2] = {4, 8}, kTiles[3] = {32, 64, 128}, kShifts[4] = {3, 4, 5, 0};
+    // kernel variants (points per lane along x, along z); a batch launches one
+    static const int kVar[4][2] = {{1, 4}, {1, 8}, {2, 4}, {2, 8}};
+    static const int kTiles[3] = {32, 64, 128}, kShifts[4] = {3, 4, 5, 0};
+    constexpr float kNone = 1e30f;
     // size classes: (atom octave, point octave, best-padding brick width)
     struct Cls {
         int key[3];
         std::vector<int> members;
-        float t[2][3][4];
+        float t[4][3][4];
     };
     std::vector<Cls> cls;
     for (int i = 0; i < n; ++i) {
@@ -788,42 +791,52 @@ int tuneJobs(fg_ctx *ctx, fg_job *const *jobs, int n, bool comp)
         }
         it->members.push_back(i);
     }
-    double total[2] = {0, 0};
+    double total[4] = {0, 0, 0, 0};
     std::string report;
     for (Cls &c : cls) {
         std::sort(c.members.begin(), c.members.end(), [&](int x, int y) {
             return double(jobs[x]->N) * double(jobs[x]->P) < double(jobs[y]->N) * double(jobs[y]->P);
         });
         fg_job *rep = jobs[c.members[c.members.size() / 2]];
-        for (int a = 0; a < 2; ++a)
+        for (int v = 0; v < 4; ++v)
             for (int b = 0; b < 3; ++b)
-                for (int d = 0; d < 4; ++d)
-                    if (int rc = timeShape(rep, kPz[a], kTiles[b], kShifts[d], comp, c.t[a][b][d])) return rc;
+                for (int d = 0; d < 4; ++d) {
+                    c.t[v][b][d] = kNone;
+                    // two points along x need a 2D brick
+                    if (kVar[v][0] == 2 && kShifts[d] == 0) continue;
+                    if (int rc = timeShape(rep, kVar[v][0], kVar[v][1], kTiles[b], kShifts[d], comp, c.t[v][b][d]))
+                        return rc;
+                }
         const double repWork = std::max(1.0, double(rep->N) * double(rep->P));
         double work = 0;
         for (int i : c.members) work += double(jobs[i]->N) * double(jobs[i]->P);
-        for (int a = 0; a < 2; ++a) {
-            float best = 1e30f;
+        for (int v = 0; v < 4; ++v) {
+            float best = kNone;
             for (int b = 0; b < 3; ++b)
-                for (int d = 0; d < 4; ++d) best = std::min(best, c.t[a][b][d]);
-            total[a] += best * work / repWork;
+                for (int d = 0; d < 4; ++d) best = std::min(best, c.t[v][b][d]);
+            total[v] += best * work / repWork;
         }
     }
-    const int pzi = total[1] < total[0] ? 1 : 0;
-    char line[256];
-    std::snprintf(line, sizeof line, "pz %d for all jobs (modelled totals: pz4 %.3f ms, pz8 %.3f ms)\n", kPz[pzi],
-                  total[0], total[1]);
+    int vi = 0;
+    for (int v = 1; v < 4; ++v)
+        if (total[v] < total[vi]) vi = v;
+    char line[320];
+    std::snprintf(line, sizeof line,
+                  "px %d pz %d for all jobs (modelled totals: px1 pz4 %.3f ms, px1 pz8 %.3f ms, px2 pz4 %.3f ms, "
+                  "px2 pz8 %.3f ms)\n",
+                  kVar[vi][0], kVar[vi][1], total[0], total[1], total[2], total[3]);
     report += line;
     for (Cls &c : cls) {
         int bb = 0, bd = 0;
         for (int b = 0; b < 3; ++b)
             for (int d = 0; d < 4; ++d)
-                if (c.t[pzi][b][d] < c.t[pzi][bb][bd]) {
+                if (c.t[vi][b][d] < c.t[vi][bb][bd]) {
                     bb = b;
                     bd = d;
                 }
         for (int i : c.members) {
-            jobs[i]->pz = kPz[pzi];
+            jobs[i]->px = kVar[vi][0];
+            jobs[i]->pz = kVar[vi][1];
             jobs[i]->tile = kTiles[bb];
             jobs[i]->bxShift = kShifts[bd];
         }
@@ -832,8 +845,8 @@ int tuneJobs(fg_ctx *ctx, fg_job *const *jobs, int n, bool comp)
                       "class atoms~2^%d points~2^%d bx%d: %zu jobs, median %d atoms %dx%dx%d -> tile %d bx %d "
                       "(%.4f ms; default shape %.4f ms)\n",
                       c.key[0], c.key[1], 1 << c.key[2], c.members.size(), rep->N, rep->g.nx, rep->g.ny, rep->g.nz,
-                      kTiles[bb], kShifts[bd] == 0 ? fgk::kElecThreads : 1 << kShifts[bd], c.t[pzi][bb][bd],
-                      c.t[pzi][1][1]);
+                      kTiles[bb], kShifts[bd] == 0 ? fgk::kElecThreads : 1 << kShifts[bd], c.t[vi][bb][bd],
+                      c.t[0][1][1]);
         report += line;
     }
     ctx->tuneReport = report;
@@ -849,7 +862,7 @@ struct fg_batch {
     fg_ctx *ctx = nullptr;
     std::vector<fg_job *> jobs;  // largest first
     bool aff = false, near = false, elec = false, comp = true, fmap = false;
-    int MM = 4, KH = 0, pz = 4;
+    int MM = 4, KH = 0, px = 1, pz = 4;
     bool pointAng = false, clamp = false, typed = true;
     int sharedTableLen = 0, maxTableLen = 0;
     int cutItems = 0, elecItems = 0, cutWGs = 0, elecWGs = 0;
@@ -927,10 +940,10 @@ int launchCutoffBatchKH(fg_batch *b, bool launch)
     return launchCutoffBatchPA<MM, 4>(b, launch);
 }
 
-template <int PZ, bool kComp, bool kClamp>
+template <int PX, int PZ, bool kComp, bool kClamp>
 int launchElecBatch(fg_batch *b, bool launch)
 {
-    auto k = fgk::k_elec_batch<PZ, kComp, kClamp>;
+    auto k = fgk::k_elec_batch<PX, PZ, kComp, kClamp>;
     const size_t lds = size_t(b->sharedTableLen > 0 ? b->sharedTableLen : b->maxTableLen) * sizeof(float);
     if (!b->elecWGs)
         if (int rc = persistentWGs(b, k, fgk::kElecThreads, lds, b->elecItems, b->elecWGs)) return rc;
@@ -940,12 +953,13 @@ int launchElecBatch(fg_batch *b, bool launch)
     return FG_OK;
 }
 
-template <int PZ>
-int launchElecBatchPZ(fg_batch *b, bool launch)
+template <int PX, int PZ>
+int launchElecBatchV(fg_batch *b, bool launch)
 {
     if (b->comp)
-        return b->clamp ? launchElecBatch<PZ, true, true>(b, launch) : launchElecBatch<PZ, true, false>(b, launch);
-    return b->clamp ? launchElecBatch<PZ, false, true>(b, launch) : launchElecBatch<PZ, false, false>(b, launch);
+        return b->clamp ? launchElecBatch<PX, PZ, true, true>(b, launch)
+                        : launchElecBatch<PX, PZ, true, false>(b, launch);
+    return b->clamp ? launchElecBatch<PX, PZ, false, true>(b, launch) : launchElecBatch<PX, PZ, false, false>(b, launch);
 }
 
 // The batch's launch sequence on the context's stream; `mid` (if any) is
@@ -979,7 +993,8 @@ int enqueueBatch(fg_batch *b, hipEvent_t mid, bool launch = true)
     }
     if (mid && launch) FG_HIP(hipEventRecord(mid, s));
     if (b->elec) {
-        int rc = b->pz == 8 ? launchElecBatchPZ<8>(b, launch) : launchElecBatchPZ<4>(b, launch);
+        int rc = b->px == 2 ? (b->pz == 8 ? launchElecBatchV<2, 8>(b, launch) : launchElecBatchV<2, 4>(b, launch))
+                            : (b->pz == 8 ? launchElecBatchV<1, 8>(b, launch) : launchElecBatchV<1, 4>(b, launch));
         if (rc) return rc;
     }
     FG_HIP(hipGetLastError());
@@ -997,11 +1012,12 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
     b->near = b->elec && comp;
     b->aff = (what & FG_RUN_AFFINITY) != 0;
     for (fg_job *j : b->jobs) b->fmap = b->fmap || ((what & FG_RUN_FLOATING) && j->floating);
-    // one pz for the batch: the one carrying the most work
-    double work[2] = {0, 0};
+    // one kernel variant (points per lane along x and z) for the batch: the
+    // one carrying the most work
+    double work[4] = {0, 0, 0, 0};
     int maxM = 0;
     for (fg_job *j : b->jobs) {
-        work[j->pz == 8 ? 1 : 0] += double(j->N) * double(j->P);
+        work[(j->px == 2 ? 2 : 0) + (j->pz == 8 ? 1 : 0)] += double(j->N) * double(j->P);
         maxM = std::max(maxM, j->M);
         b->KH = std::max(b->KH, j->KH);
         b->pointAng = b->pointAng || j->pointAng;
@@ -1009,7 +1025,13 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
         b->maxTableLen = std::max(b->maxTableLen, j->tableLen);
         b->typed = b->typed && j->typed;
     }
-    b->pz = work[1] > work[0] ? 8 : 4;
+    {
+        int v = 0;
+        for (int k = 1; k < 4; ++k)
+            if (work[k] > work[v]) v = k;
+        b->px = v >= 2 ? 2 : 1;
+        b->pz = v & 1 ? 8 : 4;
+    }
     b->MM = maxM <= 4 ? 4 : maxM <= 6 ? 6 : maxM <= 8 ? 8 : 16;
     if (b->aff && maxM == 0) b->aff = false;
     // one LDS table for all jobs when their dielectric tables agree
@@ -1031,7 +1053,10 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
         la.push_back(listArgs(j));
         ca.push_back(cutoffArgs(j, b->near ? 1 : 0));
         if (!(b->typed && b->aff)) ca.back().brickList = j->dList;
-        fgk::ElecArgs e = elecArgs(j, b->pz, j->tile, j->bxShift);
+        // two points per lane along x need a 2D brick (the shape does not
+        // change the results)
+        const int shift = b->px == 2 && j->bxShift == 0 ? 4 : j->bxShift;
+        fgk::ElecArgs e = elecArgs(j, b->px, b->pz, j->tile, shift);
         if (b->sharedTableLen > 0) e.clampMax = b->sharedTableLen - 1;
         ea.push_back(e);
         const int64_t c = int64_t(cutBase.back()) + j->nb, el = int64_t(elecBase.back()) + int64_t(e.nbx) * e.nby * e.nbz;
@@ -1207,11 +1232,10 @@ int fg_job_create(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
     for (int k = 0; k < 3; ++k)
         gridMaxAbs = std::max({gridMaxAbs, std::fabs(double(0 - dv[k]) * g.spacing), std::fabs(double(n[k] - 1 - dv[k]) * g.spacing)});
     // points per lane along z: 8 halves the per-atom overhead but needs enough
-    // z-bricks to fill the GPU; FG_ELEC_PZ overrides (tuning)
+    // z-bricks to fill the GPU
     {
         const long bricks8 = long((g.nx + 15) / 16) * ((g.ny + 15) / 16) * ((g.nz + 7) / 8);
         j->pz = bricks8 >= 1024 ? 8 : 4;
-        if (const char *e = std::getenv("FG_ELEC_PZ")) j->pz = std::atoi(e) == 8 ? 8 : 4;
     }
     int rc = buildCutoff(j, *d);
     if (!rc) rc = buildElecTiles(j, *d, gridMaxAbs);
@@ -1351,6 +1375,7 @@ int fg_job_cutoff_pairs(fg_ctx *ctx, fg_job *j, int64_t *pairs)
 int fg_job_get_shape(const fg_job *job, fg_shape *out)
 {
     if (!job || !out) return fgc::fail(FG_E_ARG, "fg_job_get_shape: NULL argument");
+    out->px = job->px;
     out->pz = job->pz;
     out->tile = job->tile;
     out->bx = job->bxShift == 0 ? fgk::kElecThreads : 1 << job->bxShift;
@@ -1361,8 +1386,12 @@ int fg_job_set_shape(fg_job *job, const fg_shape *sh)
 {
     if (!job || !sh) return fgc::fail(FG_E_ARG, "fg_job_set_shape: NULL argument");
     const int shift = sh->bx == 8 ? 3 : sh->bx == 16 ? 4 : sh->bx == 32 ? 5 : sh->bx == 256 ? 0 : -1;
-    if ((sh->pz != 4 && sh->pz != 8) || (sh->tile != 32 && sh->tile != 64 && sh->tile != 128) || shift < 0)
-        return fgc::fail(FG_E_ARG, "fg_job_set_shape: pz must be 4 or 8, tile 32/64/128, bx 8/16/32/256");
+    if ((sh->px != 1 && sh->px != 2) || (sh->pz != 4 && sh->pz != 8) ||
+        (sh->tile != 32 && sh->tile != 64 && sh->tile != 128) || shift < 0 || (sh->px == 2 && shift == 0))
+        return fgc::fail(FG_E_ARG,
+                         "fg_job_set_shape: px must be 1 or 2, pz 4 or 8, tile 32/64/128, bx 8/16/32/256 (256 only "
+                         "with px 1)");
+    job->px = sh->px;
     job->pz = sh->pz;
     job->tile = sh->tile;
     job->bxShift = shift;

@@ -691,6 +691,7 @@ __global__ __launch_bounds__(kListThreads) void k_type_sort(ListArgs a)
 
 constexpr float kFloorBias = 8388607.5f;       // 2^23 - 1/2
 constexpr unsigned kFloorBits = 0x4B000000u;   // bit pattern of 2^23
+constexpr int kFold = 64;  // atoms per FP32 partial sum of the compensated form
 
 // Loads the dielectric table into LDS (all lanes of the workgroup).
 __device__ __forceinline__ void loadEps(float *eps, const float *src, int n)
@@ -698,15 +699,19 @@ __device__ __forceinline__ void loadEps(float *eps, const float *src, int n)
     for (int i = threadIdx.x; i < n; i += kElecThreads) eps[i] = src[i];
 }
 
-// One brick of the electrostatic map: 256 lanes in x, y, PZ points per lane
-// along z.  `eps` is the table in LDS.
-template <int PZ, bool kComp, bool kClamp>
+// One brick of the electrostatic map: 256 lanes in x, y; each lane owns PX
+// consecutive points along x times PZ points along z.  `eps` is the table in
+// LDS.  The FP32 arithmetic of a (point, atom) pair does not depend on the
+// shape, and the compensated form folds its FP32 partial into the two-float
+// sum after every kFold atoms, so every shape gives the same bits.
+template <int PX, int PZ, bool kComp, bool kClamp>
 __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float *eps)
 {
+    constexpr int NP = PX * PZ;
     const int tid = threadIdx.x;
     const int sh = a.bxShift;
     int ix, iy, ix0, iy0, bx1, by1, iz0;
-    if (sh == 0) {
+    if (PX == 1 && sh == 0) {
         // 256 consecutive columns of the x-y plane (x fastest): padding only
         // at the end of each z-slab
         const int c0 = (b % a.nbx) * kElecThreads, c = c0 + tid;
@@ -719,19 +724,22 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
         ix0 = iy0 == by1 ? c0 % a.g.nx : 0;
         bx1 = iy0 == by1 ? cl % a.g.nx : a.g.nx - 1;
     } else {
+        // bxl lanes along x, each owning PX consecutive points
         const int bxl = 1 << sh, byl = kElecThreads >> sh;
         const int bx = b % a.nbx, by = (b / a.nbx) % a.nby, bz = b / (a.nbx * a.nby);
-        ix0 = bx * bxl;
+        ix0 = bx * bxl * PX;
         iy0 = by * byl;
         iz0 = bz * PZ;
-        ix = ix0 + (tid & (bxl - 1));
+        ix = ix0 + PX * (tid & (bxl - 1));
         iy = iy0 + (tid >> sh);
-        bx1 = min(ix0 + bxl - 1, a.g.nx - 1);
+        bx1 = min(ix0 + bxl * PX - 1, a.g.nx - 1);
         by1 = min(iy0 + byl - 1, a.g.ny - 1);
     }
     const float s100 = a.spacing100;
-    const float gx = float(ix - a.g.dx2) * s100, gy = float(iy - a.g.dy2) * s100;
-    float gz[PZ];
+    const float gy = float(iy - a.g.dy2) * s100;
+    float gx[PX], gz[PZ];
+#pragma unroll
+    for (int i = 0; i < PX; ++i) gx[i] = float(ix + i - a.g.dx2) * s100;
 #pragma unroll
     for (int k = 0; k < PZ; ++k) gz[k] = float(iz0 + k - a.g.dz2) * s100;
     // brick box (scaled), clamped to the grid
@@ -745,9 +753,9 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
     const unsigned biasBytes = unsigned(a.floorBitsNeg4);
     const char *epsBytes = reinterpret_cast<const char *>(eps);
 
-    float hi[PZ], lo[PZ], part[PZ];
+    float hi[NP], lo[NP], part[NP];
 #pragma unroll
-    for (int k = 0; k < PZ; ++k) hi[k] = lo[k] = part[k] = 0.f;
+    for (int q = 0; q < NP; ++q) hi[q] = lo[q] = part[q] = 0.f;
 
     for (int j0 = 0; j0 < a.nAtoms; j0 += a.tile) {
         const int j1 = min(j0 + a.tile, a.nAtoms);
@@ -766,108 +774,130 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
             }
             far = uniform(far);
         }
-        if (far) {
+        for (int c0 = j0; c0 < j1; c0 += kFold) {
+            const int c1 = min(c0 + kFold, j1);
+            if (far) {
 #pragma unroll 2
-            for (int j = j0; j < j1; ++j) {
-                const float4 at = a.tiles[j];
-                const float dx = at.x - gx, dy = at.y - gy;
-                const float dxy2 = __builtin_fmaf(dx, dx, dy * dy);
-                float ri[PZ], ev[PZ];
-                unsigned bin[PZ];
+                for (int j = c0; j < c1; ++j) {
+                    const float4 at = a.tiles[j];
+                    const float dy = at.y - gy;
+                    const float dy2 = dy * dy;
+                    float dxy2[PX], dz[PZ];
 #pragma unroll
-                for (int k = 0; k < PZ; ++k) {
-                    const float dz = at.z - gz[k];
-                    const float r2 = __builtin_fmaf(dz, dz, dxy2);
-                    ri[k] = __builtin_amdgcn_rsqf(r2);
-                    if (kComp) {
-                        // floor(100 r) in one operation: r2 * ri + (2^23 - 1/2)
-                        // lands in [2^23, 2^24), where the FP32 spacing is 1,
-                        // so its bit pattern is 2^23's plus round(100 r - 1/2)
-                        // (far pairs have 100 r >= 900)
-                        bin[k] = __float_as_uint(__builtin_fmaf(r2, ri[k], kFloorBias));
-                        if (kClamp) bin[k] = min(bin[k], clampBits);
-                    } else {
-                        bin[k] = unsigned(r2 * ri[k]);
-                        if (kClamp) bin[k] = min(bin[k], clampMax);
+                    for (int i = 0; i < PX; ++i) {
+                        const float dx = at.x - gx[i];
+                        dxy2[i] = __builtin_fmaf(dx, dx, dy2);
+                    }
+#pragma unroll
+                    for (int k = 0; k < PZ; ++k) dz[k] = at.z - gz[k];
+                    float ri[NP], ev[NP];
+                    unsigned bin[NP];
+#pragma unroll
+                    for (int q = 0; q < NP; ++q) {
+                        const float r2 = __builtin_fmaf(dz[q % PZ], dz[q % PZ], dxy2[q / PZ]);
+                        ri[q] = __builtin_amdgcn_rsqf(r2);
+                        if (kComp) {
+                            // floor(100 r) in one operation: r2 * ri + (2^23 - 1/2)
+                            // lands in [2^23, 2^24), where the FP32 spacing is 1,
+                            // so its bit pattern is 2^23's plus round(100 r - 1/2)
+                            // (far pairs have 100 r >= 900)
+                            bin[q] = __float_as_uint(__builtin_fmaf(r2, ri[q], kFloorBias));
+                            if (kClamp) bin[q] = min(bin[q], clampBits);
+                        } else {
+                            bin[q] = unsigned(r2 * ri[q]);
+                            if (kClamp) bin[q] = min(bin[q], clampMax);
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < NP; ++q) {
+                        // eps[bits - 2^23's bits] as one shift-add with a uniform byte offset
+                        ev[q] = kComp ? *reinterpret_cast<const float *>(epsBytes + ((bin[q] << 2) + biasBytes))
+                                      : eps[bin[q]];
+                    }
+#pragma unroll
+                    for (int q = 0; q < NP; ++q) {
+                        // min(1/r, 2) in scaled units; the far-field pairs of the
+                        // compensated form are all beyond 8 A
+                        const float rc = kComp ? ri[q] : fminf(ri[q], 0.02f);
+                        part[q] = __builtin_fmaf(at.w * rc, ev[q], part[q]);
                     }
                 }
+            } else {
+                for (int j = c0; j < c1; ++j) {
+                    const float4 at = a.tiles[j];
+                    const double *T64 = a.tiles64 + int64_t(j) * 3;
+                    const float dy = at.y - gy;
+                    const float dy2 = dy * dy;
+                    float dxy2[PX], dz[PZ];
 #pragma unroll
-                for (int k = 0; k < PZ; ++k) {
-                    // eps[bits - 2^23's bits] as one shift-add with a uniform byte offset
-                    ev[k] = kComp ? *reinterpret_cast<const float *>(epsBytes + ((bin[k] << 2) + biasBytes))
-                                  : eps[bin[k]];
-                }
-#pragma unroll
-                for (int k = 0; k < PZ; ++k) {
-                    // min(1/r, 2) in scaled units; the far-field pairs of the
-                    // compensated form are all beyond 8 A
-                    const float rc = kComp ? ri[k] : fminf(ri[k], 0.02f);
-                    part[k] = __builtin_fmaf(at.w * rc, ev[k], part[k]);
-                }
-            }
-        } else {
-            for (int j = j0; j < j1; ++j) {
-                const float4 at = a.tiles[j];
-                const double *T64 = a.tiles64 + int64_t(j) * 3;
-                const float dx = at.x - gx, dy = at.y - gy;
-                const float dxy2 = __builtin_fmaf(dx, dx, dy * dy);
-#pragma unroll
-                for (int k = 0; k < PZ; ++k) {
-                    const float dz = at.z - gz[k];
-                    const float r2 = __builtin_fmaf(dz, dz, dxy2);
-                    bool include = r2 > a.r2Hi;
-                    const bool amb = !include && r2 >= a.r2Lo;
-                    if (amb) {
-                        // exact FP64 predicate of the cutoff kernel: r^2 <= 64 is near
-                        const double ddx = T64[0] - (ix - a.g.dx2) * a.g.spacing;
-                        const double ddy = T64[1] - (iy - a.g.dy2) * a.g.spacing;
-                        const double ddz = T64[2] - (iz0 + k - a.g.dz2) * a.g.spacing;
-                        include = !(ddx * ddx + ddy * ddy + ddz * ddz <= 64.0);
+                    for (int i = 0; i < PX; ++i) {
+                        const float dx = at.x - gx[i];
+                        dxy2[i] = __builtin_fmaf(dx, dx, dy2);
                     }
-                    // every lane does the arithmetic and excluded pairs add
-                    // 0; the bin is the far loop's, clamped into the table
-                    // for the excluded ones (r = 0 gives a NaN pattern)
-                    const float ri = __builtin_amdgcn_rsqf(r2);
-                    unsigned bin = __float_as_uint(__builtin_fmaf(r2, ri, kFloorBias));
-                    bin = min(max(bin, kFloorBits), clampBits);
-                    const float ris = include ? ri : 0.f;
-                    const float ev = *reinterpret_cast<const float *>(epsBytes + ((bin << 2) + biasBytes));
-                    part[k] = __builtin_fmaf(at.w * ris, ev, part[k]);
+#pragma unroll
+                    for (int k = 0; k < PZ; ++k) dz[k] = at.z - gz[k];
+#pragma unroll
+                    for (int q = 0; q < NP; ++q) {
+                        const float r2 = __builtin_fmaf(dz[q % PZ], dz[q % PZ], dxy2[q / PZ]);
+                        bool include = r2 > a.r2Hi;
+                        const bool amb = !include && r2 >= a.r2Lo;
+                        if (amb) {
+                            // exact FP64 predicate of the cutoff kernel: r^2 <= 64 is near
+                            const double ddx = T64[0] - (ix + q / PZ - a.g.dx2) * a.g.spacing;
+                            const double ddy = T64[1] - (iy - a.g.dy2) * a.g.spacing;
+                            const double ddz = T64[2] - (iz0 + q % PZ - a.g.dz2) * a.g.spacing;
+                            include = !(ddx * ddx + ddy * ddy + ddz * ddz <= 64.0);
+                        }
+                        // every lane does the arithmetic; an excluded pair has
+                        // its r^2 replaced by +inf, so 1/r is 0 and its biased
+                        // bin (a NaN pattern) is clamped into the table
+                        const float ri = __builtin_amdgcn_rsqf(include ? r2 : __builtin_inff());
+                        const unsigned bin =
+                            min(__float_as_uint(__builtin_fmaf(include ? r2 : __builtin_inff(), ri, kFloorBias)),
+                                clampBits);
+                        const float ev = *reinterpret_cast<const float *>(epsBytes + ((bin << 2) + biasBytes));
+                        part[q] = __builtin_fmaf(at.w * ri, ev, part[q]);
+                    }
                 }
             }
-        }
-        if (kComp) {
-            // fold the tile's FP32 partial into the two-float accumulator (TwoSum)
+            if (kComp && ((c1 & (kFold - 1)) == 0 || c1 == a.nAtoms)) {
+                // fold the FP32 partial into the two-float accumulator (TwoSum)
 #pragma unroll
-            for (int k = 0; k < PZ; ++k) {
-                const float s = hi[k] + part[k];
-                const float bp = s - hi[k];
-                const float err = (hi[k] - (s - bp)) + (part[k] - bp);
-                hi[k] = s;
-                lo[k] += err;
-                part[k] = 0.f;
+                for (int q = 0; q < NP; ++q) {
+                    const float s = hi[q] + part[q];
+                    const float bp = s - hi[q];
+                    const float err = (hi[q] - (s - bp)) + (part[q] - bp);
+                    hi[q] = s;
+                    lo[q] += err;
+                    part[q] = 0.f;
+                }
             }
         }
     }
-    if (ix >= a.g.nx || iy >= a.g.ny) return;
+    if (iy >= a.g.ny) return;
 #pragma unroll
-    for (int k = 0; k < PZ; ++k) {
-        const int iz = iz0 + k;
-        if (iz >= a.g.nz) break;
-        const int64_t p = (int64_t(iz) * a.g.ny + iy) * a.g.nx + ix;
-        if (kComp) a.e[p] = a.e[p] + (double(hi[k]) + double(lo[k]));
-        else a.e[p] = double(part[k]);
+    for (int i = 0; i < PX; ++i) {
+        if (ix + i >= a.g.nx) break;
+#pragma unroll
+        for (int k = 0; k < PZ; ++k) {
+            const int iz = iz0 + k;
+            if (iz >= a.g.nz) break;
+            const int q = i * PZ + k;
+            const int64_t p = (int64_t(iz) * a.g.ny + iy) * a.g.nx + ix + i;
+            if (kComp) a.e[p] = a.e[p] + (double(hi[q]) + double(lo[q]));
+            else a.e[p] = double(part[q]);
+        }
     }
 }
 
 // One job: one workgroup per brick.
-template <int PZ, bool kComp, bool kClamp>
+template <int PX, int PZ, bool kComp, bool kClamp>
 __global__ __launch_bounds__(kElecThreads) void k_elec(ElecArgs a)
 {
     extern __shared__ float eps[];
     loadEps(eps, a.eps32, a.tableLen);
     __syncthreads();
-    elecBrick<PZ, kComp, kClamp>(a, int(blockIdx.x), eps);
+    elecBrick<PX, PZ, kComp, kClamp>(a, int(blockIdx.x), eps);
 }
 
 // ------------------------------------------------------------------------
@@ -905,7 +935,7 @@ __global__ __launch_bounds__(256) void k_fill_i32(int32_t *p, int n, int32_t v)
 // sharedTableLen > 0: every job uses the same table (entries past 8191
 // repeat entry 8191), loaded once from jobs[0] with that length; else each
 // job's table is loaded when the workgroup moves to it.
-template <int PZ, bool kComp, bool kClamp>
+template <int PX, int PZ, bool kComp, bool kClamp>
 __global__ __launch_bounds__(kElecThreads) void k_elec_batch(const ElecArgs *jobs, const int32_t *base, int nJobs,
                                                              int *counter, int sharedTableLen)
 {
@@ -924,7 +954,7 @@ __global__ __launch_bounds__(kElecThreads) void k_elec_batch(const ElecArgs *job
             loaded = job;
         }
         __syncthreads();  // table loaded
-        elecBrick<PZ, kComp, kClamp>(a, item - base[job], eps);
+        elecBrick<PX, PZ, kComp, kClamp>(a, item - base[job], eps);
     }
 }
 

@@ -207,11 +207,13 @@ int fg_job_cutoff_pairs(fg_ctx *ctx, fg_job *job, int64_t *pairs);
 /* rules, FG/electrostatics/Cuda.cpp:75-128)                        */
 /* ---------------------------------------------------------------- */
 /* Shape of a job's electrostatic kernel: grid points per lane along z (4 or
- * 8), receptor atoms per tile (32, 64 or 128) and lanes of the 256-lane
- * brick along x (8, 16 or 32; 256: the brick is 256 consecutive columns of
- * the x-y plane, which pads only at the end of the plane). */
+ * 8), receptor atoms per tile (32, 64 or 128), lanes of the 256-lane brick
+ * along x (8, 16 or 32; 256: the brick is 256 consecutive columns of the x-y
+ * plane, which pads only at the end of the plane) and consecutive grid points
+ * per lane along x (1 or 2; 2 needs bx 8, 16 or 32).  The shape changes only
+ * the speed: every shape gives the same bits. */
 typedef struct {
-    int32_t pz, tile, bx;
+    int32_t pz, tile, bx, px;
 } fg_shape;
 int fg_job_get_shape(const fg_job *job, fg_shape *out);
 int fg_job_set_shape(fg_job *job, const fg_shape *shape);
@@ -219,9 +221,9 @@ int fg_job_set_shape(fg_job *job, const fg_shape *shape);
 /* Groups the jobs into size classes (atom count and point count octaves,
  * best-padding brick width), times every shape on the median job of each
  * class with HIP events, and gives every job its class's fastest shape.  The
- * points per lane along z are then made common to all the jobs (a batch
- * launches one kernel for all of them): the value with the least total
- * modelled time wins.  precision: FG_PREC_COMPENSATED or FG_PREC_FP32.  The
+ * points per lane along x and z are common to all the jobs (a batch launches
+ * one kernel for all of them): the pair with the least total modelled time
+ * wins.  precision: FG_PREC_COMPENSATED or FG_PREC_FP32.  The
  * jobs' maps are undefined until their next run.  fg_tune_report() returns
  * one text line per class of the last call on ctx. */
 int fg_autotune(fg_ctx *ctx, fg_job *const *jobs, int n_jobs, int precision);
