@@ -183,20 +183,48 @@ def pocket_specs(n):
 
 def run_batch(fg, ctx, steps, warmup, n_pockets):
     """The 256-pocket batch: one hipGraph of persistent kernels against one
-    plain launch per pocket (and one compensated launch per pocket)."""
+    plain launch per pocket (and one compensated launch per pocket), plus the
+    end-to-end time from the GPF files to the maps in HBM and on the host."""
     with tempfile.TemporaryDirectory() as wd:
-        arrays = []
+        gpfs = []
         for i, (seed, atoms, npts) in enumerate(pocket_specs(n_pockets)):
             radius = (3.0 * atoms / (4.0 * math.pi * BATCH["density"])) ** (1.0 / 3.0)
             fg.synth_write(wd, f"p{i}", seed, atoms, "sphere", radius, npts, BATCH["ligand_types"])
-            cwd = os.getcwd()
-            os.chdir(wd)
-            try:
-                arrays.append(fg.Prepared(f"p{i}.gpf").arrays())
-            finally:
-                os.chdir(cwd)
-    jobs = [fg.Job(ctx, a) for a in arrays]
-    pairs = sum(a.n_atoms * a.P for a in arrays)
+            gpfs.append(f"p{i}.gpf")
+        cwd = os.getcwd()
+        os.chdir(wd)
+        try:
+            t0 = time.perf_counter()
+            fg.prepare_many(gpfs, threads=1)
+            threads = min(16, os.cpu_count() or 1)
+            t_prep1 = time.perf_counter() - t0
+            # end to end with the default kernel shapes: parse + precompute on
+            # all host cores, upload, one graph of the whole batch, maps fetched
+            t0 = time.perf_counter()
+            preps = fg.prepare_many(gpfs, threads=threads)
+            t1 = time.perf_counter()
+            jobs = fg.create_jobs(ctx, preps, threads=threads)
+            ctx.sync()
+            t2 = time.perf_counter()
+            first = fg.Batch(ctx, jobs, fg.RUN_ALL, fg.PREC_COMPENSATED)
+            t3 = time.perf_counter()
+            first.launch()
+            ctx.sync()
+            t4 = time.perf_counter()
+            for j in jobs:
+                j.fetch()
+            t5 = time.perf_counter()
+            first.close()
+        finally:
+            os.chdir(cwd)
+    e2e = {"seconds": round(t5 - t0, 3), "prepare_s": round(t1 - t0, 3), "prepare_one_thread_s": round(t_prep1, 3),
+           "host_threads": threads, "create_jobs_s": round(t2 - t1, 3), "build_graph_s": round(t3 - t2, 3),
+           "first_launch_s": round(t4 - t3, 3), "fetch_maps_s": round(t5 - t4, 3),
+           "form": "GPF files -> fg_prep_load_many (host threads) -> fg_job_create_many (host threads) -> "
+                   "fg_batch_create (graph) "
+                   "-> one launch -> fg_job_fetch of every map (FP64), default kernel shapes"}
+    log(f"batch end to end: {e2e}")
+    pairs = sum(p.n_atoms * p.P for p in preps)
     pcut = sum(j.cutoff_pairs() for j in jobs)
     log(f"batch: {n_pockets} pockets, sum N*P = {pairs:.4e}, sum P_cut = {pcut:.4e}")
     t0 = time.perf_counter()
@@ -234,7 +262,7 @@ def run_batch(fg, ctx, steps, warmup, n_pockets):
     ctx.sync()
     names = None
     worst = {"compensated": {}, "plain": {}}
-    for a, j in zip(arrays, jobs):
+    for j in jobs:
         comp = j.fetch()
         j.run(fg.RUN_ALL, fg.PREC_FP64)
         ref = j.fetch()
@@ -266,6 +294,7 @@ def run_batch(fg, ctx, steps, warmup, n_pockets):
         "speedup_vs_sequential_plain": round(t_plain / t_batch, 2),
         "speedup_vs_sequential_compensated": round(t_seq / t_batch, 2),
         "autotune_s": round(t_tune, 2),
+        "end_to_end": e2e,
         "autotune_report": report.splitlines()[0],
         "max_error_vs_fp64": worst,
     }

@@ -90,6 +90,7 @@ SIGNATURES = {
                                      C.POINTER(FgTables), C.POINTER(FgMapParams), C.c_int, C.c_double, C.c_int,
                                      C.POINTER(_dp), _dp]),
     "fg_job_create": (C.c_int, [C.c_void_p, C.POINTER(FgJobDesc), C.POINTER(C.c_void_p)]),
+    "fg_job_create_many": (C.c_int, [C.c_void_p, C.POINTER(FgJobDesc), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "fg_job_destroy": (None, [C.c_void_p]),
     "fg_job_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_int]),
     "fg_sync": (C.c_int, [C.c_void_p]),
@@ -107,6 +108,8 @@ SIGNATURES = {
     "fg_last_timing": (C.c_int, [C.c_void_p, C.POINTER(FgTiming)]),
     "fg_prep_load": (C.c_int, [C.c_char_p, C.POINTER(FgLoadOpts), C.POINTER(C.c_void_p)]),
     "fg_prep_exit_code": (C.c_int, []),
+    "fg_prep_load_many": (C.c_int, [C.POINTER(C.c_char_p), C.c_int, C.POINTER(FgLoadOpts), C.c_int,
+                                    C.POINTER(C.c_void_p)]),
     "fg_prep_free": (None, [C.c_void_p]),
     "fg_prep_desc": (C.c_int, [C.c_void_p, C.POINTER(FgJobDesc)]),
     "fg_prep_num_maps": (C.c_int, [C.c_void_p]),
@@ -258,13 +261,27 @@ class JobArrays:
 class Prepared:
     """A GPF loaded and precomputed by the host front end (fg_prep_load)."""
 
-    def __init__(self, gpf: str, v4: bool = False, ag421_param_order: bool = False):
+    def __init__(self, gpf: str | None, v4: bool = False, ag421_param_order: bool = False):
         self._h = C.c_void_p()
+        if gpf is None:  # filled by prepare_many
+            return
         opts = FgLoadOpts(1 if v4 else 0, 1 if ag421_param_order else 0)
         rc = LIB.fg_prep_load(gpf.encode(), C.byref(opts), C.byref(self._h))
         if rc != FG_OK:
             self.exit_code = LIB.fg_prep_exit_code()
             _check(rc)
+
+    @property
+    def n_atoms(self) -> int:
+        return int(LIB.fg_prep_num_atoms(self._h))
+
+    @property
+    def P(self) -> int:
+        return int(LIB.fg_prep_num_points(self._h))
+
+    @property
+    def n_maps(self) -> int:
+        return int(LIB.fg_prep_num_maps(self._h))
 
     def close(self) -> None:
         if self._h:
@@ -314,6 +331,62 @@ class Prepared:
         keep = [np.ascontiguousarray(m, dtype=np.float64) for m in maps]
         ptrs = (_dp * len(keep))(*[_ptr(m, C.c_double) for m in keep])
         _check(LIB.fg_write_outputs(self._h, ptrs, program_name.encode(), threads))
+
+
+def prepare_many(gpfs: list[str], threads: int = 0, v4: bool = False,
+                 ag421_param_order: bool = False) -> list[Prepared]:
+    """Loads and precomputes many GPFs on host threads (fg_prep_load_many)."""
+    n = len(gpfs)
+    names = (C.c_char_p * max(n, 1))(*[g.encode() for g in gpfs])
+    out = (C.c_void_p * max(n, 1))()
+    opts = FgLoadOpts(1 if v4 else 0, 1 if ag421_param_order else 0)
+    rc = LIB.fg_prep_load_many(names, n, C.byref(opts), int(threads), out)
+    if rc != FG_OK:
+        err = FgError(rc, LIB.fg_last_error().decode(errors="replace"))
+        err.exit_code = LIB.fg_prep_exit_code()
+        raise err
+    preps = []
+    for i in range(n):
+        p = Prepared(None)
+        p._h = C.c_void_p(out[i])
+        preps.append(p)
+    return preps
+
+
+@dataclass
+class _JobInfo:
+    """What a job fetches: points, affinity maps and whether a floating grid."""
+    P: int
+    n_maps: int
+    floating_grid: int
+
+
+def _describe(item):
+    """(fg_job_desc, objects to keep alive, what the job fetches) of host
+    arrays or of a Prepared."""
+    if isinstance(item, Prepared):
+        d = FgJobDesc()
+        _check(LIB.fg_prep_desc(item.handle, C.byref(d)))
+        g = d.grid
+        return d, [item], _JobInfo(P=g.nx * g.ny * g.nz, n_maps=d.n_maps, floating_grid=d.floating_grid)
+    d, keep = item.desc()
+    return d, keep, item
+
+
+def create_jobs(ctx: "Context", items: list, threads: int = 0) -> list["Job"]:
+    """Jobs of many JobArrays / Prepared at once (fg_job_create_many): the
+    host-side inputs are built on parallel threads."""
+    n = len(items)
+    described = [_describe(it) for it in items]
+    descs = (FgJobDesc * max(n, 1))(*[x[0] for x in described])
+    out = (C.c_void_p * max(n, 1))()
+    _check(LIB.fg_job_create_many(ctx.handle, descs, n, int(threads), out))
+    jobs = []
+    for i, (_, keep, info) in enumerate(described):
+        j = Job.__new__(Job)
+        j.ctx, j._h, j._keep, j.arrays = ctx, C.c_void_p(out[i]), keep, info
+        jobs.append(j)
+    return jobs
 
 
 def device_count() -> int:
@@ -394,11 +467,11 @@ class Context:
 class Job:
     """Device-resident inputs and maps of one map set (fg_job_*)."""
 
-    def __init__(self, ctx: Context, arrays: JobArrays):
+    def __init__(self, ctx: Context, arrays: JobArrays | Prepared):
+        """From host arrays, or straight from a Prepared (no copy of its arrays)."""
         self.ctx = ctx
-        self.arrays = arrays
-        d, self._keep = arrays.desc()
         self._h = C.c_void_p()
+        d, self._keep, self.arrays = _describe(arrays)
         _check(LIB.fg_job_create(ctx.handle, C.byref(d), C.byref(self._h)))
 
     def close(self) -> None:

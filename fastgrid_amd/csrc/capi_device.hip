@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -15,6 +16,8 @@
 #include <limits>
 #include <numeric>
 #include <string>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "capi_common.hpp"
@@ -102,27 +105,75 @@ struct fg_job {
     // outputs: [M affinity][e][d]
     double *dMaps = nullptr;
     std::vector<void *> owned;
+    // Device buffers are carved out of a few allocations: devAlloc/devUpload
+    // record a slot (and stage the upload's bytes); flushArena makes one
+    // allocation for the recorded slots, one copy of the staged bytes, and
+    // sets the recorded pointers.
+    struct Slot {
+        void **dst;          // the job's pointer to set
+        size_t off, bytes;   // place in the arena
+        const void *direct;  // caller memory copied straight in (not staged)
+    };
+    std::vector<Slot> slots;
+    std::vector<unsigned char> stage;  // zero-filled between staged uploads
+    size_t arenaBytes = 0;
 };
 
 namespace {
 
+// A device buffer of `count` elements in the job's next arena; `ptr` is set
+// by flushArena.
 template <class T>
 int devAlloc(fg_job *j, T *&ptr, size_t count)
 {
     ptr = nullptr;
     if (count == 0) count = 1;
-    void *p = nullptr;
-    FG_HIP(hipMalloc(&p, count * sizeof(T)));
-    j->owned.push_back(p);
-    ptr = static_cast<T *>(p);
+    const size_t off = (j->arenaBytes + 255) & ~size_t(255);
+    j->arenaBytes = off + count * sizeof(T);
+    j->slots.push_back(fg_job::Slot{reinterpret_cast<void **>(&ptr), off, count * sizeof(T), nullptr});
     return FG_OK;
 }
 
+// A device buffer holding a copy of host[0..count) (staged now, so `host`
+// may be a temporary).
 template <class T>
 int devUpload(fg_job *j, T *&ptr, const T *host, size_t count)
 {
     if (int rc = devAlloc(j, ptr, count)) return rc;
-    if (count) FG_HIP(hipMemcpy(ptr, host, count * sizeof(T), hipMemcpyHostToDevice));
+    if (count) {
+        const fg_job::Slot &sl = j->slots.back();
+        if (j->stage.size() < sl.off + sl.bytes) j->stage.resize(sl.off + sl.bytes);
+        std::memcpy(j->stage.data() + sl.off, host, count * sizeof(T));
+    }
+    return FG_OK;
+}
+
+// As devUpload for caller memory that stays valid until flushArena: copied
+// straight from `host` (large arrays are not staged twice).
+template <class T>
+int devUploadDirect(fg_job *j, T *&ptr, const T *host, size_t count)
+{
+    if (int rc = devAlloc(j, ptr, count)) return rc;
+    if (count) j->slots.back().direct = host;
+    return FG_OK;
+}
+
+// One allocation for every slot recorded since the last flush, one copy of
+// the staged bytes, then the direct copies.
+int flushArena(fg_job *j)
+{
+    if (j->slots.empty()) return FG_OK;
+    void *base = nullptr;
+    FG_HIP(hipMalloc(&base, j->arenaBytes));
+    j->owned.push_back(base);
+    char *b = static_cast<char *>(base);
+    for (const fg_job::Slot &sl : j->slots) *sl.dst = b + sl.off;
+    if (!j->stage.empty()) FG_HIP(hipMemcpy(b, j->stage.data(), j->stage.size(), hipMemcpyHostToDevice));
+    for (const fg_job::Slot &sl : j->slots)
+        if (sl.direct) FG_HIP(hipMemcpy(b + sl.off, sl.direct, sl.bytes, hipMemcpyHostToDevice));
+    j->slots.clear();
+    std::vector<unsigned char>().swap(j->stage);
+    j->arenaBytes = 0;
     return FG_OK;
 }
 
@@ -562,15 +613,6 @@ int buildCutoff(fg_job *j, const fg_job_desc &d)
     if (int rc = devAlloc(j, j->dCounts, size_t(2) * size_t(j->nb))) return rc;
     if (int rc = devAlloc(j, j->dStart, size_t(j->nb) + 1)) return rc;
     if (int rc = devAlloc(j, j->dHStart, size_t(j->nb) + 1)) return rc;
-    // size the candidate lists with one count pass
-    fgk::ListArgs la = listArgs(j);
-    if (int rc = runListCount(j, la)) return rc;
-    int32_t tot = 0, totH = 0;
-    FG_HIP(hipMemcpyAsync(&tot, j->dStart + j->nb, sizeof tot, hipMemcpyDeviceToHost, j->ctx->stream));
-    FG_HIP(hipMemcpyAsync(&totH, j->dHStart + j->nb, sizeof totH, hipMemcpyDeviceToHost, j->ctx->stream));
-    FG_HIP(hipStreamSynchronize(j->ctx->stream));
-    if (int rc = devAlloc(j, j->dList, size_t(tot))) return rc;
-    if (int rc = devAlloc(j, j->dHList, size_t(totH))) return rc;
     // the typed cutoff kernel: up to 8 maps (its LDS holds one type's rows for
     // 8 maps) and up to kMaxTypes receptor types
     j->typed = j->hasGeom && j->M > 0 && j->M <= 8 && j->T >= 1 && j->T <= fgk::kMaxTypes;
@@ -578,8 +620,6 @@ int buildCutoff(fg_job *j, const fg_job_desc &d)
         std::vector<int32_t> ty(size_t(std::max(N, 1)), 0);
         for (int i = 0; i < N; ++i) ty[size_t(i)] = A.type[i];
         if (int rc = devUpload(j, j->dType, ty.data(), ty.size())) return rc;
-        if (int rc = devAlloc(j, j->dSorted, size_t(tot))) return rc;
-        if (int rc = devAlloc(j, j->dTypeStart, size_t(j->nb) * size_t(j->T + 1))) return rc;
     }
     // tables
     const size_t pairN = size_t(std::max(d.tables.n_types, 1)) * fgk::kTableLen * size_t(std::max(j->M, 1));
@@ -602,9 +642,28 @@ int buildCutoff(fg_job *j, const fg_job_desc &d)
     if (d.dist_dep_diel) std::copy(d.tables.eps, d.tables.eps + fgk::kTableLen, eps.begin());
     if (int rc = devUpload(j, j->dEps, eps.data(), eps.size())) return rc;
     if (d.covalent && j->hasGeom) {
-        if (int rc = devUpload(j, j->dCovalent, d.covalent, size_t(j->M) * size_t(j->P))) return rc;
+        if (int rc = devUploadDirect(j, j->dCovalent, d.covalent, size_t(j->M) * size_t(j->P))) return rc;
     }
     return FG_OK;
+}
+
+// Candidate lists: one count pass sizes them (the inputs are on the device),
+// then they and the type-sorted copies get an arena of their own.
+int buildLists(fg_job *j)
+{
+    fgk::ListArgs la = listArgs(j);
+    if (int rc = runListCount(j, la)) return rc;
+    int32_t tot = 0, totH = 0;
+    FG_HIP(hipMemcpyAsync(&tot, j->dStart + j->nb, sizeof tot, hipMemcpyDeviceToHost, j->ctx->stream));
+    FG_HIP(hipMemcpyAsync(&totH, j->dHStart + j->nb, sizeof totH, hipMemcpyDeviceToHost, j->ctx->stream));
+    FG_HIP(hipStreamSynchronize(j->ctx->stream));
+    if (int rc = devAlloc(j, j->dList, size_t(tot))) return rc;
+    if (int rc = devAlloc(j, j->dHList, size_t(totH))) return rc;
+    if (j->typed) {
+        if (int rc = devAlloc(j, j->dSorted, size_t(tot))) return rc;
+        if (int rc = devAlloc(j, j->dTypeStart, size_t(j->nb) * size_t(j->T + 1))) return rc;
+    }
+    return flushArena(j);
 }
 
 // FP32 inputs of the plain form: atoms in index order, tables in output map order.
@@ -649,9 +708,8 @@ int buildPlain(fg_job *j, const fg_job_desc &d)
     j->solparQ = float(d.solpar_q);
     if (int rc = devUpload(j, j->dPair32, pair.data(), pair.size())) return rc;
     if (int rc = devUpload(j, j->dDesolv32, desolv.data(), desolv.size())) return rc;
-    if (int rc = devAlloc(j, j->dAcc32, size_t(j->P))) return rc;
-    FG_HIP(hipMemset(j->dAcc32, 0, size_t(j->P) * sizeof(float)));
-    return FG_OK;
+    // the FP32 sum grid starts at zero (cleared once the arena exists)
+    return devAlloc(j, j->dAcc32, size_t(j->P));
 }
 
 // The plain form: cutoff maps (per-point lists) then the chunked e-map.
@@ -1140,9 +1198,14 @@ void fg_destroy(fg_ctx *ctx)
     delete ctx;
 }
 
-int fg_job_create(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
+namespace {
+
+// Host half of job creation: checks the description, builds every input on
+// the host and stages it for the job's first arena.  No HIP calls, so many
+// jobs can be built on parallel threads.  On success *out is a job that is
+// not yet registered with ctx.
+int jobBuildHost(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
 {
-    if (!ctx || !d || !out) return fgc::fail(FG_E_ARG, "fg_job_create: NULL argument");
     *out = nullptr;
     const fg_grid &g = d->grid;
     if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0 || g.nx > 1025 || g.ny > 1025 || g.nz > 1025 || !(g.spacing > 0))
@@ -1150,9 +1213,9 @@ int fg_job_create(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
     if (d->atoms.n < 0 || (d->atoms.n > 0 && (!d->atoms.x || !d->atoms.y || !d->atoms.z)))
         return fgc::fail(FG_E_ARG, "fg_job_create: invalid atoms");
     if (d->n_maps < 0 || d->n_maps > 14) return fgc::fail(FG_E_ARG, "fg_job_create: n_maps must be 0..14");
-    FG_HIP(hipSetDevice(ctx->device));
     fg_job *j = new fg_job();
-    j->ctx = ctx;
+    // j->ctx stays null until the host build succeeds: destroying a job that
+    // failed here touches nothing shared (builds may run on several threads)
     j->g = fgk::GridGeom{g.nx, g.ny, g.nz, g.div2x, g.div2y, g.div2z, g.spacing};
     j->P = int64_t(g.nx) * g.ny * g.nz;
     j->N = d->atoms.n;
@@ -1237,13 +1300,32 @@ int fg_job_create(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
         const long bricks8 = long((g.nx + 15) / 16) * ((g.ny + 15) / 16) * ((g.nz + 7) / 8);
         j->pz = bricks8 >= 1024 ? 8 : 4;
     }
+    // inputs and outputs staged for one arena
     int rc = buildCutoff(j, *d);
     if (!rc) rc = buildElecTiles(j, *d, gridMaxAbs);
     if (!rc) rc = buildPlain(j, *d);
     if (!rc) rc = devAlloc(j, j->dMaps, size_t(j->M + 2) * size_t(j->P));
     if (!rc && j->floating) rc = devAlloc(j, j->dFmap, size_t(j->P));
+    if (rc) {
+        destroyJob(j);
+        return rc;
+    }
+    j->ctx = ctx;
+    *out = j;
+    return FG_OK;
+}
+
+// Device half: the first arena (inputs, outputs), the count pass and the
+// candidate lists' arena, cleared outputs; registers the job with ctx.  The
+// caller-owned arrays of `d` are read again here (direct uploads).  On
+// failure the job is destroyed.
+int jobFinishDevice(fg_ctx *ctx, fg_job *j)
+{
+    int rc = flushArena(j);
+    if (!rc) rc = buildLists(j);
     if (!rc) {
         hipError_t e = hipMemsetAsync(j->dMaps, 0, size_t(j->M + 2) * size_t(j->P) * sizeof(double), ctx->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(j->dAcc32, 0, size_t(j->P) * sizeof(float), ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) rc = fgc::fail(FG_E_DEVICE, std::string("fg_job_create: ") + hipGetErrorString(e));
     }
@@ -1252,8 +1334,74 @@ int fg_job_create(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
         return rc;
     }
     ctx->jobs.push_back(j);
+    return FG_OK;
+}
+
+}  // namespace
+
+int fg_job_create(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
+{
+    if (!ctx || !d || !out) return fgc::fail(FG_E_ARG, "fg_job_create: NULL argument");
+    *out = nullptr;
+    FG_HIP(hipSetDevice(ctx->device));
+    fg_job *j = nullptr;
+    if (int rc = jobBuildHost(ctx, d, &j)) return rc;
+    if (int rc = jobFinishDevice(ctx, j)) return rc;
     *out = j;
     return FG_OK;
+}
+
+int fg_job_create_many(fg_ctx *ctx, const fg_job_desc *descs, int n, int threads, fg_job **out)
+{
+    if (!ctx || !out || n < 0 || (!descs && n > 0)) return fgc::fail(FG_E_ARG, "fg_job_create_many: NULL argument");
+    for (int i = 0; i < n; ++i) out[i] = nullptr;
+    FG_HIP(hipSetDevice(ctx->device));
+    if (threads <= 0) threads = int(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+    threads = std::min(threads, std::max(n, 1));
+    std::vector<int> rcs(size_t(n), FG_OK);
+    std::vector<std::string> errs(static_cast<size_t>(n));
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int i; (i = next.fetch_add(1)) < n;) {
+            rcs[size_t(i)] = jobBuildHost(ctx, &descs[i], &out[i]);
+            if (rcs[size_t(i)] != FG_OK) errs[size_t(i)] = fg_last_error();
+        }
+    };
+    std::vector<std::thread> pool;
+    try {
+        for (int t = 1; t < threads; ++t) pool.emplace_back(work);
+    } catch (const std::system_error &) {
+        // fewer threads than asked for: the ones running (and this one) share the work
+    }
+    work();
+    for (std::thread &t : pool) t.join();
+    int bad = -1, rc = FG_OK;
+    std::string msg;
+    for (int i = 0; i < n && bad < 0; ++i)
+        if (rcs[size_t(i)] != FG_OK) {
+            bad = i;
+            rc = rcs[size_t(i)];
+            msg = errs[size_t(i)];
+        }
+    // the device halves, in list order, on the context's stream
+    for (int i = 0; i < n && bad < 0; ++i) {
+        fg_job *j = out[i];
+        out[i] = nullptr;
+        if ((rc = jobFinishDevice(ctx, j)) != FG_OK) {
+            bad = i;
+            msg = fg_last_error();
+        } else {
+            out[i] = j;
+        }
+    }
+    if (bad < 0) return FG_OK;
+    for (int i = 0; i < n; ++i) {
+        if (!out[i]) continue;
+        if (out[i]->ctx) (void)hipStreamSynchronize(ctx->stream);
+        destroyJob(out[i]);
+        out[i] = nullptr;
+    }
+    return fgc::fail(rc, "job " + std::to_string(bad) + ": " + msg);
 }
 
 void fg_job_destroy(fg_job *job)

@@ -1,8 +1,13 @@
 // C ABI of the host front end (include/fgmaps.h): errors, prepared inputs,
 // writers and the synthetic generator.  No device code.
+#include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <new>
 #include <string>
+#include <system_error>
+#include <thread>
+#include <vector>
 
 #include "capi_common.hpp"
 #include "host.hpp"
@@ -56,6 +61,46 @@ int fg_prep_load(const char *gpf, const fg_load_opts *opts, fg_prep **out)
     }
     g_exitCode = 0;
     *out = prep;
+    return FG_OK;
+}
+
+int fg_prep_load_many(const char *const *gpfs, int n, const fg_load_opts *opts, int threads, fg_prep **out)
+{
+    if (!out || n < 0 || (!gpfs && n > 0)) return fgc::fail(FG_E_ARG, "fg_prep_load_many: NULL argument");
+    for (int i = 0; i < n; ++i) out[i] = nullptr;
+    if (threads <= 0) threads = int(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+    threads = std::min(threads, std::max(n, 1));
+    // every worker keeps the status, message and exit code of its own loads
+    std::vector<int> rcs(size_t(n), FG_OK), exits(size_t(n), 0);
+    std::vector<std::string> errs(static_cast<size_t>(n));
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int i; (i = next.fetch_add(1)) < n;) {
+            rcs[size_t(i)] = fg_prep_load(gpfs[i], opts, &out[i]);
+            if (rcs[size_t(i)] != FG_OK) {
+                errs[size_t(i)] = g_error;
+                exits[size_t(i)] = g_exitCode;
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    try {
+        for (int t = 1; t < threads; ++t) pool.emplace_back(work);
+    } catch (const std::system_error &) {
+        // fewer threads than asked for: the ones running (and this one) share the work
+    }
+    work();
+    for (std::thread &t : pool) t.join();
+    for (int i = 0; i < n; ++i)
+        if (rcs[size_t(i)] != FG_OK) {
+            for (int k = 0; k < n; ++k) {
+                fg_prep_free(out[k]);
+                out[k] = nullptr;
+            }
+            g_exitCode = exits[size_t(i)];
+            return fgc::fail(rcs[size_t(i)], "GPF " + std::to_string(i) + ": " + errs[size_t(i)]);
+        }
+    g_exitCode = 0;
     return FG_OK;
 }
 

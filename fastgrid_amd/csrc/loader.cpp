@@ -18,6 +18,11 @@
 #include <cctype>
 #include <cmath>
 #include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
 
 #include "host.hpp"
 
@@ -79,6 +84,51 @@ std::string firstArg(const char *line)
     char buf[256] = "";
     std::sscanf(line, "%*s %255s", buf);
     return buf;
+}
+
+// First whitespace-delimited word of `s`, at most `maxLen` characters, into
+// out (as sscanf "%<maxLen>s" stores it); returns its length, leaving out
+// unchanged (and returning 0) when `s` holds no word.
+size_t word(const char *s, char *out, size_t maxLen)
+{
+    while (*s && std::isspace(static_cast<unsigned char>(*s))) ++s;
+    size_t n = 0;
+    while (s[n] && !std::isspace(static_cast<unsigned char>(s[n])) && n < maxLen) ++n;
+    if (n == 0) return 0;
+    std::memcpy(out, s, n);
+    out[n] = 0;
+    return n;
+}
+
+// Value of the number at the start of `s` as strtod gives it.  Plain
+// decimals (optional sign, at most 15 digits around one point, then a blank
+// or the end) are m / 10^d with m and 10^d exact doubles, so the one
+// correctly rounded division is the nearest double to the decimal, which is
+// strtod's result; any other text goes to strtod.
+double readNumber(const char *s)
+{
+    static const double kPow10[16] = {1e0, 1e1, 1e2, 1e3, 1e4, 1e5, 1e6, 1e7, 1e8, 1e9, 1e10, 1e11, 1e12, 1e13, 1e14, 1e15};
+    const char *q = s;
+    while (*q == ' ' || *q == '\t') ++q;
+    bool neg = false;
+    if (*q == '-' || *q == '+') neg = *q++ == '-';
+    uint64_t m = 0;
+    int digits = 0, frac = -1;
+    for (;; ++q) {
+        if (*q >= '0' && *q <= '9') {
+            m = m * 10 + uint64_t(*q - '0');
+            if (frac >= 0) ++frac;
+            if (++digits > 15) return std::strtod(s, nullptr);
+        } else if (*q == '.' && frac < 0) {
+            frac = 0;
+        } else {
+            break;
+        }
+    }
+    const bool endOk = *q == 0 || std::isspace(static_cast<unsigned char>(*q));
+    if (digits == 0 || !endOk) return std::strtod(s, nullptr);
+    const double v = frac > 0 ? double(m) / kPow10[frac] : double(m);
+    return neg ? -v : v;
 }
 
 // libstdc++ evaluates pow(double, int) in C++98 by binary exponentiation
@@ -159,20 +209,24 @@ int loadGpf(const std::string &gpfPath, const fg_load_opts &opts, Prepared &p, s
             }
             std::array<double, 3> sum{0, 0, 0};
             double qmax = -1e12, qmin = 1e12;
-            char rec[256] = "", row[256];
+            char rec[8] = "", row[256];
             while (std::fgets(row, sizeof row, rf)) {
-                std::sscanf(row, "%6s", rec);
+                // the record name is the first word, at most 6 characters
+                // (sscanf "%6s"); a blank line keeps the previous one
+                word(row, rec, 6);
                 if (std::strncmp(rec, "ATOM", 4) && std::strncmp(rec, "HETA", 4) && std::strncmp(rec, "CHAR", 4))
                     continue;
-                // fixed columns: x 30, y 38, z 46, charge 70, type 77
+                // fixed columns: x 30, y 38, z 46, charge 70, type 77; each
+                // value is read as sscanf "%lf" reads it (strtod of the text
+                // from that column, 0 when no number starts there)
                 size_t len = std::strlen(row);
                 double c[4] = {0, 0, 0, 0};
                 const size_t col[4] = {30, 38, 46, 70};
                 for (int i = 0; i < 4; ++i)
-                    if (len > col[i]) std::sscanf(row + col[i], "%lf", &c[i]);
-                char t[256] = "";
-                if (len > 77) std::sscanf(row + 77, "%255s", t);
-                std::string tname = std::string(t).substr(0, 7);
+                    if (len > col[i]) c[i] = readNumber(row + col[i]);
+                char t[8] = "";
+                if (len > 77) word(row + 77, t, 7);
+                std::string tname(t);
                 AtomParam *a = lib.find(tname);
                 if (!a) {
                     fail(-1, "receptor file contains unknown type: '" + tname + "'");
@@ -420,6 +474,77 @@ int loadGpf(const std::string &gpfPath, const fg_load_opts &opts, Prepared &p, s
 
 namespace {
 
+// r^x at r = k/100, k = 0..8191, with libm's pow(double, double) as the
+// reference calls it (FG/PairwiseInteractionEnergies.cpp:93-97).  Every row
+// of every table reads the same few exponents, so each exponent's powers are
+// computed once per process; the values are those of the direct calls.
+const double *powersOf(double x)
+{
+    static std::mutex mu;
+    static std::map<double, std::unique_ptr<double[]>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    std::unique_ptr<double[]> &slot = cache[x];
+    if (!slot) {
+        slot.reset(new double[kTableLen]);
+        for (int k = 0; k < kTableLen; ++k) slot[size_t(k)] = std::pow(double(k) * (1 / 100.0), x);
+    }
+    return slot.get();
+}
+
+// One row of the pair-energy table (FG/PairwiseInteractionEnergies.cpp:93-114):
+// min(1e5, cA / r^xA - cB / r^xB) at r = k/100, entry 0 = 1e5 and entry 8191
+// = 0, then for k >= 1 the minimum over entries [k - half, k + half) (the
+// reference's running minimum, here with a monotonic queue: the same values).
+void pairRow(double cA, double cB, int xA, int xB, int half, double *row)
+{
+    const double *rA = powersOf(double(xA)), *rB = powersOf(double(xB));
+    for (int k = 1; k < kTableLen; ++k) row[k] = std::min(100000.0, cA / rA[k] - cB / rB[k]);
+    row[0] = 100000.0;
+    row[kTableLen - 1] = 0;
+    if (half <= 0) return;
+    std::vector<double> smooth(kTableLen);
+    std::deque<int> q;  // window indices, values increasing from the front
+    int next = 0;       // next index to enter the window
+    for (int k = 1; k < kTableLen; ++k) {
+        const int lo = std::max(0, k - half), hi = std::min(kTableLen, k + half);
+        for (; next < hi; ++next) {
+            while (!q.empty() && row[q.back()] >= row[next]) q.pop_back();
+            q.push_back(next);
+        }
+        while (!q.empty() && q.front() < lo) q.pop_front();
+        smooth[size_t(k)] = q.empty() ? 100000.0 : std::min(100000.0, row[q.front()]);
+    }
+    for (int k = 1; k < kTableLen; ++k) row[k] = smooth[size_t(k)];
+}
+
+// Rows depend only on (cA, cB, xA, xB, half); preparations that share a
+// parameter set (a batch of pockets) share their rows.  At most kRowCache
+// rows (64 KiB each) are kept per process.
+constexpr size_t kRowCache = 1024;
+
+void cachedPairRow(double cA, double cB, int xA, int xB, int half, double *row)
+{
+    using Key = std::tuple<double, double, int, int, int>;
+    static std::mutex mu;
+    static std::map<Key, std::unique_ptr<double[]>> cache;
+    const Key key{cA, cB, xA, xB, half};
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            std::copy(it->second.get(), it->second.get() + kTableLen, row);
+            return;
+        }
+    }
+    pairRow(cA, cB, xA, xB, half, row);
+    std::lock_guard<std::mutex> lock(mu);
+    if (cache.size() < kRowCache && !cache.count(key)) {
+        std::unique_ptr<double[]> v(new double[kTableLen]);
+        std::copy(row, row + kTableLen, v.get());
+        cache.emplace(key, std::move(v));
+    }
+}
+
 // Pair-energy table T[type][k][map] (FG/PairwiseInteractionEnergies.cpp:59-115):
 // clamped 12-6 / 12-10 potential at r = k/100, then a running minimum over
 // +-rSmooth/2.
@@ -434,7 +559,7 @@ int pairTables(Prepared &p, std::string &err)
     p.tableTypes = int32_t(rows);
     p.pair.assign(rows * kTableLen * std::max<size_t>(M, 1), 0.0);
     const int half = int(p.rSmooth * 100 / 2);
-    std::vector<double> row(kTableLen), smooth(kTableLen);
+    std::vector<double> row(kTableLen);
     for (size_t m = 0; m < M; ++m) {
         const MapDef &md = p.maps[m];
         if (md.isCovalent) continue;
@@ -451,20 +576,7 @@ int pairTables(Prepared &p, std::string &err)
                 err = "Van der Waals exponent is 0.";
                 return -2;
             }
-            for (int k = 1; k < kTableLen; ++k) {
-                double r = double(k) * (1 / 100.0);
-                row[size_t(k)] = std::min(100000.0, cA / std::pow(r, double(xA)) - cB / std::pow(r, double(xB)));
-            }
-            row[0] = 100000.0;
-            row[kTableLen - 1] = 0;
-            if (half > 0) {
-                for (int k = 1; k < kTableLen; ++k) {
-                    double v = 100000;
-                    for (int j = std::max(0, k - half); j < std::min(kTableLen, k + half); ++j) v = std::min(v, row[size_t(j)]);
-                    smooth[size_t(k)] = v;
-                }
-                for (int k = 1; k < kTableLen; ++k) row[size_t(k)] = smooth[size_t(k)];
-            }
+            cachedPairRow(cA, cB, xA, xB, half, row.data());
             for (int k = 0; k < kTableLen; ++k) p.pair[(t * kTableLen + size_t(k)) * M + m] = row[size_t(k)];
         }
     }
@@ -589,22 +701,36 @@ void bondVectors(Prepared &p, ParamLibrary &lib)
 int precompute(Prepared &p, ParamLibrary &lib, std::string &err)
 {
     // inverse distance-dependent dielectric of Mehler & Solmajer at r = k/100
-    // (FG/autogrid.h:106-112, FG/InputDataLoader.cpp:722-723)
-    p.eps.assign(kTableLen, 0.0);
-    if (p.distDepDiel)
-        for (int k = 0; k < kTableLen; ++k) {
-            double r = double(k) * (1 / 100.0);
-            double E = std::exp(-0.3153767175 * r);
-            p.eps[size_t(k)] = (332.0 + (332 * 7.7839) * E) / (78.4 + -66.57180475 * E);
+    // (FG/autogrid.h:106-112, FG/InputDataLoader.cpp:722-723), and the
+    // desolvation Gaussian, sigma 3.6 A, times coeff_desolv
+    // (FG/DesolvExpFunc.cpp:27-43; entry 0 is only read for r < 0.01 A and is
+    // 0 here).  Both are computed once per process (per coefficient).
+    static std::mutex mu;
+    static std::vector<double> epsTable;
+    static std::map<double, std::vector<double>> desolvTables;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        if (epsTable.empty()) {
+            epsTable.assign(kTableLen, 0.0);
+            for (int k = 0; k < kTableLen; ++k) {
+                double r = double(k) * (1 / 100.0);
+                double E = std::exp(-0.3153767175 * r);
+                epsTable[size_t(k)] = (332.0 + (332 * 7.7839) * E) / (78.4 + -66.57180475 * E);
+            }
         }
-    // desolvation Gaussian, sigma 3.6 A (FG/DesolvExpFunc.cpp:27-43); entry 0
-    // is only read for r < 0.01 A and is 0 here.
-    p.desolv.assign(kTableLen, 0.0);
-    const double sigma = 3.6, f = -1 / (2 * sigma * sigma);
-    for (int k = 1; k < kTableLen; ++k) {
-        double r = double(k) * (1 / 100.0);
-        p.desolv[size_t(k)] = std::exp((r * r) * f);
-        p.desolv[size_t(k)] *= lib.coeffDesolv;
+        std::vector<double> &dt = desolvTables[lib.coeffDesolv];
+        if (dt.empty()) {
+            dt.assign(kTableLen, 0.0);
+            const double sigma = 3.6, f = -1 / (2 * sigma * sigma);
+            for (int k = 1; k < kTableLen; ++k) {
+                double r = double(k) * (1 / 100.0);
+                dt[size_t(k)] = std::exp((r * r) * f);
+                dt[size_t(k)] *= lib.coeffDesolv;
+            }
+        }
+        if (p.distDepDiel) p.eps = epsTable;
+        else p.eps.assign(kTableLen, 0.0);
+        p.desolv = dt;
     }
     if (int rc = pairTables(p, err)) return rc;
     bondVectors(p, lib);

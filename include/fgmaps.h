@@ -188,6 +188,12 @@ int fg_floating_grid(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, fl
 /* stay in HBM between runs, for benchmarking and batching)         */
 /* ---------------------------------------------------------------- */
 int fg_job_create(fg_ctx *ctx, const fg_job_desc *desc, fg_job **out);
+/* n jobs at once: the host-side inputs (atom records, tiles, cells, tables)
+ * are built on up to `threads` host threads (<= 0: one per core, at most 16),
+ * then uploaded and sized on ctx's stream in list order.  On an error no job
+ * is kept (every out[i] is NULL) and the message names the first failing
+ * job.  No reference counterpart (one receptor per process there). */
+int fg_job_create_many(fg_ctx *ctx, const fg_job_desc *descs, int n, int threads, fg_job **out);
 void fg_job_destroy(fg_job *job);
 /* Enqueue the computation on the context's stream; returns immediately. */
 int fg_job_run(fg_ctx *ctx, fg_job *job, unsigned what, int precision);
@@ -276,6 +282,15 @@ typedef struct {
  * reference's exit code for the failure. */
 int fg_prep_load(const char *gpf_path, const fg_load_opts *opts, fg_prep **out);
 int fg_prep_exit_code(void);
+/* Prepares n GPFs on up to `threads` host threads (<= 0: one per core, at
+ * most 16) and stores out[i] for gpfs[i]; the files are read and written
+ * (.xyz) as fg_prep_load does, paths relative to the process's working
+ * directory.  On
+ * an error no prep is kept (every out[i] is NULL) and the status, message and
+ * exit code are those of the first failing GPF in list order.  No reference
+ * counterpart: the reference prepares one receptor per process
+ * (FG/main.cpp:161-193); this is the host side of batches of pockets. */
+int fg_prep_load_many(const char *const *gpf_paths, int n, const fg_load_opts *opts, int threads, fg_prep **out);
 void fg_prep_free(fg_prep *prep);
 /* Views into the prepared arrays (valid until fg_prep_free). */
 int fg_prep_desc(const fg_prep *prep, fg_job_desc *out);

@@ -1,11 +1,11 @@
 """Batched form and autotuner on an MI355X.
 
 A batch runs many jobs as one hipGraph of persistent kernels.  Each grid
-point's arithmetic depends on the tile size (where the two-float fold
-happens) but not on the brick shape, the points per lane or the order in
-which workgroups take bricks, so a batch must reproduce every job's own run
-bit for bit; against the FP64 oracle the compensated bound of the one-job
-tests applies."""
+point's arithmetic depends neither on the kernel shape (tile size, brick,
+points per lane) nor on the order in which workgroups take bricks, so a batch
+must reproduce every job's own run bit for bit; against the FP64 oracle the
+compensated bound of the one-job tests applies.  Jobs made together on host
+threads (fg_job_create_many) equal jobs made one by one."""
 import numpy as np
 import pytest
 
@@ -171,3 +171,23 @@ def test_shapes_and_autotuner(ctx, pockets):
     fg.batch_run(ctx, jobs)
     for i, j in enumerate(jobs):
         assert _same(j.fetch(), want[i]), i
+
+
+def test_create_jobs_equals_one_by_one(ctx, pockets):
+    """fg_job_create_many builds the host-side inputs on threads: its jobs
+    compute the same maps as jobs made one by one, and an invalid job in the
+    list fails the call without keeping any job."""
+    import dataclasses
+    import fastgrid_amd as fg
+    one = _jobs(ctx, pockets)
+    many = fg.create_jobs(ctx, pockets, threads=4)
+    want = _single(one, fg.PREC_COMPENSATED)
+    got = _single(many, fg.PREC_COMPENSATED)
+    for i in range(len(pockets)):
+        assert _same(got[i], want[i]), i
+    for j in one + many:
+        j.close()
+    bad = dataclasses.replace(pockets[0], n=(0, 21, 21))
+    with pytest.raises(fg.FgError) as ei:
+        fg.create_jobs(ctx, [pockets[1], bad, pockets[2]], threads=2)
+    assert "job 1" in str(ei.value) and "invalid grid" in str(ei.value)
