@@ -76,6 +76,19 @@ class FgSynthSpec(C.Structure):
                 ("stem", C.c_char_p)]
 
 
+class FgMapGrid(C.Structure):
+    """AutoDock's map-set geometry (GridMapSetInfo, fg_map_grid)."""
+    _fields_ = [("spacing", C.c_double), ("inv_spacing", C.c_double), ("num_points", C.c_int32 * 3),
+                ("num_points1", C.c_int32 * 3), ("lo", C.c_double * 3), ("hi", C.c_double * 3),
+                ("center", C.c_double * 3)]
+
+
+class FgLigand(C.Structure):
+    _fields_ = [("natoms", C.c_int32), ("type", C.POINTER(C.c_int32)), ("charge", C.POINTER(C.c_float)),
+                ("abs_charge", C.POINTER(C.c_float)), ("ignore", C.POINTER(C.c_int32)), ("elec_slot", C.c_int32),
+                ("desolv_slot", C.c_int32)]
+
+
 # Every symbol include/fgmaps.h declares, with its ctypes signature.
 SIGNATURES = {
     "fg_last_error": (C.c_char_p, []),
@@ -120,6 +133,17 @@ SIGNATURES = {
     "fg_prep_summary": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "fg_write_outputs": (C.c_int, [C.c_void_p, C.POINTER(_dp), C.c_char_p, C.c_int]),
     "fg_synth_write": (C.c_int, [C.POINTER(FgSynthSpec), C.c_char_p]),
+    "fg_read_fld": (C.c_int, [C.c_char_p, C.POINTER(FgMapGrid)]),
+    "fg_read_map": (C.c_int, [C.c_char_p, C.POINTER(FgMapGrid), C.c_int, C.c_int, C.POINTER(C.c_float)]),
+    "fg_prep_map_grid": (C.c_int, [C.c_void_p, C.POINTER(FgMapGrid)]),
+    "fg_mapset_create": (C.c_int, [C.c_void_p, C.POINTER(FgMapGrid), C.c_int, C.POINTER(C.c_float),
+                                   C.POINTER(C.c_void_p)]),
+    "fg_mapset_from_job": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FgMapGrid), C.POINTER(C.c_void_p)]),
+    "fg_mapset_fetch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
+    "fg_mapset_info": (C.c_int, [C.c_void_p, C.POINTER(FgMapGrid), C.POINTER(C.c_int)]),
+    "fg_mapset_destroy": (None, [C.c_void_p]),
+    "fg_trilinterp": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FgLigand), C.c_int, C.POINTER(C.c_float),
+                                C.POINTER(C.c_float), C.POINTER(C.c_float), _dp, _dp, C.POINTER(C.c_float)]),
 }
 
 
@@ -326,6 +350,12 @@ class Prepared:
             solpar_q=d.solpar_q, hydrogen=d.hydrogen, dist_dep_diel=d.dist_dep_diel,
             cov_on=d.cov_on, cov_point=tuple(d.cov_point), cov_hw_sq_inv=d.cov_hw_sq_inv, cov_barrier=d.cov_barrier,
             floating_grid=d.floating_grid)
+
+    def map_grid(self) -> FgMapGrid:
+        """The geometry AutoDock reads back from this prep's .fld."""
+        g = FgMapGrid()
+        _check(LIB.fg_prep_map_grid(self._h, C.byref(g)))
+        return g
 
     def write_outputs(self, maps: list[np.ndarray], program_name: str = "fastgrid4", threads: int = 0) -> None:
         keep = [np.ascontiguousarray(m, dtype=np.float64) for m in maps]
@@ -580,3 +610,94 @@ def synth_write(directory: str, stem: str, seed: int, n_atoms: int, shape: str, 
     s.stem = stem.encode()
     _check(LIB.fg_synth_write(C.byref(s), directory.encode()))
     return os.path.join(directory, stem + ".pdbqt"), os.path.join(directory, stem + ".gpf")
+
+
+# ---------------------------------------------------------------------------
+# The maps' consumer: AutoDock map sets and trilinear interpolation
+# ---------------------------------------------------------------------------
+def read_fld(path: str) -> FgMapGrid:
+    """AutoDock's readfield of a .fld (fg_read_fld)."""
+    g = FgMapGrid()
+    _check(LIB.fg_read_fld(path.encode(), C.byref(g)))
+    return g
+
+
+def read_mapset(g: FgMapGrid, paths: list[str]) -> np.ndarray:
+    """AutoDock's readmap of every file, slot i = paths[i]: floats [z][y][x][slot]."""
+    n = list(g.num_points1)
+    out = np.zeros((n[2], n[1], n[0], len(paths)), dtype=np.float32)
+    for slot, p in enumerate(paths):
+        _check(LIB.fg_read_map(p.encode(), C.byref(g), slot, len(paths), _ptr(out, C.c_float)))
+    return out
+
+
+class MapSet:
+    """A map set in HBM, floats [z][y][x][slot] (fg_mapset_*)."""
+
+    def __init__(self, ctx: Context, g: FgMapGrid | None = None, values: np.ndarray | None = None,
+                 job: "Job | None" = None):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        if job is not None:
+            _check(LIB.fg_mapset_from_job(ctx.handle, job.handle, C.byref(g), C.byref(self._h)))
+        else:
+            v = np.ascontiguousarray(values, dtype=np.float32)
+            self._keep = v
+            _check(LIB.fg_mapset_create(ctx.handle, C.byref(g), int(v.shape[-1]), _ptr(v, C.c_float),
+                                        C.byref(self._h)))
+
+    @classmethod
+    def from_job(cls, ctx: Context, job: "Job", g: FgMapGrid) -> "MapSet":
+        """The job's maps as AutoDock reads them from the printed files,
+        without text (affinity maps, then e, then d)."""
+        return cls(ctx, g, job=job)
+
+    def info(self) -> tuple[FgMapGrid, int]:
+        g, n = FgMapGrid(), C.c_int()
+        _check(LIB.fg_mapset_info(self._h, C.byref(g), C.byref(n)))
+        return g, int(n.value)
+
+    def fetch(self) -> np.ndarray:
+        g, n = self.info()
+        p = list(g.num_points1)
+        out = np.zeros((p[2], p[1], p[0], n), dtype=np.float32)
+        _check(LIB.fg_mapset_fetch(self.ctx.handle, self._h, _ptr(out, C.c_float)))
+        return out
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self) -> None:
+        if self._h:
+            LIB.fg_mapset_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def trilinterp(ctx: Context, ms: MapSet, xyz: np.ndarray, types, charge, abs_charge, elec_slot: int,
+               desolv_slot: int, ignore=None) -> dict:
+    """AutoDock's trilinterp of every pose of xyz [pose][atom][3] on the GPU
+    (fg_trilinterp).  Returns elec/emap per atom (float32), the two double
+    totals and the returned float per pose."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+    npose, natoms = xyz.shape[0], xyz.shape[1]
+    t = np.ascontiguousarray(types, dtype=np.int32)
+    q = np.ascontiguousarray(charge, dtype=np.float32)
+    aq = np.ascontiguousarray(abs_charge, dtype=np.float32)
+    ig = None if ignore is None else np.ascontiguousarray(ignore, dtype=np.int32)
+    lig = FgLigand(natoms, _ptr(t, C.c_int32), _ptr(q, C.c_float), _ptr(aq, C.c_float),
+                   _ptr(ig, C.c_int32) if ig is not None else C.cast(None, C.POINTER(C.c_int32)),
+                   int(elec_slot), int(desolv_slot))
+    out = {"elec": np.zeros((npose, natoms), np.float32), "emap": np.zeros((npose, natoms), np.float32),
+           "elec_total": np.zeros(npose), "emap_total": np.zeros(npose), "total": np.zeros(npose, np.float32)}
+    _check(LIB.fg_trilinterp(ctx.handle, ms.handle, C.byref(lig), npose, _ptr(xyz, C.c_float),
+                             _ptr(out["elec"], C.c_float), _ptr(out["emap"], C.c_float),
+                             _ptr(out["elec_total"], C.c_double), _ptr(out["emap_total"], C.c_double),
+                             _ptr(out["total"], C.c_float)))
+    return out

@@ -21,9 +21,9 @@ ARCH = os.environ.get("FG_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
-HOST_SOURCES = ["params.cpp", "loader.cpp", "writers.cpp", "synth.cpp", "capi_host.cpp"]
+HOST_SOURCES = ["params.cpp", "loader.cpp", "writers.cpp", "synth.cpp", "mapset.cpp", "capi_host.cpp"]
 DEVICE_SOURCES = ["capi_device.hip"]
-HEADERS = ["host.hpp", "capi_common.hpp", "kernels.hpp", "kernels.hip", "kernels_plain.hip"]
+HEADERS = ["host.hpp", "capi_common.hpp", "kernels.hpp", "kernels.hip", "kernels_plain.hip", "kernels_interp.hip"]
 
 CXXFLAGS = ["-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-I" + INCLUDE]
 HIPFLAGS = [

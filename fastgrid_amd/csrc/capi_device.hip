@@ -24,6 +24,7 @@
 #include "fgmaps.h"
 #include "kernels.hip"
 #include "kernels_plain.hip"
+#include "kernels_interp.hip"
 
 using fgk::CutAtom;
 
@@ -55,6 +56,15 @@ struct fg_ctx {
     // afterwards only frees their memory
     std::vector<fg_job *> jobs;
     std::vector<fg_batch *> batches;
+    std::vector<fg_mapset *> mapsets;
+};
+
+// An AutoDock map set in HBM: floats [z][y][x][slot].
+struct fg_mapset {
+    fg_ctx *ctx = nullptr;
+    fg_map_grid g{};
+    int nslots = 0;
+    float *d = nullptr;
 };
 
 struct fg_job {
@@ -1192,6 +1202,7 @@ void fg_destroy(fg_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (fg_job *j : ctx->jobs) j->ctx = nullptr;
     for (fg_batch *b : ctx->batches) b->ctx = nullptr;
+    for (fg_mapset *m : ctx->mapsets) m->ctx = nullptr;
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1721,6 +1732,201 @@ int fg_affinity_d_maps(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, 
     if (!rc) rc = fg_job_fetch(ctx, j, outs.data());
     fg_job_destroy(j);
     return rc;
+}
+
+// ------------------------------------------------------------------------
+// The maps' consumer: AutoDock map sets and trilinear interpolation
+// ------------------------------------------------------------------------
+namespace {
+
+int checkGrid(const fg_map_grid *g, const char *who)
+{
+    for (int k = 0; k < 3; ++k)
+        if (g->num_points1[k] <= 1 || g->num_points1[k] > 4097 || g->num_points1[k] != g->num_points[k] + 1)
+            return fgc::fail(FG_E_ARG, std::string(who) + ": invalid grid");
+    if (!(g->inv_spacing > 0)) return fgc::fail(FG_E_ARG, std::string(who) + ": invalid grid spacing");
+    return FG_OK;
+}
+
+int newMapset(fg_ctx *ctx, const fg_map_grid *g, int nslots, fg_mapset **out)
+{
+    fg_mapset *ms = new fg_mapset();
+    ms->ctx = ctx;
+    ms->g = *g;
+    ms->nslots = nslots;
+    const size_t n = size_t(g->num_points1[0]) * g->num_points1[1] * g->num_points1[2] * size_t(nslots);
+    if (hipMalloc(&ms->d, n * sizeof(float)) != hipSuccess) {
+        delete ms;
+        return fgc::fail(FG_E_DEVICE, "fg_mapset: out of device memory");
+    }
+    ctx->mapsets.push_back(ms);
+    *out = ms;
+    return FG_OK;
+}
+
+}  // namespace
+
+int fg_mapset_create(fg_ctx *ctx, const fg_map_grid *g, int nslots, const float *values, fg_mapset **out)
+{
+    if (!ctx || !g || !values || !out) return fgc::fail(FG_E_ARG, "fg_mapset_create: NULL argument");
+    *out = nullptr;
+    if (nslots <= 0 || nslots > 64) return fgc::fail(FG_E_ARG, "fg_mapset_create: nslots must be 1..64");
+    if (int rc = checkGrid(g, "fg_mapset_create")) return rc;
+    FG_HIP(hipSetDevice(ctx->device));
+    fg_mapset *ms = nullptr;
+    if (int rc = newMapset(ctx, g, nslots, &ms)) return rc;
+    const size_t n = size_t(g->num_points1[0]) * g->num_points1[1] * g->num_points1[2] * size_t(nslots);
+    if (hipMemcpy(ms->d, values, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        fg_mapset_destroy(ms);
+        return fgc::fail(FG_E_DEVICE, "fg_mapset_create: upload failed");
+    }
+    *out = ms;
+    return FG_OK;
+}
+
+int fg_mapset_from_job(fg_ctx *ctx, fg_job *j, const fg_map_grid *g, fg_mapset **out)
+{
+    if (!ctx || !j || !g || !out || j->ctx != ctx) return fgc::fail(FG_E_ARG, "fg_mapset_from_job: bad argument");
+    *out = nullptr;
+    if (int rc = checkGrid(g, "fg_mapset_from_job")) return rc;
+    if (g->num_points1[0] != j->g.nx || g->num_points1[1] != j->g.ny || g->num_points1[2] != j->g.nz)
+        return fgc::fail(FG_E_ARG, "fg_mapset_from_job: the grid's points do not match the job's");
+    FG_HIP(hipSetDevice(ctx->device));
+    fg_mapset *ms = nullptr;
+    const int V = j->M + 2;
+    if (int rc = newMapset(ctx, g, V, &ms)) return rc;
+    hipLaunchKernelGGL(fgk::k_export_mapset, dim3(unsigned((j->P + fgk::kInterpThreads - 1) / fgk::kInterpThreads)),
+                       dim3(fgk::kInterpThreads), 0, ctx->stream, j->dMaps, j->P, V, ms->d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        fg_mapset_destroy(ms);
+        return fgc::fail(FG_E_DEVICE, std::string("fg_mapset_from_job: ") + hipGetErrorString(e));
+    }
+    *out = ms;
+    return FG_OK;
+}
+
+int fg_mapset_fetch(fg_ctx *ctx, const fg_mapset *ms, float *values)
+{
+    if (!ctx || !ms || !values) return fgc::fail(FG_E_ARG, "fg_mapset_fetch: NULL argument");
+    if (ms->ctx != ctx) return fgc::fail(FG_E_ARG, "fg_mapset_fetch: map set does not belong to this context");
+    FG_HIP(hipSetDevice(ctx->device));
+    const size_t n = size_t(ms->g.num_points1[0]) * ms->g.num_points1[1] * ms->g.num_points1[2] * size_t(ms->nslots);
+    FG_HIP(hipMemcpyAsync(values, ms->d, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    FG_HIP(hipStreamSynchronize(ctx->stream));
+    return FG_OK;
+}
+
+int fg_mapset_info(const fg_mapset *ms, fg_map_grid *g, int *nslots)
+{
+    if (!ms) return fgc::fail(FG_E_ARG, "fg_mapset_info: NULL argument");
+    if (g) *g = ms->g;
+    if (nslots) *nslots = ms->nslots;
+    return FG_OK;
+}
+
+void fg_mapset_destroy(fg_mapset *ms)
+{
+    if (!ms) return;
+    if (ms->ctx) {
+        (void)hipStreamSynchronize(ms->ctx->stream);
+        forget(ms->ctx->mapsets, ms);
+    }
+    if (ms->d) (void)hipFree(ms->d);
+    delete ms;
+}
+
+int fg_trilinterp(fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int npose, const float *xyz, float *elec,
+                  float *emap, double *elecTotal, double *emapTotal, float *total)
+{
+    if (!ctx || !ms || !lig || (npose > 0 && !xyz)) return fgc::fail(FG_E_ARG, "fg_trilinterp: NULL argument");
+    if (ms->ctx != ctx) return fgc::fail(FG_E_ARG, "fg_trilinterp: map set does not belong to this context");
+    const int A = lig->natoms, V = ms->nslots;
+    if (A <= 0 || npose < 0 || !lig->type || !lig->charge || !lig->abs_charge)
+        return fgc::fail(FG_E_ARG, "fg_trilinterp: invalid ligand");
+    if (lig->elec_slot < 0 || lig->elec_slot >= V || lig->desolv_slot < 0 || lig->desolv_slot >= V)
+        return fgc::fail(FG_E_ARG, "fg_trilinterp: e/d slot out of range");
+    for (int i = 0; i < A; ++i)
+        if (lig->type[i] < 0 || lig->type[i] >= V) return fgc::fail(FG_E_ARG, "fg_trilinterp: atom type slot out of range");
+    if (npose == 0) return FG_OK;
+    const int64_t n = int64_t(npose) * A;
+    if (n > (int64_t(1) << 31)) return fgc::fail(FG_E_ARG, "fg_trilinterp: too many (pose, atom) pairs");
+    FG_HIP(hipSetDevice(ctx->device));
+    // one allocation: xyz, per-atom inputs, per-atom outputs, contributions, totals
+    auto align = [](size_t b) { return (b + 255) & ~size_t(255); };
+    size_t off = 0;
+    const size_t oXyz = off; off = align(off + size_t(n) * 3 * sizeof(float));
+    const size_t oType = off; off = align(off + size_t(A) * sizeof(int32_t));
+    const size_t oQ = off; off = align(off + size_t(A) * sizeof(float));
+    const size_t oAQ = off; off = align(off + size_t(A) * sizeof(float));
+    const size_t oIgn = off; off = align(off + size_t(A) * sizeof(int32_t));
+    const size_t oElec = off; off = align(off + size_t(n) * sizeof(float));
+    const size_t oEmap = off; off = align(off + size_t(n) * sizeof(float));
+    const size_t oCE = off; off = align(off + size_t(n) * sizeof(double));
+    const size_t oCM = off; off = align(off + size_t(n) * sizeof(double));
+    const size_t oET = off; off = align(off + size_t(npose) * sizeof(double));
+    const size_t oMT = off; off = align(off + size_t(npose) * sizeof(double));
+    const size_t oTot = off; off = align(off + size_t(npose) * sizeof(float));
+    char *b = nullptr;
+    FG_HIP(hipMalloc(&b, off));
+    hipStream_t s = ctx->stream;
+    auto up = [&](size_t o, const void *src, size_t bytes) { return hipMemcpyAsync(b + o, src, bytes, hipMemcpyHostToDevice, s); };
+    hipError_t e = up(oXyz, xyz, size_t(n) * 3 * sizeof(float));
+    if (e == hipSuccess) e = up(oType, lig->type, size_t(A) * sizeof(int32_t));
+    if (e == hipSuccess) e = up(oQ, lig->charge, size_t(A) * sizeof(float));
+    if (e == hipSuccess) e = up(oAQ, lig->abs_charge, size_t(A) * sizeof(float));
+    if (e == hipSuccess && lig->ignore) e = up(oIgn, lig->ignore, size_t(A) * sizeof(int32_t));
+    fgk::InterpArgs a{};
+    for (int k = 0; k < 3; ++k) {
+        a.lo[k] = ms->g.lo[k];
+        a.hi[k] = ms->g.hi[k];
+        a.center[k] = ms->g.center[k];
+    }
+    a.inv = ms->g.inv_spacing;
+    a.nx = ms->g.num_points1[0];
+    a.ny = ms->g.num_points1[1];
+    a.nz = ms->g.num_points1[2];
+    a.V = V;
+    a.natoms = A;
+    a.npose = npose;
+    a.elecSlot = lig->elec_slot;
+    a.desolvSlot = lig->desolv_slot;
+    a.map = ms->d;
+    a.xyz = reinterpret_cast<const float *>(b + oXyz);
+    a.type = reinterpret_cast<const int32_t *>(b + oType);
+    a.charge = reinterpret_cast<const float *>(b + oQ);
+    a.absq = reinterpret_cast<const float *>(b + oAQ);
+    a.ignore = lig->ignore ? reinterpret_cast<const int32_t *>(b + oIgn) : nullptr;
+    a.elec = elec ? reinterpret_cast<float *>(b + oElec) : nullptr;
+    a.emap = emap ? reinterpret_cast<float *>(b + oEmap) : nullptr;
+    a.ce = reinterpret_cast<double *>(b + oCE);
+    a.cm = reinterpret_cast<double *>(b + oCM);
+    double *dET = reinterpret_cast<double *>(b + oET), *dMT = reinterpret_cast<double *>(b + oMT);
+    float *dTot = reinterpret_cast<float *>(b + oTot);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev[0], s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(fgk::k_interp_atoms, dim3(unsigned((n + fgk::kInterpThreads - 1) / fgk::kInterpThreads)),
+                           dim3(fgk::kInterpThreads), 0, s, a);
+        hipLaunchKernelGGL(fgk::k_interp_totals, dim3(unsigned((npose + fgk::kInterpThreads - 1) / fgk::kInterpThreads)),
+                           dim3(fgk::kInterpThreads), 0, s, a.ce, a.cm, A, npose, dET, dMT, dTot);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev[2], s);
+    auto down = [&](void *dst, size_t o, size_t bytes) {
+        return dst ? hipMemcpyAsync(dst, b + o, bytes, hipMemcpyDeviceToHost, s) : hipSuccess;
+    };
+    if (e == hipSuccess) e = down(elec, oElec, size_t(n) * sizeof(float));
+    if (e == hipSuccess) e = down(emap, oEmap, size_t(n) * sizeof(float));
+    if (e == hipSuccess) e = down(elecTotal, oET, size_t(npose) * sizeof(double));
+    if (e == hipSuccess) e = down(emapTotal, oMT, size_t(npose) * sizeof(double));
+    if (e == hipSuccess) e = down(total, oTot, size_t(npose) * sizeof(float));
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fgc::fail(FG_E_DEVICE, std::string("fg_trilinterp: ") + hipGetErrorString(e));
+    ctx->haveTiming = true;
+    ctx->splitTiming = false;
+    return FG_OK;
 }
 
 }  // extern "C"

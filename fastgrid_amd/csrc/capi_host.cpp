@@ -2,6 +2,8 @@
 // writers and the synthetic generator.  No device code.
 #include <algorithm>
 #include <atomic>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -133,6 +135,29 @@ int fg_prep_summary(const fg_prep *prep, int m, double *mn, double *mx)
     if (m < 0 || size_t(m) >= prep->p.mapMin.size()) return fgc::fail(FG_E_ARG, "fg_prep_summary: no such map");
     *mn = prep->p.mapMin[size_t(m)];
     *mx = prep->p.mapMax[size_t(m)];
+    return FG_OK;
+}
+
+int fg_prep_map_grid(const fg_prep *prep, fg_map_grid *g)
+{
+    if (!prep || !g) return fgc::fail(FG_E_ARG, "fg_prep_map_grid: NULL argument");
+    const fgh::Prepared &p = prep->p;
+    // the .fld's "#SPACING %.3f" of the float spacing and "#CENTER %.3lf",
+    // read back with "%lf" (AD/readfield.cc:72-114)
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%.3f", double(float(p.spacing)));
+    g->spacing = std::strtod(buf, nullptr);
+    g->inv_spacing = 1. / g->spacing;
+    for (int k = 0; k < 3; ++k) {
+        std::snprintf(buf, sizeof buf, "%.3lf", p.center[k]);
+        g->center[k] = std::strtod(buf, nullptr);
+        g->num_points[k] = p.n[k] - 1;
+        g->num_points1[k] = p.n[k];
+        if (g->num_points[k] % 2 != 0 || g->num_points[k] <= 0)
+            return fgc::fail(FG_E_INPUT, "fg_prep_map_grid: AutoDock needs an even NELEMENTS on every axis");
+        g->lo[k] = g->center[k] - (g->num_points[k] / 2) * g->spacing;
+        g->hi[k] = g->center[k] + (g->num_points[k] / 2) * g->spacing;
+    }
     return FG_OK;
 }
 

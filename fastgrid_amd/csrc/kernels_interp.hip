@@ -1,0 +1,155 @@
+// The maps' consumer on gfx950: AutoDock 4.2.1's float map set and its
+// trilinear interpolation of ligand atoms (AD/trilinterp.cc:87-185, AD =
+// autogrid_original_4_2_1_ref/autodock in the reference).  Included by
+// capi_device.hip.
+//
+// The double arithmetic keeps the reference's operation order (no
+// contraction in this file), so every output equals the reference's bit for
+// bit.  Totals are the reference's running sums in atom order: one lane per
+// (pose, atom) computes the atom's two contributions, one lane per pose adds
+// them in order.
+#pragma clang fp contract(off)
+
+#include "kernels.hpp"
+
+namespace fgk {
+
+constexpr int kInterpThreads = 256;
+
+// The float AutoDock reads back from a printed map value: "0" below 0.0005
+// in magnitude, else "%.3f" of floor(1000 v + 0.5) * 0.001, which is the
+// decimal k / 1000 (k = floor(1000 v + 0.5)); read with "%f" that is the
+// float nearest to k / 1000, i.e. float(double(k) / 1000).
+__device__ __forceinline__ float autodockValue(double v)
+{
+    if (fabs(v) < 0.0005) return 0.f;
+    const double k = floor(v * 1000 + 0.5);
+    return float(k / 1000.0);
+}
+
+// maps [V][P] (x fastest, as a job holds them) -> out [P][V] (AutoDock's
+// [z][y][x][slot]).  One lane per grid point.
+__global__ __launch_bounds__(kInterpThreads) void k_export_mapset(const double *maps, int64_t P, int V, float *out)
+{
+    const int64_t p = int64_t(blockIdx.x) * kInterpThreads + threadIdx.x;
+    if (p >= P) return;
+    for (int m = 0; m < V; ++m) out[p * V + m] = autodockValue(maps[int64_t(m) * P + p]);
+}
+
+struct InterpArgs {
+    double lo[3], hi[3], center[3], inv;
+    int32_t nx, ny, nz, V;       // points per axis, slots
+    int32_t natoms, npose, elecSlot, desolvSlot;
+    const float *map;            // [nz][ny][nx][V]
+    const float *xyz;            // [npose][natoms][3]
+    const int32_t *type;         // [natoms]
+    const float *charge, *absq;  // [natoms]
+    const int32_t *ignore;       // [natoms] or null
+    float *elec, *emap;          // [npose][natoms] or null
+    double *ce, *cm;             // [npose][natoms]: what the atom adds to the two totals
+};
+
+// One lane per (pose, atom).
+__global__ __launch_bounds__(kInterpThreads) void k_interp_atoms(InterpArgs a)
+{
+    const int64_t idx = int64_t(blockIdx.x) * kInterpThreads + threadIdx.x;
+    if (idx >= int64_t(a.npose) * a.natoms) return;
+    const int i = int(idx % a.natoms);
+    double addE, addM;
+    if (a.ignore && a.ignore[i]) {
+        if (a.elec) a.elec[idx] = 0.f;
+        if (a.emap) a.emap[idx] = 0.f;
+        a.ce[idx] = 0.0;
+        a.cm[idx] = 0.0;
+        return;
+    }
+    const float fx = a.xyz[3 * idx], fy = a.xyz[3 * idx + 1], fz = a.xyz[3 * idx + 2];
+    double x = fx, y = fy, z = fz;
+    if (x <= a.lo[0] || x >= a.hi[0] || y <= a.lo[1] || y >= a.hi[1] || z <= a.lo[2] || z >= a.hi[2]) {
+        // outside the grid: 500 x the squared distance from the centre
+        x -= a.center[0];
+        y -= a.center[1];
+        z -= a.center[2];
+        const double ep = (x * x + y * y + z * z) * 500.0;
+        addE = addM = ep;
+        if (a.elec) a.elec[idx] = float(ep);
+        if (a.emap) a.emap[idx] = float(ep);
+    } else {
+        const int t = a.type[i];
+        const double u = (double(fx) - a.lo[0]) * a.inv;
+        const int u0 = int(u), u1 = u0 + 1;
+        const double p0u = u - double(u0), p1u = 1.0 - p0u;
+        const double v = (double(fy) - a.lo[1]) * a.inv;
+        const int v0 = int(v), v1 = v0 + 1;
+        const double p0v = v - double(v0), p1v = 1.0 - p0v;
+        const double w = (double(fz) - a.lo[2]) * a.inv;
+        const int w0 = int(w), w1 = w0 + 1;
+        const double p0w = w - double(w0), p1w = 1.0 - p0w;
+        // slot value at a corner; 0 one past the grid's last point
+        auto val = [&](int wz, int vy, int ux, int s) -> double {
+            if (ux >= a.nx || vy >= a.ny || wz >= a.nz) return 0.0;
+            return double(a.map[((int64_t(wz) * a.ny + vy) * a.nx + ux) * a.V + s]);
+        };
+        const int es = a.elecSlot, ds = a.desolvSlot;
+        double e = 0, m = 0, d = 0;
+        e += p1u * p1v * p1w * val(w0, v0, u0, es);
+        m += p1u * p1v * p1w * val(w0, v0, u0, t);
+        d += p1u * p1v * p1w * val(w0, v0, u0, ds);
+
+        d += p0u * p1v * p1w * val(w0, v0, u1, ds);
+        m += p0u * p1v * p1w * val(w0, v0, u1, t);
+        e += p0u * p1v * p1w * val(w0, v0, u1, es);
+
+        e += p1u * p0v * p1w * val(w0, v1, u0, es);
+        m += p1u * p0v * p1w * val(w0, v1, u0, t);
+        d += p1u * p0v * p1w * val(w0, v1, u0, ds);
+
+        d += p0u * p0v * p1w * val(w0, v1, u1, ds);
+        m += p0u * p0v * p1w * val(w0, v1, u1, t);
+        e += p0u * p0v * p1w * val(w0, v1, u1, es);
+
+        e += p1u * p1v * p0w * val(w1, v0, u0, es);
+        m += p1u * p1v * p0w * val(w1, v0, u0, t);
+        d += p1u * p1v * p0w * val(w1, v0, u0, ds);
+
+        d += p0u * p1v * p0w * val(w1, v0, u1, ds);
+        m += p0u * p1v * p0w * val(w1, v0, u1, t);
+        e += p0u * p1v * p0w * val(w1, v0, u1, es);
+
+        e += p1u * p0v * p0w * val(w1, v1, u0, es);
+        m += p1u * p0v * p0w * val(w1, v1, u0, t);
+        d += p1u * p0v * p0w * val(w1, v1, u0, ds);
+
+        d += p0u * p0v * p0w * val(w1, v1, u1, ds);
+        m += p0u * p0v * p0w * val(w1, v1, u1, t);
+        e += p0u * p0v * p0w * val(w1, v1, u1, es);
+
+        addE = e * double(a.charge[i]);
+        addM = m + d * double(a.absq[i]);
+        if (a.elec) a.elec[idx] = float(addE);
+        if (a.emap) a.emap[idx] = float(addM);
+    }
+    a.ce[idx] = addE;
+    a.cm[idx] = addM;
+}
+
+// One lane per pose: the reference's running sums in atom order and its
+// returned (Real)elec_total + emap_total.
+__global__ __launch_bounds__(kInterpThreads) void k_interp_totals(const double *ce, const double *cm, int natoms,
+                                                                  int npose, double *elecTotal, double *emapTotal,
+                                                                  float *total)
+{
+    const int p = int(blockIdx.x) * kInterpThreads + threadIdx.x;
+    if (p >= npose) return;
+    double et = 0, mt = 0;
+    const int64_t base = int64_t(p) * natoms;
+    for (int i = 0; i < natoms; ++i) {
+        et += ce[base + i];
+        mt += cm[base + i];
+    }
+    if (elecTotal) elecTotal[p] = et;
+    if (emapTotal) emapTotal[p] = mt;
+    if (total) total[p] = float(double(float(et)) + mt);
+}
+
+}  // namespace fgk

@@ -285,9 +285,9 @@ int fg_prep_exit_code(void);
 /* Prepares n GPFs on up to `threads` host threads (<= 0: one per core, at
  * most 16) and stores out[i] for gpfs[i]; the files are read and written
  * (.xyz) as fg_prep_load does, paths relative to the process's working
- * directory.  On
- * an error no prep is kept (every out[i] is NULL) and the status, message and
- * exit code are those of the first failing GPF in list order.  No reference
+ * directory.  On an error no prep is kept (every out[i] is NULL) and the
+ * status, message and exit code are those of the first failing GPF in list
+ * order.  No reference
  * counterpart: the reference prepares one receptor per process
  * (FG/main.cpp:161-193); this is the host side of batches of pockets. */
 int fg_prep_load_many(const char *const *gpf_paths, int n, const fg_load_opts *opts, int threads, fg_prep **out);
@@ -311,6 +311,71 @@ int fg_prep_summary(const fg_prep *prep, int m, double *min, double *max);
  * with "%.3f" as the reference prints its float values).  Also
  * records each map's min/max in the prep for fg_prep_summary. */
 int fg_write_outputs(fg_prep *prep, double *const *maps, const char *program_name, int threads);
+
+/* ---------------------------------------------------------------- */
+/* The maps' consumer: AutoDock 4.2.1's map set and its trilinear    */
+/* interpolation of ligand atoms (AD = autogrid_original_4_2_1_ref/  */
+/* autodock in the reference)                                       */
+/* ---------------------------------------------------------------- */
+/* The geometry of GridMapSetInfo (AD/grid.h:36-54) as readfield leaves it:
+ * lo = center - (num_points / 2) * spacing, hi likewise. */
+typedef struct {
+    double spacing, inv_spacing;
+    int32_t num_points[3];   /* NELEMENTS: points per axis minus 1 (even) */
+    int32_t num_points1[3];  /* points per axis */
+    double lo[3], hi[3], center[3];
+} fg_map_grid;
+
+/* readfield (AD/readfield.cc:55-140): #SPACING, #NELEMENTS (even, > 0),
+ * #CENTER. */
+int fg_read_fld(const char *fld_path, fg_map_grid *out);
+/* readmap (AD/readmap.cc:110-300) of one .map file into slot `slot` of a
+ * host map set values[((z * ny1 + y) * nx1 + x) * nslots + slot]: SPACING,
+ * NELEMENTS and CENTER must match g (compared as floats, as the reference
+ * compares them), then one value per line as sscanf "%f" reads it; fails
+ * unless every grid point got a value. */
+int fg_read_map(const char *map_path, const fg_map_grid *g, int slot, int nslots, float *values);
+/* The geometry AutoDock reads back from the .fld fg_write_outputs writes for
+ * this prep (its printed spacing and center, parsed as readfield parses
+ * them). */
+int fg_prep_map_grid(const fg_prep *prep, fg_map_grid *out);
+
+/* A map set in HBM: floats [z][y][x][slot], AutoDock's layout
+ * (AD/map_declare.h) without its padding. */
+typedef struct fg_mapset fg_mapset;
+int fg_mapset_create(fg_ctx *ctx, const fg_map_grid *g, int nslots, const float *values, fg_mapset **out);
+/* Binary hand-off of a job's maps with no text in between: slots 0..n-1 are
+ * the affinity maps in the job's order, then e and d (n_maps + 2 slots), and
+ * every value is the float AutoDock reads from the printed map (0 below
+ * 0.0005 in magnitude, else floor(1000 v + 0.5) / 1000).  g is the grid the
+ * maps' .fld describes (fg_prep_map_grid); its points must match the job's. */
+int fg_mapset_from_job(fg_ctx *ctx, fg_job *job, const fg_map_grid *g, fg_mapset **out);
+int fg_mapset_fetch(fg_ctx *ctx, const fg_mapset *ms, float *values);
+int fg_mapset_info(const fg_mapset *ms, fg_map_grid *g, int *nslots);
+void fg_mapset_destroy(fg_mapset *ms);
+
+/* One ligand's atoms as trilinterp reads them (AD/trilinterp.cc:46-66). */
+typedef struct {
+    int32_t natoms;
+    const int32_t *type;        /* map slot of each atom's affinity map */
+    const float *charge;        /* partial charges (Real) */
+    const float *abs_charge;    /* their magnitudes */
+    const int32_t *ignore;      /* NULL, or nonzero to skip an atom (ignore_inter) */
+    int32_t elec_slot;          /* slot of the e map (the reference's ElecMap) */
+    int32_t desolv_slot;        /* slot of the d map (DesolvMap) */
+} fg_ligand;
+
+/* trilinterp (AD/trilinterp.cc:87-185) of npose poses of the ligand at once:
+ * xyz[pose][atom][3] (floats, host).  Per pose and atom elec/emap as the
+ * reference stores them (floats), per pose the double totals (the
+ * reference's running sums, in atom order) and the returned value
+ * (float)((float)elec_total + emap_total).  Any output may be NULL.  Atoms
+ * outside the grid get 500 x their squared distance from the centre (the
+ * reference's some_atoms_outside_grid path, taken for every atom here); a
+ * corner one past the grid's last point reads 0, as AutoDock's zero-padded
+ * array does.  fg_last_timing() reports the device time (total_ms). */
+int fg_trilinterp(fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int npose, const float *xyz, float *elec,
+                  float *emap, double *elec_total, double *emap_total, float *total);
 
 /* ---------------------------------------------------------------- */
 /* Synthetic receptors (seeded; deterministic across machines)      */

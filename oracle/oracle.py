@@ -26,6 +26,13 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+class OiGrid(C.Structure):
+    """AutoDock's GridMapSetInfo geometry (oracle/fginterp.h)."""
+    _fields_ = [("spacing", C.c_double), ("inv_spacing", C.c_double), ("num_points", C.c_int * 3),
+                ("num_points1", C.c_int * 3), ("lo", C.c_double * 3), ("hi", C.c_double * 3),
+                ("center", C.c_double * 3)]
+
+
 def _load() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         build()
@@ -56,6 +63,11 @@ def _load() -> C.CDLL:
     lib.or_covalent_map.argtypes = [_ip, _ip, C.c_double, _dp, C.c_double, C.c_double, _dp]
     lib.or_ddd_table.argtypes = [_dp]
     lib.or_desolv_table.argtypes = [C.c_double, _dp]
+    _fp = C.POINTER(C.c_float)
+    lib.oi_read_fld.argtypes = [C.c_char_p, C.POINTER(OiGrid)]
+    lib.oi_read_map.argtypes = [C.c_char_p, C.POINTER(OiGrid), C.c_int, C.c_int, _fp]
+    lib.oi_trilinterp_poses.argtypes = [C.POINTER(OiGrid), _fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _fp,
+                                        _fp, _fp, _ip, C.c_int, _ip, _fp, _fp, _dp, _dp, _fp, C.c_int]
     return lib
 
 
@@ -203,3 +215,47 @@ def run_cli(gpf: str, cwd: str, v4: bool = False, ag421_param_order: bool = Fals
     if threads:
         cmd += ["--omp", str(threads)]
     return subprocess.run(cmd, cwd=cwd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE).returncode
+
+
+def read_fld(path: str) -> OiGrid:
+    """AutoDock's readfield (autodock/readfield.cc:55-140)."""
+    g = OiGrid()
+    rc = LIB.oi_read_fld(path.encode(), C.byref(g))
+    if rc:
+        raise RuntimeError(f"oi_read_fld({path}) = {rc}")
+    return g
+
+
+def read_mapset(g: OiGrid, paths: list) -> np.ndarray:
+    """AutoDock's readmap of every file into one [z][y][x][slot] float array."""
+    n = list(g.num_points1)
+    out = np.zeros((n[2], n[1], n[0], len(paths)), dtype=np.float32)
+    for slot, p in enumerate(paths):
+        rc = LIB.oi_read_map(p.encode(), C.byref(g), slot, len(paths), _p(out, C.c_float))
+        if rc:
+            raise RuntimeError(f"oi_read_map({p}) = {rc}")
+    return out
+
+
+def trilinterp(g: OiGrid, maps: np.ndarray, elec_slot: int, desolv_slot: int, xyz: np.ndarray, charge, abs_charge,
+               types, ignore=None, outside: bool = True, threads: int = 0):
+    """AutoDock's trilinterp (autodock/trilinterp.cc:46-185) for every pose of
+    xyz [pose][atom][3] (float32).  Returns (elec, emap) per atom (float32),
+    (elec_total, emap_total) per pose (float64) and the returned Real."""
+    maps = np.ascontiguousarray(maps, dtype=np.float32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+    npose, natoms = xyz.shape[0], xyz.shape[1]
+    q = np.ascontiguousarray(charge, dtype=np.float32)
+    aq = np.ascontiguousarray(abs_charge, dtype=np.float32)
+    t = np.ascontiguousarray(types, dtype=np.int32)
+    ig = None if ignore is None else np.ascontiguousarray(ignore, dtype=np.int32)
+    elec = np.zeros((npose, natoms), np.float32)
+    emap = np.zeros((npose, natoms), np.float32)
+    et = np.zeros(npose)
+    mt = np.zeros(npose)
+    tot = np.zeros(npose, np.float32)
+    LIB.oi_trilinterp_poses(C.byref(g), _p(maps, C.c_float), maps.shape[3], elec_slot, desolv_slot, natoms, npose,
+                            _p(xyz, C.c_float), _p(q, C.c_float), _p(aq, C.c_float), _p(t, C.c_int), int(outside),
+                            _p(ig, C.c_int) if ig is not None else None, _p(elec, C.c_float), _p(emap, C.c_float),
+                            _p(et), _p(mt), _p(tot, C.c_float), threads or (os.cpu_count() or 1))
+    return elec, emap, et, mt, tot

@@ -6,6 +6,10 @@
 #   make -f oracle/ref.mk -j8          (from the repository root)
 #
 # Sources: fastgrid_master/fastgrid/CMakeLists.txt:7-46 (the non-CUDA list).
+# Also AutoDock 4.2.1's trilinear interpolation (AD/trilinterp.cc, AD =
+# autogrid_original_4_2_1_ref/autodock) with the driver
+# oracle/ref_trilinterp_main.cpp, compiled as AutoDock's default build does
+# (Real = float, no FMA contraction on x86-64).
 # AG_CUDA is left undefined so electrostatics/Cpu.cpp:114-127 provides the
 # e-map backend.  The only portability fix is a forced <stdint.h>/<cassert>
 # include (math/Int.h uses int32_t without including it); no source is edited
@@ -33,7 +37,9 @@ SRCS := ann/src/perf.cpp ann/src/kd_dump.cpp ann/src/bd_tree.cpp ann/src/ANN.cpp
 
 OBJS := $(addprefix $(OBJ)/,$(SRCS:.cpp=.o))
 
-all: $(OUT)/fastgrid4_ref $(OUT)/testdiff
+AD       := $(REF)/autogrid_original_4_2_1_ref/autodock
+
+all: $(OUT)/fastgrid4_ref $(OUT)/testdiff $(OUT)/ref_trilinterp
 
 $(OBJ)/%.o: $(FG)/%.cpp
 	@mkdir -p $(dir $@)
@@ -45,5 +51,11 @@ $(OUT)/fastgrid4_ref: $(OBJS)
 $(OUT)/testdiff: $(REF)/testdiff_master/main.cpp
 	@mkdir -p $(OUT)
 	$(CXX) -O2 -w -include cstdlib -include cstring -include climits -o $@ $<
+
+$(OUT)/ref_trilinterp: $(AD)/trilinterp.cc oracle/ref_trilinterp_main.cpp
+	@mkdir -p $(OBJ)
+	$(CXX) -std=gnu++98 -O2 -w -I$(AD) -c $(AD)/trilinterp.cc -o $(OBJ)/ad_trilinterp.o
+	$(CXX) -std=gnu++98 -O2 -w -I$(AD) -c oracle/ref_trilinterp_main.cpp -o $(OBJ)/ref_trilinterp_main.o
+	$(CXX) -o $@ $(OBJ)/ad_trilinterp.o $(OBJ)/ref_trilinterp_main.o -lm
 
 .PHONY: all
