@@ -22,9 +22,12 @@ with r <= 8 A.  Reported beside it:
   * the batch: 256 synthetic pockets (atoms log-uniform in 600-12000, odd
     grid edges uniform in 21-101 per axis, 6 affinity maps) as one hipGraph of
     persistent kernels, against 256 sequential plain launches;
+  * AutoDock's consumer of the maps: trilinear interpolation of 2^19
+    synthetic 32-atom ligand poses in the L maps handed off in HBM
+    (trilinterp), checked bit for bit against the oracle on a sample;
   * the FP64 CPU oracle timed on a bounded sample of L (cpu_baseline).
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-batch] [--stress]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-batch] [--no-interp] [--stress]
 """
 from __future__ import annotations
 
@@ -50,6 +53,9 @@ WORKLOAD_L = dict(tag="L", seed=3, n_atoms=50000, shape="box", dims=(70.0, 70.0,
 WORKLOAD_X = dict(tag="X", seed=5, n_atoms=250000, shape="box", dims=(130.0, 130.0, 90.0), npts=(161, 161, 161),
                   ligand_types="A C HD N NA OA SA S")
 BATCH = dict(pockets=256, seed=7, atoms=(600, 12000), edges=(21, 101), ligand_types="A C HD N OA SA", density=0.03)
+# AutoDock's consumer of the maps: trilinear interpolation of ligand poses
+INTERP = dict(poses=1 << 19, atoms=32, seed=11, site=20.0, spread=5.0, check_poses=4096, cpu_poses=1 << 17)
+BYTES_PER_INTERP_ATOM = (24 + 3) * 4   # 8 corners x 3 map floats read and xyz read per atom
 # HBM bytes per launch of the electrostatic kernel on L, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE scaled by a device copy of known size
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_elec_L_hbm_traffic.json")
@@ -300,7 +306,77 @@ def run_batch(fg, ctx, steps, warmup, n_pockets):
     }
 
 
-def run_workload(fg, ctx, w, steps, warmup, tune=True):
+def run_interp(fg, ctx, job, prep, steps, warmup, cpu):
+    """AutoDock's trilinear interpolation (autodock/trilinterp.cc) of 2^19
+    synthetic 32-atom ligand poses in the job's maps, handed off in HBM as
+    AutoDock reads them from the printed files (no text).  Poses as a docking
+    search makes them: centres uniform in a 20 A box at the grid centre, atoms
+    within +-5 A of the centre.  Reports atom evaluations/s of the device
+    kernels computing the per-pose totals (a scoring loop's output), a
+    bit-for-bit check of every output on a sample against the oracle, and
+    the oracle on the host cores as the CPU baseline."""
+    from oracle import oracle as orc
+    g = prep.map_grid()
+    ms = fg.MapSet.from_job(ctx, job, g)
+    M = job.arrays.n_maps
+    n, A = INTERP["poses"], INTERP["atoms"]
+    rng = np.random.default_rng(INTERP["seed"])
+    types = rng.integers(0, M, A).astype(np.int32)
+    q = rng.uniform(-0.7, 0.7, A).astype(np.float32)
+    aq = np.abs(q)
+    centre = np.array(list(g.center))
+    site, spread = INTERP["site"], INTERP["spread"]
+    centres = centre + rng.uniform(-site / 2, site / 2, (n, 1, 3))
+    xyz = (centres + rng.uniform(-spread, spread, (n, A, 3))).astype(np.float32)
+    for _ in range(warmup):
+        fg.trilinterp(ctx, ms, xyz, types, q, aq, M, M + 1, per_atom=False)
+    kms, walls = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        fg.trilinterp(ctx, ms, xyz, types, q, aq, M, M + 1, per_atom=False)
+        walls.append(time.perf_counter() - t0)
+        kms.append(ctx.timing().total_ms)
+    r = fg.trilinterp(ctx, ms, xyz[:INTERP["check_poses"]], types, q, aq, M, M + 1)
+    k = float(np.mean(kms))
+    evals = n * A
+    # bit-for-bit check of the first poses against the oracle on the same maps
+    c = INTERP["check_poses"]
+    og = orc.OiGrid()
+    for f in ("spacing", "inv_spacing"):
+        setattr(og, f, getattr(g, f))
+    for f in ("num_points", "num_points1", "lo", "hi", "center"):
+        for i in range(3):
+            getattr(og, f)[i] = getattr(g, f)[i]
+    maps = ms.fetch()
+    oe, om, oet, omt, otot = orc.trilinterp(og, maps, M, M + 1, xyz[:c], q, aq, types)
+    exact = bool(np.array_equal(r["elec"][:c], oe) and np.array_equal(r["emap"][:c], om)
+                 and np.array_equal(r["elec_total"][:c], oet) and np.array_equal(r["emap_total"][:c], omt)
+                 and np.array_equal(r["total"][:c], otot))
+    out = {"workload": f"{n} poses x {A} atoms (random types of the {M} affinity maps, charges in +-0.7), pose "
+                       f"centres uniform in a {site:g} A box at the grid centre, atoms within +-{spread:g} A, in "
+                       f"the L map set ({M} affinity maps + e + d, {g.num_points1[0]}x{g.num_points1[1]}x"
+                       f"{g.num_points1[2]}) handed off from the job in HBM; per-pose totals",
+           "atom_evals_per_launch": evals, "kernel_ms": round(k, 4),
+           "gevals_per_s": round(evals / (k * 1e-3) / 1e9, 2),
+           "algorithmic_gbs": round(evals * BYTES_PER_INTERP_ATOM / (k * 1e-3) / 1e9, 1),
+           "seconds_per_call_with_pcie": round(float(np.mean(walls)), 4),
+           "bit_exact_vs_oracle_poses": c if exact else 0}
+    log(f"trilinterp: {out}")
+    if not exact:
+        raise SystemExit("trilinterp differs from the oracle")
+    if cpu:
+        s = INTERP["cpu_poses"]
+        threads = os.cpu_count() or 1
+        t0 = time.perf_counter()
+        orc.trilinterp(og, maps, M, M + 1, xyz[:s], q, aq, types, threads=threads)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(s * A / dt / 1e9, 4), "unit": "Gatom-evals/s", "cores": threads,
+                               "kind": "port", "sample": f"oracle restatement of trilinterp, first {s} poses"}
+    ms.close()
+    return out
+
+
+def run_workload(fg, ctx, w, steps, warmup, tune=True, interp=False, cpu=True):
     """One large job: FP64 form (error reference), plain, FP32 and
     compensated forms.  Returns a dict of results and the job arrays."""
     with tempfile.TemporaryDirectory() as wd:
@@ -326,6 +402,8 @@ def run_workload(fg, ctx, w, steps, warmup, tune=True):
         t, el, af = time_steps(fg, ctx, job, steps, warmup, prec)
         res[key] = {"t": t, "elec_ms": el, "cutoff_ms": af, "errors": map_errors(names, job.fetch(), ref)}
         log(f"{key}: {t * 1e3:.2f} ms/step (elec {el:.2f}, cutoff+lists {af:.2f})")
+    if interp:
+        res["trilinterp"] = run_interp(fg, ctx, job, prep, steps, warmup, cpu)
     job.close()
     return res, a
 
@@ -338,6 +416,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-pocket batch")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end command-line timing")
+    ap.add_argument("--no-interp", action="store_true", help="skip the trilinear-interpolation line")
     ap.add_argument("--pockets", type=int, default=BATCH["pockets"])
     ap.add_argument("--stress", action="store_true", help="also run the 250k-atom / 161^3 stress size")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -353,7 +432,7 @@ def main():
 
     import fastgrid_amd as fg
     ctx = fg.Context(0)
-    L, a = run_workload(fg, ctx, WORKLOAD_L, args.steps, args.warmup)
+    L, a = run_workload(fg, ctx, WORKLOAD_L, args.steps, args.warmup, interp=not args.no_interp, cpu=not args.no_cpu)
     comp, plain, fp32 = L["compensated"], L["plain"], L["fp32"]
     tot_pairs = L["elec_pairs"] + L["cutoff_pairs"]
     value = tot_pairs / comp["t"] / 1e9
@@ -403,6 +482,8 @@ def main():
                      "hbm_gbs": round(tr["bytes_per_launch"] / (comp["elec_ms"] * 1e-3) / 1e9, 1) if tr else None,
                      "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if tr else None},
     }
+    if "trilinterp" in L:
+        line["trilinterp"] = L["trilinterp"]
     if not args.no_e2e:
         with tempfile.TemporaryDirectory() as wd:
             w = WORKLOAD_L

@@ -681,10 +681,10 @@ class MapSet:
 
 
 def trilinterp(ctx: Context, ms: MapSet, xyz: np.ndarray, types, charge, abs_charge, elec_slot: int,
-               desolv_slot: int, ignore=None) -> dict:
+               desolv_slot: int, ignore=None, per_atom: bool = True) -> dict:
     """AutoDock's trilinterp of every pose of xyz [pose][atom][3] on the GPU
-    (fg_trilinterp).  Returns elec/emap per atom (float32), the two double
-    totals and the returned float per pose."""
+    (fg_trilinterp).  Returns elec/emap per atom (float32; None unless
+    per_atom), the two double totals and the returned float per pose."""
     xyz = np.ascontiguousarray(xyz, dtype=np.float32)
     npose, natoms = xyz.shape[0], xyz.shape[1]
     t = np.ascontiguousarray(types, dtype=np.int32)
@@ -694,7 +694,8 @@ def trilinterp(ctx: Context, ms: MapSet, xyz: np.ndarray, types, charge, abs_cha
     lig = FgLigand(natoms, _ptr(t, C.c_int32), _ptr(q, C.c_float), _ptr(aq, C.c_float),
                    _ptr(ig, C.c_int32) if ig is not None else C.cast(None, C.POINTER(C.c_int32)),
                    int(elec_slot), int(desolv_slot))
-    out = {"elec": np.zeros((npose, natoms), np.float32), "emap": np.zeros((npose, natoms), np.float32),
+    out = {"elec": np.zeros((npose, natoms), np.float32) if per_atom else None,
+           "emap": np.zeros((npose, natoms), np.float32) if per_atom else None,
            "elec_total": np.zeros(npose), "emap_total": np.zeros(npose), "total": np.zeros(npose, np.float32)}
     _check(LIB.fg_trilinterp(ctx.handle, ms.handle, C.byref(lig), npose, _ptr(xyz, C.c_float),
                              _ptr(out["elec"], C.c_float), _ptr(out["emap"], C.c_float),

@@ -1853,7 +1853,7 @@ int fg_trilinterp(fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int np
     const int64_t n = int64_t(npose) * A;
     if (n > (int64_t(1) << 31)) return fgc::fail(FG_E_ARG, "fg_trilinterp: too many (pose, atom) pairs");
     FG_HIP(hipSetDevice(ctx->device));
-    // one allocation: xyz, per-atom inputs, per-atom outputs, contributions, totals
+    // one allocation: xyz, per-atom inputs, per-atom outputs, totals
     auto align = [](size_t b) { return (b + 255) & ~size_t(255); };
     size_t off = 0;
     const size_t oXyz = off; off = align(off + size_t(n) * 3 * sizeof(float));
@@ -1861,10 +1861,8 @@ int fg_trilinterp(fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int np
     const size_t oQ = off; off = align(off + size_t(A) * sizeof(float));
     const size_t oAQ = off; off = align(off + size_t(A) * sizeof(float));
     const size_t oIgn = off; off = align(off + size_t(A) * sizeof(int32_t));
-    const size_t oElec = off; off = align(off + size_t(n) * sizeof(float));
-    const size_t oEmap = off; off = align(off + size_t(n) * sizeof(float));
-    const size_t oCE = off; off = align(off + size_t(n) * sizeof(double));
-    const size_t oCM = off; off = align(off + size_t(n) * sizeof(double));
+    const size_t oElec = off; off = align(off + (elec ? size_t(n) * sizeof(float) : 0));
+    const size_t oEmap = off; off = align(off + (emap ? size_t(n) * sizeof(float) : 0));
     const size_t oET = off; off = align(off + size_t(npose) * sizeof(double));
     const size_t oMT = off; off = align(off + size_t(npose) * sizeof(double));
     const size_t oTot = off; off = align(off + size_t(npose) * sizeof(float));
@@ -1900,16 +1898,18 @@ int fg_trilinterp(fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int np
     a.ignore = lig->ignore ? reinterpret_cast<const int32_t *>(b + oIgn) : nullptr;
     a.elec = elec ? reinterpret_cast<float *>(b + oElec) : nullptr;
     a.emap = emap ? reinterpret_cast<float *>(b + oEmap) : nullptr;
-    a.ce = reinterpret_cast<double *>(b + oCE);
-    a.cm = reinterpret_cast<double *>(b + oCM);
-    double *dET = reinterpret_cast<double *>(b + oET), *dMT = reinterpret_cast<double *>(b + oMT);
-    float *dTot = reinterpret_cast<float *>(b + oTot);
+    a.elecTotal = reinterpret_cast<double *>(b + oET);
+    a.emapTotal = reinterpret_cast<double *>(b + oMT);
+    a.total = reinterpret_cast<float *>(b + oTot);
     if (e == hipSuccess) e = hipEventRecord(ctx->ev[0], s);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(fgk::k_interp_atoms, dim3(unsigned((n + fgk::kInterpThreads - 1) / fgk::kInterpThreads)),
-                           dim3(fgk::kInterpThreads), 0, s, a);
-        hipLaunchKernelGGL(fgk::k_interp_totals, dim3(unsigned((npose + fgk::kInterpThreads - 1) / fgk::kInterpThreads)),
-                           dim3(fgk::kInterpThreads), 0, s, a.ce, a.cm, A, npose, dET, dMT, dTot);
+        if (A <= fgk::kInterpThreads) {
+            const int ppw = fgk::kInterpThreads / A;
+            hipLaunchKernelGGL(fgk::k_interp_poses, dim3(unsigned((npose + ppw - 1) / ppw)), dim3(fgk::kInterpThreads),
+                               0, s, a);
+        } else {
+            hipLaunchKernelGGL(fgk::k_interp_big_poses, dim3(unsigned(npose)), dim3(fgk::kInterpThreads), 0, s, a);
+        }
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(ctx->ev[2], s);

@@ -6,8 +6,8 @@
 // The double arithmetic keeps the reference's operation order (no
 // contraction in this file), so every output equals the reference's bit for
 // bit.  Totals are the reference's running sums in atom order: one lane per
-// (pose, atom) computes the atom's two contributions, one lane per pose adds
-// them in order.
+// (pose, atom) computes the atom's two contributions into LDS, one lane per
+// pose adds them in order.
 #pragma clang fp contract(off)
 
 #include "kernels.hpp"
@@ -46,21 +46,18 @@ struct InterpArgs {
     const float *charge, *absq;  // [natoms]
     const int32_t *ignore;       // [natoms] or null
     float *elec, *emap;          // [npose][natoms] or null
-    double *ce, *cm;             // [npose][natoms]: what the atom adds to the two totals
+    double *elecTotal, *emapTotal;  // [npose] or null
+    float *total;                // [npose] or null
 };
 
-// One lane per (pose, atom).
-__global__ __launch_bounds__(kInterpThreads) void k_interp_atoms(InterpArgs a)
+// Atom i of pose `pose` (idx = pose * natoms + i): what it adds to the two
+// totals (0 for an ignored atom), and its per-atom outputs.
+__device__ __forceinline__ void interpAtom(const InterpArgs &a, int64_t idx, int i, double &addE, double &addM)
 {
-    const int64_t idx = int64_t(blockIdx.x) * kInterpThreads + threadIdx.x;
-    if (idx >= int64_t(a.npose) * a.natoms) return;
-    const int i = int(idx % a.natoms);
-    double addE, addM;
     if (a.ignore && a.ignore[i]) {
         if (a.elec) a.elec[idx] = 0.f;
         if (a.emap) a.emap[idx] = 0.f;
-        a.ce[idx] = 0.0;
-        a.cm[idx] = 0.0;
+        addE = addM = 0.0;
         return;
     }
     const float fx = a.xyz[3 * idx], fy = a.xyz[3 * idx + 1], fz = a.xyz[3 * idx + 2];
@@ -74,82 +71,112 @@ __global__ __launch_bounds__(kInterpThreads) void k_interp_atoms(InterpArgs a)
         addE = addM = ep;
         if (a.elec) a.elec[idx] = float(ep);
         if (a.emap) a.emap[idx] = float(ep);
-    } else {
-        const int t = a.type[i];
-        const double u = (double(fx) - a.lo[0]) * a.inv;
-        const int u0 = int(u), u1 = u0 + 1;
-        const double p0u = u - double(u0), p1u = 1.0 - p0u;
-        const double v = (double(fy) - a.lo[1]) * a.inv;
-        const int v0 = int(v), v1 = v0 + 1;
-        const double p0v = v - double(v0), p1v = 1.0 - p0v;
-        const double w = (double(fz) - a.lo[2]) * a.inv;
-        const int w0 = int(w), w1 = w0 + 1;
-        const double p0w = w - double(w0), p1w = 1.0 - p0w;
-        // slot value at a corner; 0 one past the grid's last point
-        auto val = [&](int wz, int vy, int ux, int s) -> double {
-            if (ux >= a.nx || vy >= a.ny || wz >= a.nz) return 0.0;
-            return double(a.map[((int64_t(wz) * a.ny + vy) * a.nx + ux) * a.V + s]);
-        };
-        const int es = a.elecSlot, ds = a.desolvSlot;
-        double e = 0, m = 0, d = 0;
-        e += p1u * p1v * p1w * val(w0, v0, u0, es);
-        m += p1u * p1v * p1w * val(w0, v0, u0, t);
-        d += p1u * p1v * p1w * val(w0, v0, u0, ds);
-
-        d += p0u * p1v * p1w * val(w0, v0, u1, ds);
-        m += p0u * p1v * p1w * val(w0, v0, u1, t);
-        e += p0u * p1v * p1w * val(w0, v0, u1, es);
-
-        e += p1u * p0v * p1w * val(w0, v1, u0, es);
-        m += p1u * p0v * p1w * val(w0, v1, u0, t);
-        d += p1u * p0v * p1w * val(w0, v1, u0, ds);
-
-        d += p0u * p0v * p1w * val(w0, v1, u1, ds);
-        m += p0u * p0v * p1w * val(w0, v1, u1, t);
-        e += p0u * p0v * p1w * val(w0, v1, u1, es);
-
-        e += p1u * p1v * p0w * val(w1, v0, u0, es);
-        m += p1u * p1v * p0w * val(w1, v0, u0, t);
-        d += p1u * p1v * p0w * val(w1, v0, u0, ds);
-
-        d += p0u * p1v * p0w * val(w1, v0, u1, ds);
-        m += p0u * p1v * p0w * val(w1, v0, u1, t);
-        e += p0u * p1v * p0w * val(w1, v0, u1, es);
-
-        e += p1u * p0v * p0w * val(w1, v1, u0, es);
-        m += p1u * p0v * p0w * val(w1, v1, u0, t);
-        d += p1u * p0v * p0w * val(w1, v1, u0, ds);
-
-        d += p0u * p0v * p0w * val(w1, v1, u1, ds);
-        m += p0u * p0v * p0w * val(w1, v1, u1, t);
-        e += p0u * p0v * p0w * val(w1, v1, u1, es);
-
-        addE = e * double(a.charge[i]);
-        addM = m + d * double(a.absq[i]);
-        if (a.elec) a.elec[idx] = float(addE);
-        if (a.emap) a.emap[idx] = float(addM);
+        return;
     }
-    a.ce[idx] = addE;
-    a.cm[idx] = addM;
+    const int t = a.type[i];
+    const double u = (double(fx) - a.lo[0]) * a.inv;
+    const int u0 = int(u), u1 = u0 + 1;
+    const double p0u = u - double(u0), p1u = 1.0 - p0u;
+    const double v = (double(fy) - a.lo[1]) * a.inv;
+    const int v0 = int(v), v1 = v0 + 1;
+    const double p0v = v - double(v0), p1v = 1.0 - p0v;
+    const double w = (double(fz) - a.lo[2]) * a.inv;
+    const int w0 = int(w), w1 = w0 + 1;
+    const double p0w = w - double(w0), p1w = 1.0 - p0w;
+    // slot value at a corner; 0 one past the grid's last point
+    auto val = [&](int wz, int vy, int ux, int s) -> double {
+        if (ux >= a.nx || vy >= a.ny || wz >= a.nz) return 0.0;
+        return double(a.map[((int64_t(wz) * a.ny + vy) * a.nx + ux) * a.V + s]);
+    };
+    const int es = a.elecSlot, ds = a.desolvSlot;
+    double e = 0, m = 0, d = 0;
+    e += p1u * p1v * p1w * val(w0, v0, u0, es);
+    m += p1u * p1v * p1w * val(w0, v0, u0, t);
+    d += p1u * p1v * p1w * val(w0, v0, u0, ds);
+
+    d += p0u * p1v * p1w * val(w0, v0, u1, ds);
+    m += p0u * p1v * p1w * val(w0, v0, u1, t);
+    e += p0u * p1v * p1w * val(w0, v0, u1, es);
+
+    e += p1u * p0v * p1w * val(w0, v1, u0, es);
+    m += p1u * p0v * p1w * val(w0, v1, u0, t);
+    d += p1u * p0v * p1w * val(w0, v1, u0, ds);
+
+    d += p0u * p0v * p1w * val(w0, v1, u1, ds);
+    m += p0u * p0v * p1w * val(w0, v1, u1, t);
+    e += p0u * p0v * p1w * val(w0, v1, u1, es);
+
+    e += p1u * p1v * p0w * val(w1, v0, u0, es);
+    m += p1u * p1v * p0w * val(w1, v0, u0, t);
+    d += p1u * p1v * p0w * val(w1, v0, u0, ds);
+
+    d += p0u * p1v * p0w * val(w1, v0, u1, ds);
+    m += p0u * p1v * p0w * val(w1, v0, u1, t);
+    e += p0u * p1v * p0w * val(w1, v0, u1, es);
+
+    e += p1u * p0v * p0w * val(w1, v1, u0, es);
+    m += p1u * p0v * p0w * val(w1, v1, u0, t);
+    d += p1u * p0v * p0w * val(w1, v1, u0, ds);
+
+    d += p0u * p0v * p0w * val(w1, v1, u1, ds);
+    m += p0u * p0v * p0w * val(w1, v1, u1, t);
+    e += p0u * p0v * p0w * val(w1, v1, u1, es);
+
+    addE = e * double(a.charge[i]);
+    addM = m + d * double(a.absq[i]);
+    if (a.elec) a.elec[idx] = float(addE);
+    if (a.emap) a.emap[idx] = float(addM);
 }
 
-// One lane per pose: the reference's running sums in atom order and its
-// returned (Real)elec_total + emap_total.
-__global__ __launch_bounds__(kInterpThreads) void k_interp_totals(const double *ce, const double *cm, int natoms,
-                                                                  int npose, double *elecTotal, double *emapTotal,
-                                                                  float *total)
+__device__ __forceinline__ void writeTotals(const InterpArgs &a, int pose, double et, double mt)
 {
-    const int p = int(blockIdx.x) * kInterpThreads + threadIdx.x;
-    if (p >= npose) return;
-    double et = 0, mt = 0;
-    const int64_t base = int64_t(p) * natoms;
-    for (int i = 0; i < natoms; ++i) {
-        et += ce[base + i];
-        mt += cm[base + i];
+    if (a.elecTotal) a.elecTotal[pose] = et;
+    if (a.emapTotal) a.emapTotal[pose] = mt;
+    if (a.total) a.total[pose] = float(double(float(et)) + mt);
+}
+
+// Poses of up to kInterpThreads atoms: each workgroup takes
+// kInterpThreads / natoms whole poses, one lane per atom; the contributions
+// go through LDS and one lane per pose adds them in atom order (the
+// reference's running sums), then writes the totals.
+__global__ __launch_bounds__(kInterpThreads) void k_interp_poses(InterpArgs a)
+{
+    __shared__ double sE[kInterpThreads], sM[kInterpThreads];
+    const int t = threadIdx.x, A = a.natoms;
+    const int ppw = kInterpThreads / A;
+    const int pose = int(blockIdx.x) * ppw + t / A, i = t % A;
+    if (t < ppw * A && pose < a.npose) interpAtom(a, int64_t(pose) * A + i, i, sE[t], sM[t]);
+    __syncthreads();
+    const int own = int(blockIdx.x) * ppw + t;
+    if (t < ppw && own < a.npose) {
+        double et = 0, mt = 0;
+        for (int k = 0; k < A; ++k) {
+            et += sE[t * A + k];
+            mt += sM[t * A + k];
+        }
+        writeTotals(a, own, et, mt);
     }
-    if (elecTotal) elecTotal[p] = et;
-    if (emapTotal) emapTotal[p] = mt;
-    if (total) total[p] = float(double(float(et)) + mt);
+}
+
+// Poses of more than kInterpThreads atoms: one workgroup per pose, atoms in
+// chunks of kInterpThreads, lane 0 adding each chunk in atom order.
+__global__ __launch_bounds__(kInterpThreads) void k_interp_big_poses(InterpArgs a)
+{
+    __shared__ double sE[kInterpThreads], sM[kInterpThreads];
+    const int t = threadIdx.x, A = a.natoms, pose = int(blockIdx.x);
+    double et = 0, mt = 0;
+    for (int c0 = 0; c0 < A; c0 += kInterpThreads) {
+        const int i = c0 + t, n = min(kInterpThreads, A - c0);
+        if (i < A) interpAtom(a, int64_t(pose) * A + i, i, sE[t], sM[t]);
+        __syncthreads();
+        if (t == 0)
+            for (int k = 0; k < n; ++k) {
+                et += sE[k];
+                mt += sM[k];
+            }
+        __syncthreads();
+    }
+    if (t == 0) writeTotals(a, pose, et, mt);
 }
 
 }  // namespace fgk

@@ -62,7 +62,8 @@ def test_trilinterp_matches_reference_vectors(ctx, pgp, tag):
 
 def test_trilinterp_linear_field_and_many_poses(ctx):
     """Maps linear in the grid indices, 20,000 random poses of 24 atoms over
-    the whole box and beyond it: bit for bit with the oracle."""
+    the whole box and beyond it, and 64 poses of 300 atoms (more than a
+    workgroup's lanes) with ignored atoms: bit for bit with the oracle."""
     import fastgrid_amd as fg
     g0, maps = linear_case()
     g = fg.FgMapGrid()
@@ -83,6 +84,18 @@ def test_trilinterp_linear_field_and_many_poses(ctx):
     assert same(r["elec_total"], oet) and same(r["emap_total"], omt) and same(r["total"], otot)
     with pytest.raises(fg.FgError):
         fg.trilinterp(ctx, ms, xyz[:2], np.full(24, 3, np.int32), q, np.abs(q), 1, 2)
+    # poses of more atoms than a workgroup has lanes (chunked in-order sums)
+    big = (lo - 0.5 + rng.uniform(0, 1, (64, 300, 3)) * (hi - lo + 1.0)).astype(np.float32)
+    bt = rng.integers(0, 3, 300).astype(np.int32)
+    bq = rng.uniform(-0.7, 0.7, 300).astype(np.float32)
+    ign = (rng.uniform(0, 1, 300) < 0.05).astype(np.int32)
+    r = fg.trilinterp(ctx, ms, big, bt, bq, np.abs(bq), 1, 2, ignore=ign)
+    oe, om, oet, omt, otot = orc.trilinterp(g0, maps, 1, 2, big, bq, np.abs(bq), bt, ignore=ign)
+    assert same(r["elec"], oe) and same(r["emap"], om)
+    assert same(r["elec_total"], oet) and same(r["emap_total"], omt) and same(r["total"], otot)
+    # totals alone (no per-atom outputs) are the same
+    r2 = fg.trilinterp(ctx, ms, xyz, types, q, np.abs(q), 1, 2, per_atom=False)
+    assert r2["elec"] is None and same(r2["total"], fg.trilinterp(ctx, ms, xyz, types, q, np.abs(q), 1, 2)["total"])
     ms.close()
 
 
