@@ -162,17 +162,39 @@ def copy_calibration(nbytes=1 << 30):
     print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
 
 
+def elec_shapes(fg, shapes):
+    """Launches the compensated electrostatic kernel on L at each listed shape
+    (pz, tile, bx, px), twice, in order, and prints the HIP-event times: the
+    dispatches a rocprofv3 counter run of this mode measures."""
+    with tempfile.TemporaryDirectory() as wd:
+        prep, a = make_inputs(fg, wd, WORKLOAD_L)
+    ctx = fg.Context(0)
+    job = fg.Job(ctx, a)
+    job.run(fg.RUN_ALL, fg.PREC_COMPENSATED)
+    ctx.sync()
+    for sh in shapes:
+        job.shape = sh
+        ms = []
+        for _ in range(2):
+            job.run(fg.RUN_ELEC, fg.PREC_COMPENSATED)
+            ctx.sync()
+            ms.append(round(ctx.timing().elec_ms, 3))
+        print(json.dumps({"shape": list(sh), "elec_ms": ms}), flush=True)
+    job.close()
+    ctx.close()
+
+
 def traffic_entry(shape):
     """HBM bytes per electrostatic launch on L from the committed counter
-    summary, when it was measured with the same kernel shape
-    (pz, tile, bx, px)."""
+    summary, for the kernel shape (pz, tile, bx, px) the run used."""
     try:
         t = json.load(open(TRAFFIC_FILE))
     except (OSError, ValueError):
         return None
-    if list(t.get("shape", [])) != list(shape):
-        return None
-    return t
+    for e in t.get("shapes", []):
+        if list(e["shape"]) == list(shape):
+            return e
+    return None
 
 
 def pocket_specs(n):
@@ -419,15 +441,22 @@ def main():
     ap.add_argument("--no-interp", action="store_true", help="skip the trilinear-interpolation line")
     ap.add_argument("--pockets", type=int, default=BATCH["pockets"])
     ap.add_argument("--stress", action="store_true", help="also run the 250k-atom / 161^3 stress size")
+    ap.add_argument("--elec-shapes", default="",
+                    help="only launch the electrostatic kernel on L at these shapes, e.g. '4,32,16,1;4,32,8,2' "
+                         "(pz,tile,bx,px; for rocprofv3 counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (reference dispatch for HBM byte counters)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
         return
+    if args.elec_shapes:
+        import fastgrid_amd as fg
+        elec_shapes(fg, [tuple(int(v) for v in sh.split(",")) for sh in args.elec_shapes.split(";")])
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU split is described in DESIGN.md)", file=sys.stderr)
+              "(the grid would be split into z-slabs, one per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import fastgrid_amd as fg
