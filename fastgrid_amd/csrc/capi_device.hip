@@ -933,7 +933,7 @@ struct fg_batch {
     int MM = 4, KH = 0, px = 1, pz = 4;
     bool pointAng = false, clamp = false, typed = true;
     int sharedTableLen = 0, maxTableLen = 0;
-    int cutItems = 0, elecItems = 0, cutWGs = 0, elecWGs = 0;
+    int cutItems = 0, elecItems = 0, elecWGs = 0;
     fgk::ListArgs *dList = nullptr;
     fgk::CutoffArgs *dCut = nullptr;
     fgk::ElecArgs *dElec = nullptr;
@@ -977,16 +977,13 @@ int persistentWGs(fg_batch *b, K kernel, int threads, size_t lds, int items, int
     return FG_OK;
 }
 
-// With launch false, only sizes the persistent grid (done before a capture).
+// One workgroup per (job, brick) item; with launch false, nothing to size.
 template <int MM, int KH, bool kAff, bool kPA, bool kTY>
 int launchCutoffBatch(fg_batch *b, bool launch)
 {
-    auto k = fgk::k_cutoff_batch<MM, KH, kAff, kPA, kTY>;
-    if (!b->cutWGs)
-        if (int rc = persistentWGs(b, k, fgk::kCutThreads, 0, b->cutItems, b->cutWGs)) return rc;
     if (!launch) return FG_OK;
-    hipLaunchKernelGGL(k, dim3(b->cutWGs), dim3(fgk::kCutThreads), 0, b->ctx->stream, b->dCut, b->dCutBase,
-                       int(b->jobs.size()), b->dCounters);
+    hipLaunchKernelGGL((fgk::k_cutoff_batch<MM, KH, kAff, kPA, kTY>), dim3(unsigned(b->cutItems)),
+                       dim3(fgk::kCutThreads), 0, b->ctx->stream, b->dCut, b->dCutBase, int(b->jobs.size()));
     return FG_OK;
 }
 

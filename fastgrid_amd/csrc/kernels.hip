@@ -39,6 +39,29 @@ __device__ __forceinline__ int binOf(double r)
 }
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Read-only data of a launch seen through the constant address space:
+// wave-uniform loads of it become scalar loads, also in persistent loops
+// whose stores (elsewhere) would otherwise make the compiler reload it per
+// lane.  The cast keeps the address (global and constant memory share it).
+#define FG_CONST __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ const FG_CONST T *constView(const T *p)
+{
+    return (const FG_CONST T *)p;
+}
+// A copy of *p (uniform address) loaded with scalar loads.
+template <class T>
+__device__ __forceinline__ T loadConst(const T *p)
+{
+    static_assert(sizeof(T) % 4 == 0, "whole 32-bit words");
+    T out;
+    const FG_CONST int32_t *src = (const FG_CONST int32_t *)p;
+    int32_t *dst = reinterpret_cast<int32_t *>(&out);
+#pragma unroll
+    for (int i = 0; i < int(sizeof(T) / 4); ++i) dst[i] = src[i];
+    return out;
+}
+
 // ------------------------------------------------------------------------
 // Candidate lists: for each brick of the cutoff kernel, every atom whose
 // distance to the brick's box is <= 8 A (+1e-4 A).  Order: coarse cells in
@@ -385,10 +408,13 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
     double cHrv[3] = {0, 0, 0};
     if (kAffinity) {
         double rmin = 999999.0 * 999999.0;
-        const int h0 = a.hStart[b], h1 = a.hStart[b + 1];
+        // uniform reads of read-only inputs: scalar loads
+        const FG_CONST int32_t *hStart = constView(a.hStart), *hList = constView(a.hList);
+        const FG_CONST CutAtom *atoms = constView(a.atoms);
+        const int h0 = hStart[b], h1 = hStart[b + 1];
         for (int c = h0; c < h1; ++c) {
-            const int ia = uniform(a.hList[c]);
-            const double ax = a.atoms[ia].x, ay = a.atoms[ia].y, az = a.atoms[ia].z;
+            const int ia = uniform(hList[c]);
+            const double ax = atoms[ia].x, ay = atoms[ia].y, az = atoms[ia].z;
             double dx = ax - gx, dy = ay - gy, dz = az - gz;
             double r2 = dx * dx + dy * dy + dz * dz;
             if (r2 < rmin || (r2 == rmin && ia < cH)) {
@@ -396,9 +422,9 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                 cH = ia;
             }
         }
-        cHrv[0] = a.atoms[cH].rv[0];
-        cHrv[1] = a.atoms[cH].rv[1];
-        cHrv[2] = a.atoms[cH].rv[2];
+        cHrv[0] = atoms[cH].rv[0];
+        cHrv[1] = atoms[cH].rv[1];
+        cHrv[2] = atoms[cH].rv[2];
     }
 
     double e[MM], hmin[KH > 0 ? KH : 1], hmax[KH > 0 ? KH : 1];
@@ -422,8 +448,8 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
     for (int run = 0; run < nRuns; ++run) {
         int c0, c1;
         if (kTyped) {
-            c0 = a.typeStart[b * (a.T + 1) + run];
-            c1 = a.typeStart[b * (a.T + 1) + run + 1];
+            c0 = constView(a.typeStart)[b * (a.T + 1) + run];
+            c1 = constView(a.typeStart)[b * (a.T + 1) + run + 1];
             if (c0 == c1) continue;
             __syncthreads();  // every lane is done with the previous type's rows
             const double *src = a.pair + int64_t(run) * kTableLen * M;
@@ -433,8 +459,8 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                 L.row[q] = m < M ? src[bin * M + m] : 0.0;
             }
         } else {
-            c0 = a.brickStart[b];
-            c1 = a.brickStart[b + 1];
+            c0 = constView(a.brickStart)[b];
+            c1 = constView(a.brickStart)[b + 1];
         }
         for (int cb = c0; cb < c1; cb += kChunk) {
             const int nc = min(kChunk, c1 - cb);
@@ -753,6 +779,11 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
     const unsigned biasBytes = unsigned(a.floorBitsNeg4);
     const char *epsBytes = reinterpret_cast<const char *>(eps);
 
+    // atom tiles, their boxes and FP64 coordinates: read-only, uniform loads
+    const FG_CONST float *tiles = constView(reinterpret_cast<const float *>(a.tiles));
+    const FG_CONST float *subBox = constView(a.subBox);
+    const FG_CONST double *tiles64 = constView(a.tiles64);
+
     float hi[NP], lo[NP], part[NP];
 #pragma unroll
     for (int q = 0; q < NP; ++q) hi[q] = lo[q] = part[q] = 0.f;
@@ -763,7 +794,7 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
         if (kComp) {
             // the tile is far when every sub-tile box is beyond farCut
             for (int q = j0; q < j1; q += kSubTile) {
-                const float *tb = a.subBox + 6 * (q / kSubTile);
+                const FG_CONST float *tb = subBox + 6 * (q / kSubTile);
                 float dd = 0;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
@@ -779,7 +810,7 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
             if (far) {
 #pragma unroll 2
                 for (int j = c0; j < c1; ++j) {
-                    const float4 at = a.tiles[j];
+                    const float4 at = make_float4(tiles[4 * j], tiles[4 * j + 1], tiles[4 * j + 2], tiles[4 * j + 3]);
                     const float dy = at.y - gy;
                     const float dy2 = dy * dy;
                     float dxy2[PX], dz[PZ];
@@ -824,8 +855,8 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
                 }
             } else {
                 for (int j = c0; j < c1; ++j) {
-                    const float4 at = a.tiles[j];
-                    const double *T64 = a.tiles64 + int64_t(j) * 3;
+                    const float4 at = make_float4(tiles[4 * j], tiles[4 * j + 1], tiles[4 * j + 2], tiles[4 * j + 3]);
+                    const FG_CONST double *T64 = tiles64 + int64_t(j) * 3;
                     const float dy = at.y - gy;
                     const float dy2 = dy * dy;
                     float dxy2[PX], dz[PZ];
@@ -943,12 +974,12 @@ __global__ __launch_bounds__(kElecThreads) void k_elec_batch(const ElecArgs *job
     __shared__ int slot;
     const int total = base[nJobs];
     int loaded = -1;
-    if (sharedTableLen > 0) loadEps(eps, jobs[0].eps32, sharedTableLen);
+    if (sharedTableLen > 0) loadEps(eps, loadConst(jobs).eps32, sharedTableLen);
     for (;;) {
         const int item = nextItem(counter, &slot);
         if (item >= total) break;
         const int job = uniform(jobOfItem(base, nJobs, item));
-        const ElecArgs &a = jobs[job];
+        const ElecArgs a = loadConst(jobs + job);
         if (sharedTableLen <= 0 && job != loaded) {
             loadEps(eps, a.eps32, a.tableLen);
             loaded = job;
@@ -967,22 +998,19 @@ __global__ __launch_bounds__(kListThreads) void k_brick_lists_batch(const ListAr
     __shared__ ListLds S;
     const int item = int(blockIdx.x);
     const int job = uniform(jobOfItem(base, nJobs, item));
-    brickList<kFill>(jobs[job], item - base[job], S);
+    brickList<kFill>(loadConst(jobs + job), item - base[job], S);
 }
 
+// Cutoff maps of every job: one workgroup per (job, brick) item, items in
+// queue order (largest jobs first), so the hardware's in-order dispatch
+// balances the load.
 template <int MM, int KH, bool kAffinity, bool kPointAng, bool kTyped>
-__global__ __launch_bounds__(kCutThreads) void k_cutoff_batch(const CutoffArgs *jobs, const int32_t *base, int nJobs,
-                                                              int *counter)
+__global__ __launch_bounds__(kCutThreads) void k_cutoff_batch(const CutoffArgs *jobs, const int32_t *base, int nJobs)
 {
     __shared__ CutoffLds<MM, kAffinity, kTyped> L;
-    __shared__ int slot;
-    const int total = base[nJobs];
-    for (;;) {
-        const int item = nextItem(counter, &slot);
-        if (item >= total) break;
-        const int job = uniform(jobOfItem(base, nJobs, item));
-        cutoffBrick<MM, KH, kAffinity, kPointAng, kTyped>(jobs[job], item - base[job], L);
-    }
+    const int item = int(blockIdx.x);
+    const int job = uniform(jobOfItem(base, nJobs, item));
+    cutoffBrick<MM, KH, kAffinity, kPointAng, kTyped>(loadConst(jobs + job), item - base[job], L);
 }
 
 // Floating grids of every job that has one: one workgroup per (job, brick) item.
@@ -990,7 +1018,8 @@ __global__ __launch_bounds__(kCutThreads) void k_fmap_batch(const CutoffArgs *jo
 {
     const int item = int(blockIdx.x);
     const int job = uniform(jobOfItem(base, nJobs, item));
-    if (jobs[job].fmap) fmapBrick(jobs[job], item - base[job]);
+    const CutoffArgs a = loadConst(jobs + job);
+    if (a.fmap) fmapBrick(a, item - base[job]);
 }
 
 // Type sort of every job's lists: one workgroup per (job, brick) item.
@@ -999,7 +1028,7 @@ __global__ __launch_bounds__(kListThreads) void k_type_sort_batch(const ListArgs
     __shared__ int32_t cnt[kMaxTypes], runStart[kMaxTypes], wcnt[4 * kMaxTypes];
     const int item = int(blockIdx.x);
     const int job = uniform(jobOfItem(base, nJobs, item));
-    typeSortBrick(jobs[job], item - base[job], cnt, runStart, wcnt);
+    typeSortBrick(loadConst(jobs + job), item - base[job], cnt, runStart, wcnt);
 }
 
 // ------------------------------------------------------------------------
