@@ -315,6 +315,25 @@ __host__ __device__ __forceinline__ void hbondAngular(int hb, int rexp, int diso
     }
 }
 
+// hbondAngular for one (point, bonded atom) pair, out of line.  The FP64
+// acos/cos/sin it evaluates need far more registers than the pair loop; as a
+// call their registers are not added to the loop's, so the loop keeps the
+// occupancy of its plain terms and only pairs of bonded donors/acceptors pay
+// for the call.
+struct AngTerms {
+    double racc, rdon, Hramp;
+};
+__device__ __noinline__ AngTerms pointAngular(const CutAtom &A, bool isClosest, double c0, double c1, double c2,
+                                              double ux, double uy, double uz)
+{
+    const double rv[3] = {A.rv[0], A.rv[1], A.rv[2]}, rv2[3] = {A.rv2[0], A.rv2[1], A.rv2[2]};
+    const double cHrv[3] = {c0, c1, c2};
+    AngTerms t;
+    hbondAngular(A.hbond, A.rexp, A.flags & kFlagDisorder, isClosest, rv, rv2, cHrv, ux, uy, uz, t.racc, t.rdon,
+                 t.Hramp);
+    return t;
+}
+
 // FP64 1/sqrt(x) and x*that from the hardware estimate plus two Newton steps:
 // a few ulp from the correctly rounded values, which is all the continuous
 // uses need; table-bin decisions made from it are guarded (see binGuarded).
@@ -349,6 +368,11 @@ __device__ __forceinline__ double covalentValue(const CutoffArgs &a, double gx, 
     return a.covBarrier * (1 - exp(-0.69314718055994529 * r));
 }
 
+// Waves per SIMD the cutoff kernels are register-allocated for.  The typed
+// kernel's LDS (39.4 KB with 8 maps) admits 4 workgroups of 4 waves per CU;
+// left to itself the compiler schedules the unrolled map loop for 3 waves
+// (up to 163 VGPRs with the directional terms inline).
+constexpr int kCutWaves = 4;
 constexpr int kChunk = 32;     // candidate atoms staged in LDS per round
 constexpr int kRowBins = 801;  // table bins of pairs with r <= 8 A: floor(100 r) <= 800
 constexpr int kMaxTypes = 32;  // receptor types the type-sorted lists support
@@ -516,10 +540,11 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                 } else if (kPointAng && kind == kAngPoint) {
                     // directional terms use the unit vector exactly as the reference forms it
                     const double invR = 1 / sqrt(rSq);
-                    const double rv[3] = {A.rv[0], A.rv[1], A.rv[2]};
-                    const double rv2[3] = {A.rv2[0], A.rv2[1], A.rv2[2]};
-                    hbondAngular(ahb, uniform(A.rexp), dis, uniform(sIdx[c]) == cH, rv, rv2, cHrv, dx * invR,
-                                 dy * invR, dz * invR, racc, rdon, Hramp);
+                    const AngTerms t = pointAngular(A, uniform(sIdx[c]) == cH, cHrv[0], cHrv[1], cHrv[2], dx * invR,
+                                                    dy * invR, dz * invR);
+                    racc = t.racc;
+                    rdon = t.rdon;
+                    Hramp = t.Hramp;
                 }
                 const double S = a.desolv[idx];
                 const double *grow = a.pair + (int64_t(type) * kTableLen + idx) * M;
@@ -593,7 +618,8 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
 }
 
 template <int MM, int KH, bool kAffinity, bool kPointAng, bool kTyped>
-__global__ __launch_bounds__(kCutThreads) void k_cutoff(CutoffArgs a)
+__global__ __launch_bounds__(kCutThreads) __attribute__((amdgpu_waves_per_eu(kCutWaves))) void k_cutoff(
+    CutoffArgs a)
 {
     __shared__ CutoffLds<MM, kAffinity, kTyped> L;
     // XCD-aware brick order: consecutive workgroups go to different XCDs, so
@@ -1005,7 +1031,8 @@ __global__ __launch_bounds__(kListThreads) void k_brick_lists_batch(const ListAr
 // queue order (largest jobs first), so the hardware's in-order dispatch
 // balances the load.
 template <int MM, int KH, bool kAffinity, bool kPointAng, bool kTyped>
-__global__ __launch_bounds__(kCutThreads) void k_cutoff_batch(const CutoffArgs *jobs, const int32_t *base, int nJobs)
+__global__ __launch_bounds__(kCutThreads) __attribute__((amdgpu_waves_per_eu(kCutWaves))) void k_cutoff_batch(
+    const CutoffArgs *jobs, const int32_t *base, int nJobs)
 {
     __shared__ CutoffLds<MM, kAffinity, kTyped> L;
     const int item = int(blockIdx.x);
