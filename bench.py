@@ -162,6 +162,12 @@ def copy_calibration(nbytes=1 << 30):
     print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
 
 
+# the autotuner's candidate shapes (pz, tile, bx, px): variant (px, pz) x tile
+# x brick width, the 256-column brick for px 1 only
+ELEC_SHAPES = [(pz, tile, bx, px) for px, pz in ((1, 4), (1, 8), (2, 4), (2, 8)) for tile in (32, 64, 128)
+               for bx in ((8, 16, 32, 256) if px == 1 else (8, 16, 32))]
+
+
 def elec_shapes(fg, shapes):
     """Launches the compensated electrostatic kernel on L at each listed shape
     (pz, tile, bx, px), twice, in order, and prints the HIP-event times: the
@@ -398,6 +404,60 @@ def run_interp(fg, ctx, job, prep, steps, warmup, cpu):
     return out
 
 
+# A real receptor: half of collagen 1T60 (25,437 atoms, every polar H and O
+# bonded), the fixture of the reference's own tests, on a 127^3 grid
+REAL = dict(pdbqt=os.path.join(ROOT, "tests", "golden", "colagen2", "macromol_1t60_half_prot.pdbqt.gz"),
+            npts=126, spacing=0.375, center=(64.0, -16.0, 37.0), receptor_types="A C HD N OA SA",
+            ligand_types="A C HD N NA OA SA S")
+
+
+def run_real(fg, ctx, steps, warmup):
+    """The real receptor: FP64 form as the error reference, autotuned
+    compensated and plain forms timed."""
+    import gzip
+    w = REAL
+    with tempfile.TemporaryDirectory() as wd:
+        with gzip.open(w["pdbqt"], "rb") as g:
+            open(os.path.join(wd, "rec.pdbqt"), "wb").write(g.read())
+        lig = w["ligand_types"].split()
+        n = w["npts"]
+        gpf = [f"npts {n} {n} {n}", f"gridfld rec.maps.fld", f"spacing {w['spacing']}",
+               f"receptor_types {w['receptor_types']}", f"ligand_types {w['ligand_types']}",
+               "receptor rec.pdbqt", "gridcenter {:.3f} {:.3f} {:.3f}".format(*w["center"]), "smooth 0.5"]
+        gpf += [f"map rec.{t}.map" for t in lig] + ["elecmap rec.e.map", "dsolvmap rec.d.map",
+                                                    "dielectric -0.1465"]
+        open(os.path.join(wd, "rec.gpf"), "w").write("\n".join(gpf) + "\n")
+        cwd = os.getcwd()
+        os.chdir(wd)
+        try:
+            prep = fg.Prepared("rec.gpf")
+            a = prep.arrays()
+            names = prep.map_types()[:a.n_maps] + ["e", "d"]
+        finally:
+            os.chdir(cwd)
+    rv, rv2 = a.rvector.reshape(-1, 3), a.rvector2.reshape(-1, 3)
+    bonded = ~((rv == 0).all(1) & (rv2 == 0).all(1)) & np.isin(a.hbond, (2, 4, 5))
+    job = fg.Job(ctx, a)
+    pcut = job.cutoff_pairs()
+    tot = a.n_atoms * a.P + pcut
+    job.run(fg.RUN_ALL, fg.PREC_FP64)
+    ctx.sync()
+    ref = job.fetch()
+    fg.autotune(ctx, [job])
+    res = {"workload": f"half of collagen 1T60 ({a.n_atoms} atoms, {int(bonded.sum())} bonded H-bond donors/"
+                       f"acceptors), {n + 1}^3 grid @ {w['spacing']} A, {a.n_maps} affinity maps + e + d",
+           "cutoff_pairs": pcut, "shape": job.shape}
+    for key, prec in (("compensated", fg.PREC_COMPENSATED), ("plain", fg.PREC_PLAIN)):
+        t, el, af = time_steps(fg, ctx, job, steps, warmup, prec)
+        err = map_errors(names, job.fetch(), ref)
+        res[key] = {"value": round(tot / t / 1e9, 2), "ms_per_step": round(t * 1e3, 3), "elec_ms": round(el, 3),
+                    "cutoff_ms": round(af, 3), "e_max_abs_vs_fp64": err["e"]["max_abs"],
+                    "affinity_d_max_abs_vs_fp64": max(v["max_abs"] for k, v in err.items() if k != "e")}
+        log(f"real receptor {key}: {t * 1e3:.2f} ms/step (elec {el:.2f}, cutoff+lists {af:.2f})")
+    job.close()
+    return res
+
+
 def run_workload(fg, ctx, w, steps, warmup, tune=True, interp=False, cpu=True):
     """One large job: FP64 form (error reference), plain, FP32 and
     compensated forms.  Returns a dict of results and the job arrays."""
@@ -439,11 +499,12 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-pocket batch")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end command-line timing")
     ap.add_argument("--no-interp", action="store_true", help="skip the trilinear-interpolation line")
+    ap.add_argument("--no-real", action="store_true", help="skip the real-receptor line")
     ap.add_argument("--pockets", type=int, default=BATCH["pockets"])
     ap.add_argument("--stress", action="store_true", help="also run the 250k-atom / 161^3 stress size")
     ap.add_argument("--elec-shapes", default="",
                     help="only launch the electrostatic kernel on L at these shapes, e.g. '4,32,16,1;4,32,8,2' "
-                         "(pz,tile,bx,px; for rocprofv3 counter runs)")
+                         "(pz,tile,bx,px), or 'all' for the autotuner's 42 shapes (for rocprofv3 counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (reference dispatch for HBM byte counters)")
     args = ap.parse_args()
@@ -452,7 +513,8 @@ def main():
         return
     if args.elec_shapes:
         import fastgrid_amd as fg
-        elec_shapes(fg, [tuple(int(v) for v in sh.split(",")) for sh in args.elec_shapes.split(";")])
+        elec_shapes(fg, ELEC_SHAPES if args.elec_shapes == "all" else
+                    [tuple(int(v) for v in sh.split(",")) for sh in args.elec_shapes.split(";")])
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -519,6 +581,8 @@ def main():
             fg.synth_write(wd, w["tag"], w["seed"], w["n_atoms"], w["shape"], w["dims"], w["npts"], w["ligand_types"])
             line["seconds_per_map_set_end_to_end"] = end_to_end(fg, wd, w["tag"])
             log("end to end:", line["seconds_per_map_set_end_to_end"], "s")
+    if not args.no_real:
+        line["real_receptor"] = run_real(fg, ctx, args.steps, args.warmup)
     if not args.no_batch:
         line["batch"] = run_batch(fg, ctx, max(1, args.steps // 2), 1, args.pockets)
     if args.stress:
