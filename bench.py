@@ -505,6 +505,8 @@ def main():
     ap.add_argument("--elec-shapes", default="",
                     help="only launch the electrostatic kernel on L at these shapes, e.g. '4,32,16,1;4,32,8,2' "
                          "(pz,tile,bx,px), or 'all' for the autotuner's 42 shapes (for rocprofv3 counter runs)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only run the 256-pocket batch and print its results (for rocprofv3 counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (reference dispatch for HBM byte counters)")
     args = ap.parse_args()
@@ -523,6 +525,9 @@ def main():
 
     import fastgrid_amd as fg
     ctx = fg.Context(0)
+    if args.batch_only:
+        print(json.dumps({"batch": run_batch(fg, ctx, max(1, args.steps // 2), 1, args.pockets)}), flush=True)
+        return
     L, a = run_workload(fg, ctx, WORKLOAD_L, args.steps, args.warmup, interp=not args.no_interp, cpu=not args.no_cpu)
     comp, plain, fp32 = L["compensated"], L["plain"], L["fp32"]
     tot_pairs = L["elec_pairs"] + L["cutoff_pairs"]

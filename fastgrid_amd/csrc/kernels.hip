@@ -763,6 +763,11 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
     const int tid = threadIdx.x;
     const int sh = a.bxShift;
     int ix, iy, ix0, iy0, bx1, by1, iz0;
+    // (ix, iy): the lane's column, for the stores; (cx, cy, cz): the points
+    // it computes.  Padding lanes past the grid compute a copy of the brick's
+    // last point on that axis (and store nothing), so every lane's points lie
+    // in the brick box and a far test against the box holds for all of them.
+    int cx[PX], cy, cz[PZ];
     if (PX == 1 && sh == 0) {
         // 256 consecutive columns of the x-y plane (x fastest): padding only
         // at the end of each z-slab
@@ -771,6 +776,8 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
         iz0 = (b / a.nbx) * PZ;
         ix = c % a.g.nx;
         iy = c / a.g.nx;  // >= ny past the plane: the lane stores nothing
+        cx[0] = min(c, cl) % a.g.nx;
+        cy = min(c, cl) / a.g.nx;
         iy0 = c0 / a.g.nx;
         by1 = cl / a.g.nx;
         ix0 = iy0 == by1 ? c0 % a.g.nx : 0;
@@ -786,18 +793,43 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
         iy = iy0 + (tid >> sh);
         bx1 = min(ix0 + bxl * PX - 1, a.g.nx - 1);
         by1 = min(iy0 + byl - 1, a.g.ny - 1);
+#pragma unroll
+        for (int i = 0; i < PX; ++i) cx[i] = min(ix + i, bx1);
+        cy = min(iy, by1);
     }
+    const int bz1 = min(iz0 + PZ - 1, a.g.nz - 1);
+#pragma unroll
+    for (int k = 0; k < PZ; ++k) cz[k] = min(iz0 + k, bz1);
     const float s100 = a.spacing100;
-    const float gy = float(iy - a.g.dy2) * s100;
+    const float gy = float(cy - a.g.dy2) * s100;
     float gx[PX], gz[PZ];
 #pragma unroll
-    for (int i = 0; i < PX; ++i) gx[i] = float(ix + i - a.g.dx2) * s100;
+    for (int i = 0; i < PX; ++i) gx[i] = float(cx[i] - a.g.dx2) * s100;
 #pragma unroll
-    for (int k = 0; k < PZ; ++k) gz[k] = float(iz0 + k - a.g.dz2) * s100;
+    for (int k = 0; k < PZ; ++k) gz[k] = float(cz[k] - a.g.dz2) * s100;
     // brick box (scaled), clamped to the grid
-    const int bz1 = min(iz0 + PZ - 1, a.g.nz - 1);
     const float blo[3] = {float(ix0 - a.g.dx2) * s100, float(iy0 - a.g.dy2) * s100, float(iz0 - a.g.dz2) * s100};
     const float bhi[3] = {float(bx1 - a.g.dx2) * s100, float(by1 - a.g.dy2) * s100, float(bz1 - a.g.dz2) * s100};
+    // the wave's own box (its 64 lanes' points): near tiles are tested again
+    // per wave and sub-tile, so a wave far from a sub-tile skips the
+    // per-pair exclusion of the near loop
+    float wlo[3], whi[3];
+    {
+        float xl = gx[0], xh = gx[PX - 1], yl = gy, yh = gy;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            xl = fminf(xl, __shfl_xor(xl, o));
+            xh = fmaxf(xh, __shfl_xor(xh, o));
+            yl = fminf(yl, __shfl_xor(yl, o));
+            yh = fmaxf(yh, __shfl_xor(yh, o));
+        }
+        wlo[0] = __int_as_float(uniform(__float_as_int(xl)));
+        whi[0] = __int_as_float(uniform(__float_as_int(xh)));
+        wlo[1] = __int_as_float(uniform(__float_as_int(yl)));
+        whi[1] = __int_as_float(uniform(__float_as_int(yh)));
+        wlo[2] = gz[0];
+        whi[2] = gz[PZ - 1];
+    }
     const unsigned clampMax = unsigned(a.clampMax);
     const unsigned clampBits = kFloorBits + clampMax;
     // byte offset that turns a biased bit pattern into a table address; read
@@ -814,107 +846,123 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
 #pragma unroll
     for (int q = 0; q < NP; ++q) hi[q] = lo[q] = part[q] = 0.f;
 
+    // far field: every lane's pair is beyond 8 A (box tests at 9 A)
+    auto farAtoms = [&](int c0, int c1) {
+#pragma unroll 2
+        for (int j = c0; j < c1; ++j) {
+            const float4 at = make_float4(tiles[4 * j], tiles[4 * j + 1], tiles[4 * j + 2], tiles[4 * j + 3]);
+            const float dy = at.y - gy;
+            const float dy2 = dy * dy;
+            float dxy2[PX], dz[PZ];
+#pragma unroll
+            for (int i = 0; i < PX; ++i) {
+                const float dx = at.x - gx[i];
+                dxy2[i] = __builtin_fmaf(dx, dx, dy2);
+            }
+#pragma unroll
+            for (int k = 0; k < PZ; ++k) dz[k] = at.z - gz[k];
+            float ri[NP], ev[NP];
+            unsigned bin[NP];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                const float r2 = __builtin_fmaf(dz[q % PZ], dz[q % PZ], dxy2[q / PZ]);
+                ri[q] = __builtin_amdgcn_rsqf(r2);
+                if (kComp) {
+                    // floor(100 r) in one operation: r2 * ri + (2^23 - 1/2)
+                    // lands in [2^23, 2^24), where the FP32 spacing is 1,
+                    // so its bit pattern is 2^23's plus round(100 r - 1/2)
+                    // (far pairs have 100 r >= 900)
+                    bin[q] = __float_as_uint(__builtin_fmaf(r2, ri[q], kFloorBias));
+                    if (kClamp) bin[q] = min(bin[q], clampBits);
+                } else {
+                    bin[q] = unsigned(r2 * ri[q]);
+                    if (kClamp) bin[q] = min(bin[q], clampMax);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                // eps[bits - 2^23's bits] as one shift-add with a uniform byte offset
+                ev[q] = kComp ? *reinterpret_cast<const float *>(epsBytes + ((bin[q] << 2) + biasBytes))
+                              : eps[bin[q]];
+            }
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                // min(1/r, 2) in scaled units; the far-field pairs of the
+                // compensated form are all beyond 8 A
+                const float rc = kComp ? ri[q] : fminf(ri[q], 0.02f);
+                part[q] = __builtin_fmaf(at.w * rc, ev[q], part[q]);
+            }
+        }
+    };
+    // near field: pairs within 8 A (they belong to k_cutoff) are excluded
+    auto nearAtoms = [&](int c0, int c1) {
+        for (int j = c0; j < c1; ++j) {
+            const float4 at = make_float4(tiles[4 * j], tiles[4 * j + 1], tiles[4 * j + 2], tiles[4 * j + 3]);
+            const FG_CONST double *T64 = tiles64 + int64_t(j) * 3;
+            const float dy = at.y - gy;
+            const float dy2 = dy * dy;
+            float dxy2[PX], dz[PZ];
+#pragma unroll
+            for (int i = 0; i < PX; ++i) {
+                const float dx = at.x - gx[i];
+                dxy2[i] = __builtin_fmaf(dx, dx, dy2);
+            }
+#pragma unroll
+            for (int k = 0; k < PZ; ++k) dz[k] = at.z - gz[k];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                const float r2 = __builtin_fmaf(dz[q % PZ], dz[q % PZ], dxy2[q / PZ]);
+                bool include = r2 > a.r2Hi;
+                const bool amb = !include && r2 >= a.r2Lo;
+                if (amb) {
+                    // exact FP64 predicate of the cutoff kernel: r^2 <= 64 is near
+                    const double ddx = T64[0] - (cx[q / PZ] - a.g.dx2) * a.g.spacing;
+                    const double ddy = T64[1] - (cy - a.g.dy2) * a.g.spacing;
+                    const double ddz = T64[2] - (cz[q % PZ] - a.g.dz2) * a.g.spacing;
+                    include = !(ddx * ddx + ddy * ddy + ddz * ddz <= 64.0);
+                }
+                // every lane does the arithmetic; an excluded pair has
+                // its r^2 replaced by +inf, so 1/r is 0 and its biased
+                // bin (a NaN pattern) is clamped into the table
+                const float ri = __builtin_amdgcn_rsqf(include ? r2 : __builtin_inff());
+                const unsigned bin =
+                    min(__float_as_uint(__builtin_fmaf(include ? r2 : __builtin_inff(), ri, kFloorBias)), clampBits);
+                const float ev = *reinterpret_cast<const float *>(epsBytes + ((bin << 2) + biasBytes));
+                part[q] = __builtin_fmaf(at.w * ri, ev, part[q]);
+            }
+        }
+    };
+    // sub-tile q0's box beyond 9 A of the box (lo, hi)
+    auto farFrom = [&](int q0, const float *lo, const float *hi) {
+        const FG_CONST float *tb = subBox + 6 * (q0 / kSubTile);
+        float dd = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float d = fmaxf(fmaxf(tb[k] - hi[k], lo[k] - tb[3 + k]), 0.f);
+            dd = __builtin_fmaf(d, d, dd);
+        }
+        return dd > a.farCut2;
+    };
+
     for (int j0 = 0; j0 < a.nAtoms; j0 += a.tile) {
         const int j1 = min(j0 + a.tile, a.nAtoms);
         bool far = true;
         if (kComp) {
-            // the tile is far when every sub-tile box is beyond farCut
-            for (int q = j0; q < j1; q += kSubTile) {
-                const FG_CONST float *tb = subBox + 6 * (q / kSubTile);
-                float dd = 0;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    float d = fmaxf(fmaxf(tb[k] - bhi[k], blo[k] - tb[3 + k]), 0.f);
-                    dd = __builtin_fmaf(d, d, dd);
-                }
-                far = far && dd > a.farCut2;
-            }
+            // the tile is far when every sub-tile box is beyond farCut of the brick
+            for (int q = j0; q < j1; q += kSubTile) far = far && farFrom(q, blo, bhi);
             far = uniform(far);
         }
         for (int c0 = j0; c0 < j1; c0 += kFold) {
             const int c1 = min(c0 + kFold, j1);
             if (far) {
-#pragma unroll 2
-                for (int j = c0; j < c1; ++j) {
-                    const float4 at = make_float4(tiles[4 * j], tiles[4 * j + 1], tiles[4 * j + 2], tiles[4 * j + 3]);
-                    const float dy = at.y - gy;
-                    const float dy2 = dy * dy;
-                    float dxy2[PX], dz[PZ];
-#pragma unroll
-                    for (int i = 0; i < PX; ++i) {
-                        const float dx = at.x - gx[i];
-                        dxy2[i] = __builtin_fmaf(dx, dx, dy2);
-                    }
-#pragma unroll
-                    for (int k = 0; k < PZ; ++k) dz[k] = at.z - gz[k];
-                    float ri[NP], ev[NP];
-                    unsigned bin[NP];
-#pragma unroll
-                    for (int q = 0; q < NP; ++q) {
-                        const float r2 = __builtin_fmaf(dz[q % PZ], dz[q % PZ], dxy2[q / PZ]);
-                        ri[q] = __builtin_amdgcn_rsqf(r2);
-                        if (kComp) {
-                            // floor(100 r) in one operation: r2 * ri + (2^23 - 1/2)
-                            // lands in [2^23, 2^24), where the FP32 spacing is 1,
-                            // so its bit pattern is 2^23's plus round(100 r - 1/2)
-                            // (far pairs have 100 r >= 900)
-                            bin[q] = __float_as_uint(__builtin_fmaf(r2, ri[q], kFloorBias));
-                            if (kClamp) bin[q] = min(bin[q], clampBits);
-                        } else {
-                            bin[q] = unsigned(r2 * ri[q]);
-                            if (kClamp) bin[q] = min(bin[q], clampMax);
-                        }
-                    }
-#pragma unroll
-                    for (int q = 0; q < NP; ++q) {
-                        // eps[bits - 2^23's bits] as one shift-add with a uniform byte offset
-                        ev[q] = kComp ? *reinterpret_cast<const float *>(epsBytes + ((bin[q] << 2) + biasBytes))
-                                      : eps[bin[q]];
-                    }
-#pragma unroll
-                    for (int q = 0; q < NP; ++q) {
-                        // min(1/r, 2) in scaled units; the far-field pairs of the
-                        // compensated form are all beyond 8 A
-                        const float rc = kComp ? ri[q] : fminf(ri[q], 0.02f);
-                        part[q] = __builtin_fmaf(at.w * rc, ev[q], part[q]);
-                    }
-                }
+                farAtoms(c0, c1);
             } else {
-                for (int j = c0; j < c1; ++j) {
-                    const float4 at = make_float4(tiles[4 * j], tiles[4 * j + 1], tiles[4 * j + 2], tiles[4 * j + 3]);
-                    const FG_CONST double *T64 = tiles64 + int64_t(j) * 3;
-                    const float dy = at.y - gy;
-                    const float dy2 = dy * dy;
-                    float dxy2[PX], dz[PZ];
-#pragma unroll
-                    for (int i = 0; i < PX; ++i) {
-                        const float dx = at.x - gx[i];
-                        dxy2[i] = __builtin_fmaf(dx, dx, dy2);
-                    }
-#pragma unroll
-                    for (int k = 0; k < PZ; ++k) dz[k] = at.z - gz[k];
-#pragma unroll
-                    for (int q = 0; q < NP; ++q) {
-                        const float r2 = __builtin_fmaf(dz[q % PZ], dz[q % PZ], dxy2[q / PZ]);
-                        bool include = r2 > a.r2Hi;
-                        const bool amb = !include && r2 >= a.r2Lo;
-                        if (amb) {
-                            // exact FP64 predicate of the cutoff kernel: r^2 <= 64 is near
-                            const double ddx = T64[0] - (ix + q / PZ - a.g.dx2) * a.g.spacing;
-                            const double ddy = T64[1] - (iy - a.g.dy2) * a.g.spacing;
-                            const double ddz = T64[2] - (iz0 + q % PZ - a.g.dz2) * a.g.spacing;
-                            include = !(ddx * ddx + ddy * ddy + ddz * ddz <= 64.0);
-                        }
-                        // every lane does the arithmetic; an excluded pair has
-                        // its r^2 replaced by +inf, so 1/r is 0 and its biased
-                        // bin (a NaN pattern) is clamped into the table
-                        const float ri = __builtin_amdgcn_rsqf(include ? r2 : __builtin_inff());
-                        const unsigned bin =
-                            min(__float_as_uint(__builtin_fmaf(include ? r2 : __builtin_inff(), ri, kFloorBias)),
-                                clampBits);
-                        const float ev = *reinterpret_cast<const float *>(epsBytes + ((bin << 2) + biasBytes));
-                        part[q] = __builtin_fmaf(at.w * ri, ev, part[q]);
-                    }
+                // near tile: decide again per wave and sub-tile (a fold chunk
+                // holds whole sub-tiles)
+                for (int q = c0; q < c1; q += kSubTile) {
+                    const int q1 = min(q + kSubTile, c1);
+                    if (uniform(farFrom(q, wlo, whi))) farAtoms(q, q1);
+                    else nearAtoms(q, q1);
                 }
             }
             if (kComp && ((c1 & (kFold - 1)) == 0 || c1 == a.nAtoms)) {
