@@ -476,7 +476,13 @@ int buildElecTiles(fg_job *j, const fg_job_desc &d, double gridMaxAbs)
     const double B2 = 3 * u * 6.5e5 + 2 * dmax * std::sqrt(3.0) * Ed + 3 * Ed * Ed;
     j->r2Lo = float(6.4e5 - 4 * B2 - 1.0);
     j->r2Hi = float(6.4e5 + 4 * B2 + 1.0);
-    j->farCut2 = float(900.0 * 900.0);  // tile boxes farther than 9 A hold only far pairs
+    // Boxes farther apart than ~8.02 A hold only far pairs: the FP32
+    // coordinates and outward-rounded boxes differ from the FP64 ones by at
+    // most Ed per axis (< 1e-5 A for atoms within 100 A of the origin), far
+    // inside the 0.02 A margin.  A pair beyond the guard band gets the same
+    // FP32 arithmetic in the near and far loops, so the cut moves no bits.
+    const double farCut = 802.0 + 4 * Ed;
+    j->farCut2 = float(farCut * farCut);
     j->spacing100 = float(s * 100.0);
     // dielectric table for k_elec: extended past 8191 (no clamp) when it fits
     double span2 = 0;

@@ -846,7 +846,7 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
 #pragma unroll
     for (int q = 0; q < NP; ++q) hi[q] = lo[q] = part[q] = 0.f;
 
-    // far field: every lane's pair is beyond 8 A (box tests at 9 A)
+    // far field: every lane's pair is beyond 8 A (box tests at 8.02 A)
     auto farAtoms = [&](int c0, int c1) {
 #pragma unroll 2
         for (int j = c0; j < c1; ++j) {
@@ -871,7 +871,7 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
                     // floor(100 r) in one operation: r2 * ri + (2^23 - 1/2)
                     // lands in [2^23, 2^24), where the FP32 spacing is 1,
                     // so its bit pattern is 2^23's plus round(100 r - 1/2)
-                    // (far pairs have 100 r >= 900)
+                    // (far pairs have 100 r > 800)
                     bin[q] = __float_as_uint(__builtin_fmaf(r2, ri[q], kFloorBias));
                     if (kClamp) bin[q] = min(bin[q], clampBits);
                 } else {
@@ -932,7 +932,7 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
             }
         }
     };
-    // sub-tile q0's box beyond 9 A of the box (lo, hi)
+    // sub-tile q0's box beyond farCut of the box (lo, hi)
     auto farFrom = [&](int q0, const float *lo, const float *hi) {
         const FG_CONST float *tb = subBox + 6 * (q0 / kSubTile);
         float dd = 0;

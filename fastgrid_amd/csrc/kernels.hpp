@@ -131,7 +131,7 @@ struct ElecArgs {
     const float *subBox;     // [nAtoms/kSubTile][6] lo xyz, hi xyz (scaled)
     const float *eps32;      // [tableLen]
     double *e;               // [P] electrostatic map (near part already there when compensated)
-    float farCut2;           // (900)^2: sub-tiles farther than 9 A hold only far pairs
+    float farCut2;           // (802)^2: sub-tiles farther than 8.02 A hold only far pairs
     float r2Lo, r2Hi;        // ambiguity band around 64e4 in scaled r^2
     float spacing100;        // 100 * spacing
 };
