@@ -820,6 +820,33 @@ int timeShape(fg_job *j, int px, int pz, int tile, int bxShift, bool comp, float
     return FG_OK;
 }
 
+template <int PX, int PZ>
+const void *elecKernelV(bool comp, bool clamp)
+{
+    if (comp) return clamp ? reinterpret_cast<const void *>(fgk::k_elec<PX, PZ, true, true>)
+                           : reinterpret_cast<const void *>(fgk::k_elec<PX, PZ, true, false>);
+    return clamp ? reinterpret_cast<const void *>(fgk::k_elec<PX, PZ, false, true>)
+                 : reinterpret_cast<const void *>(fgk::k_elec<PX, PZ, false, false>);
+}
+
+// Share of the device's workgroup slots one launch of shape (px, pz, tile,
+// bxShift) keeps busy on average: bricks / (slots x rounds).  A launch of
+// fewer bricks than slots leaves slots idle, so its time measures one
+// brick's latency rather than throughput.
+int elecFill(const fg_job *j, int px, int pz, int bxShift, bool comp, double &fill)
+{
+    const fgk::ElecArgs a = elecArgs(j, px, pz, 32, bxShift);
+    const void *k = px == 2 ? (pz == 8 ? elecKernelV<2, 8>(comp, j->clamp) : elecKernelV<2, 4>(comp, j->clamp))
+                            : (pz == 8 ? elecKernelV<1, 8>(comp, j->clamp) : elecKernelV<1, 4>(comp, j->clamp));
+    int perCU = 0;
+    FG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k, fgk::kElecThreads,
+                                                         size_t(j->tableLen) * sizeof(float)));
+    const double slots = double(std::max(perCU, 1)) * double(j->ctx->numCUs);
+    const double bricks = double(a.nbx) * a.nby * a.nbz;
+    fill = bricks / (slots * std::ceil(bricks / slots));
+    return FG_OK;
+}
+
 // Padded xy area of a 256-lane brick of width 1 << shift over the grid.
 long paddedArea(const fg_job *j, int shift)
 {
@@ -880,6 +907,14 @@ int tuneJobs(fg_ctx *ctx, fg_job *const *jobs, int n, bool comp)
                     if (kVar[v][0] == 2 && kShifts[d] == 0) continue;
                     if (int rc = timeShape(rep, kVar[v][0], kVar[v][1], kTiles[b], kShifts[d], comp, c.t[v][b][d]))
                         return rc;
+                    if (n > 1) {
+                        // jobs tuned together run as a batch, which fills the
+                        // device with all of their bricks: compare shapes by
+                        // throughput (time x share of the slots kept busy)
+                        double fill = 1;
+                        if (int rc = elecFill(rep, kVar[v][0], kVar[v][1], kShifts[d], comp, fill)) return rc;
+                        c.t[v][b][d] = float(c.t[v][b][d] * fill);
+                    }
                 }
         const double repWork = std::max(1.0, double(rep->N) * double(rep->P));
         double work = 0;
@@ -897,8 +932,9 @@ int tuneJobs(fg_ctx *ctx, fg_job *const *jobs, int n, bool comp)
     char line[320];
     std::snprintf(line, sizeof line,
                   "px %d pz %d for all jobs (modelled totals: px1 pz4 %.3f ms, px1 pz8 %.3f ms, px2 pz4 %.3f ms, "
-                  "px2 pz8 %.3f ms)\n",
-                  kVar[vi][0], kVar[vi][1], total[0], total[1], total[2], total[3]);
+                  "px2 pz8 %.3f ms)%s\n",
+                  kVar[vi][0], kVar[vi][1], total[0], total[1], total[2], total[3],
+                  n > 1 ? "; times x device fill (throughput of a batch)" : "");
     report += line;
     for (Cls &c : cls) {
         int bb = 0, bd = 0;

@@ -226,7 +226,11 @@ int fg_job_set_shape(fg_job *job, const fg_shape *shape);
 
 /* Groups the jobs into size classes (atom count and point count octaves,
  * best-padding brick width), times every shape on the median job of each
- * class with HIP events, and gives every job its class's fastest shape.  The
+ * class with HIP events, and gives every job its class's fastest shape.  With
+ * more than one job the jobs are taken to run as one batch, which fills the
+ * device with all of their bricks: each time is multiplied by the share of
+ * the device's workgroup slots its launch kept busy, so shapes compare by
+ * throughput rather than by one small launch's latency.  The
  * points per lane along x and z are common to all the jobs (a batch launches
  * one kernel for all of them): the pair with the least total modelled time
  * wins.  precision: FG_PREC_COMPENSATED or FG_PREC_FP32.  The
