@@ -1913,9 +1913,10 @@ int fg_trilinterp(fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int np
     const bool sorted = A <= fgk::kInterpThreads && npose >= kInterpSortMin;
     size_t sortBytes = 0;
     if (sorted)
-        FG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, static_cast<const uint32_t *>(nullptr),
-                                                  static_cast<uint32_t *>(nullptr), static_cast<const int32_t *>(nullptr),
-                                                  static_cast<int32_t *>(nullptr), npose, 0, 30, ctx->stream));
+        FG_HIP(hipcub::DeviceRadixSort::SortPairs(
+            nullptr, sortBytes, static_cast<const uint32_t *>(nullptr), static_cast<uint32_t *>(nullptr),
+            static_cast<const int32_t *>(nullptr), static_cast<int32_t *>(nullptr), npose, 0, fgk::kPoseKeyBits,
+            ctx->stream));
     const size_t oKey = off; off = align(off + (sorted ? size_t(npose) * 2 * sizeof(uint32_t) : 0));
     const size_t oIdx = off; off = align(off + (sorted ? size_t(npose) * 2 * sizeof(int32_t) : 0));
     const size_t oSort = off; off = align(off + sortBytes);
@@ -1964,8 +1965,8 @@ int fg_trilinterp(fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int np
         e = hipGetLastError();
         size_t bytes = sortBytes;
         if (e == hipSuccess)
-            e = hipcub::DeviceRadixSort::SortPairs(b + oSort, bytes, keys, keys + npose, idx, idx + npose, npose, 0, 30,
-                                                   s);
+            e = hipcub::DeviceRadixSort::SortPairs(b + oSort, bytes, keys, keys + npose, idx, idx + npose, npose, 0,
+                                                   fgk::kPoseKeyBits, s);
         a.order = idx + npose;
     }
     if (e == hipSuccess) {

@@ -15,6 +15,7 @@
 namespace fgk {
 
 constexpr int kInterpThreads = 256;
+constexpr int kPoseKeyBits = 18;  // 6 bits per axis
 
 // The float AutoDock reads back from a printed map value: "0" below 0.0005
 // in magnitude, else "%.3f" of floor(1000 v + 0.5) * 0.001, which is the
@@ -136,8 +137,9 @@ __device__ __forceinline__ void writeTotals(const InterpArgs &a, int pose, doubl
     if (a.total) a.total[pose] = float(double(float(et)) + mt);
 }
 
-// Spatial sort key of each pose: Morton code (10 bits per axis) of its
-// first atom's cell in the grid box.
+// Spatial sort key of each pose: Morton code of its first atom's cell in a
+// 64^3 division of the grid box (kPoseKeyBits bits; finer cells would only
+// add radix passes).
 __global__ __launch_bounds__(kInterpThreads) void k_pose_keys(InterpArgs a, uint32_t *keys, int32_t *idx)
 {
     const int p = int(blockIdx.x) * kInterpThreads + threadIdx.x;
@@ -146,8 +148,8 @@ __global__ __launch_bounds__(kInterpThreads) void k_pose_keys(InterpArgs a, uint
     uint32_t key = 0;
     for (int k = 0; k < 3; ++k) {
         const double f = (double(c[k]) - a.lo[k]) / (a.hi[k] - a.lo[k]);
-        const uint32_t q = uint32_t(fmin(fmax(f, 0.0), 1.0) * 1023.0);
-        uint32_t v = q;  // spread 10 bits to every third bit
+        const uint32_t q = uint32_t(fmin(fmax(f, 0.0), 1.0) * 63.0);
+        uint32_t v = q;  // spread the bits to every third bit
         v = (v | (v << 16)) & 0x030000FFu;
         v = (v | (v << 8)) & 0x0300F00Fu;
         v = (v | (v << 4)) & 0x030C30C3u;
