@@ -81,7 +81,7 @@ struct fg_job {
     bool pointAng = false;          // some atom's directional terms depend on the point
     // cutoff inputs
     CutAtom *dAtoms = nullptr;
-    double *dAB = nullptr;
+    double *dVS = nullptr;
     double *dPair = nullptr, *dDesolv = nullptr, *dEps = nullptr, *dCovalent = nullptr;
     double *dXYZ = nullptr;
     int32_t *dHbond = nullptr, *dCellStart = nullptr, *dCellAtoms = nullptr;
@@ -245,7 +245,7 @@ fgk::CutoffArgs cutoffArgs(const fg_job *j, int doElecNear)
     a.doElecNear = doElecNear;
     a.P = j->P;
     a.atoms = j->dAtoms;
-    a.ab = j->dAB;
+    a.vs = j->dVS;
     a.brickStart = j->dStart;
     a.brickList = j->typed ? j->dSorted : j->dList;
     a.hStart = j->dHStart;
@@ -570,18 +570,15 @@ int buildCutoff(fg_job *j, const fg_job_desc &d)
         hb[size_t(i)] = c.hbond;
     }
     if (int rc = devUpload(j, j->dAtoms, atoms.data(), atoms.size())) return rc;
-    // desolvation products per atom and map, formed as FG/CalculateGridmaps.cpp:234-235 forms them
-    std::vector<double> ab(size_t(std::max(N, 1)) * size_t(2 * std::max(j->M, 1)), 0.0);
+    // per atom the two factors of the desolvation term (FG/CalculateGridmaps.cpp:234-235):
+    // vol and solpar + solparQ |q|; the probe's factors are per map (MapParam)
+    std::vector<double> vs(size_t(std::max(N, 1)) * 2, 0.0);
     if (j->hasGeom)
         for (int i = 0; i < N; ++i) {
-            const double solQ = A.solpar[i] + d.solpar_q * std::fabs(A.charge[i]);
-            for (int k = 0; k < j->M; ++k) {
-                const fg_map_params &mp = d.maps[j->order[size_t(k)]];
-                ab[size_t(i) * 2 * j->M + 2 * k] = mp.solpar_probe * A.vol[i];
-                ab[size_t(i) * 2 * j->M + 2 * k + 1] = solQ * mp.vol_probe;
-            }
+            vs[size_t(2 * i)] = A.vol[i];
+            vs[size_t(2 * i + 1)] = A.solpar[i] + d.solpar_q * std::fabs(A.charge[i]);
         }
-    if (int rc = devUpload(j, j->dAB, ab.data(), ab.size())) return rc;
+    if (int rc = devUpload(j, j->dVS, vs.data(), vs.size())) return rc;
     if (int rc = devUpload(j, j->dXYZ, xyz.data(), xyz.size())) return rc;
     if (int rc = devUpload(j, j->dHbond, hb.data(), hb.size())) return rc;
     // coarse cells: edge >= 4 A and >= the brick edge, so a brick's 8 A
@@ -1339,7 +1336,8 @@ int jobBuildHost(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
     j->mp.resize(size_t(j->M));
     for (int k = 0; k < j->M; ++k) {
         const int m = j->order[size_t(k)];
-        j->mp[size_t(k)] = fgk::MapParam{actions[size_t(m)], d->maps[m].is_covalent, m, 0};
+        j->mp[size_t(k)] = fgk::MapParam{actions[size_t(m)], d->maps[m].is_covalent, m, 0, d->maps[m].solpar_probe,
+                                         d->maps[m].vol_probe};
     }
     double gridMaxAbs = 0;
     const int n[3] = {g.nx, g.ny, g.nz}, dv[3] = {g.div2x, g.div2y, g.div2z};

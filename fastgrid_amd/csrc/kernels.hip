@@ -391,7 +391,7 @@ constexpr int ldsMaps()
 template <int MM, bool kAffinity, bool kTyped>
 struct CutoffLds {
     CutAtom atom[kChunk];
-    double ab[kAffinity ? kChunk * 2 * MM : 1];
+    double vs[kAffinity ? kChunk * 2 : 1];
     int32_t idx[kChunk];
     double row[kTyped ? kRowBins * ldsMaps<MM>() : 1];
     double eps[kTyped ? kRowBins : 1];
@@ -405,7 +405,7 @@ template <int MM, int KH, bool kAffinity, bool kPointAng, bool kTyped>
 __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLds<MM, kAffinity, kTyped> &L)
 {
     CutAtom *sAtom = L.atom;
-    double *sAB = L.ab;
+    double *sVS = L.vs;
     int32_t *sIdx = L.idx;
     const int bx = b % a.nbx, by = (b / a.nbx) % a.nby, bz = b / (a.nbx * a.nby);
     // wave = 4x4x4 cube of points, workgroup = 2x2x1 waves
@@ -464,7 +464,7 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
         hmin[k] = 999999;
         hmax[k] = -999999;
     }
-    double dmap = 0, enear = 0;
+    double dmap = 0, enear = 0, sv = 0, sq = 0;
 
     // candidates: one run over the brick's list, or one run per receptor
     // type over the type-sorted list with that type's rows in LDS
@@ -489,21 +489,15 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
         for (int cb = c0; cb < c1; cb += kChunk) {
             const int nc = min(kChunk, c1 - cb);
             __syncthreads();  // every lane is done with the previous chunk
-            // stage nc atom records (16-byte pieces) and their desolvation products
+            // stage nc atom records (16-byte pieces) and their desolvation factors
             for (int q = tid; q < nc * 8; q += kCutThreads) {
                 const int ia = a.brickList[cb + (q >> 3)];
                 if ((q & 7) == 0) sIdx[q >> 3] = ia;
                 reinterpret_cast<float4 *>(sAtom)[q] = reinterpret_cast<const float4 *>(a.atoms + ia)[q & 7];
             }
-            if (kAffinity)
-                for (int q = tid; q < nc * MM; q += kCutThreads) {
-                    const int c = q / MM, m = q - c * MM;
-                    if (m < M) {
-                        const double *src = a.ab + int64_t(a.brickList[cb + c]) * 2 * M + 2 * m;
-                        sAB[(c * MM + m) * 2] = src[0];
-                        sAB[(c * MM + m) * 2 + 1] = src[1];
-                    }
-                }
+            if (kAffinity && tid < nc)
+                reinterpret_cast<double2 *>(sVS)[tid] =
+                    reinterpret_cast<const double2 *>(a.vs)[a.brickList[cb + tid]];
             __syncthreads();
             for (int c = 0; c < nc; ++c) {
                 const CutAtom &A = sAtom[c];
@@ -556,7 +550,7 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                     (dis && a.hydrogen >= 0)
                         ? a.pair + (int64_t(a.hydrogen) * kTableLen + (idx - 110 > 0 ? idx - 110 : 0)) * M
                         : row;
-                const double *ab = &sAB[c * MM * 2];
+                const double vol = sVS[2 * c], solQ = sVS[2 * c + 1];
                 const int shift = 2 * ahb;
                 if (((a.addOnlyClasses >> ahb) & 1) && kTyped) {
                     // every map adds the table value for this atom (no H-bond
@@ -564,10 +558,9 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                     // m >= M (the kernel is compiled for 4, 6, 8 or 16 maps)
                     // accumulate values that are never stored.
 #pragma unroll
-                    for (int m = 0; m < MM; ++m) {
-                        e[m] += FG_ROW(m);
-                        e[m] += ab[2 * m] * S + ab[2 * m + 1] * S;
-                    }
+                    for (int m = 0; m < MM; ++m) e[m] += FG_ROW(m);
+                    sv = __builtin_fma(vol, S, sv);
+                    sq = __builtin_fma(solQ, S, sq);
                     dmap += A.qvol * S;
                     continue;
                 }
@@ -596,8 +589,9 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                         }
                     }
                     e[m] += add;
-                    e[m] += ab[2 * m] * S + ab[2 * m + 1] * S;
                 }
+                sv = __builtin_fma(vol, S, sv);
+                sq = __builtin_fma(solQ, S, sq);
                 dmap += A.qvol * S;
 #undef FG_ROW
             }
@@ -609,6 +603,9 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
         for (int m = 0; m < MM; ++m) {
             if (m >= M) continue;
             double v = e[m];
+            // the desolvation term summed over atoms with the probe's factors
+            // taken out: solProbe * sum(vol S) + volProbe * sum(solQ S)
+            if (!a.mp[m].isCovalent) v += a.mp[m].solProbe * sv + a.mp[m].volProbe * sq;
             if (m < KH && (flags & (1u << m))) v += hmin[m < KH ? m : 0] + hmax[m < KH ? m : 0];
             a.maps[int64_t(a.mp[m].out) * a.P + p] = v;
         }

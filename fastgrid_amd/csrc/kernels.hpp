@@ -74,6 +74,7 @@ struct MapParam {
     int32_t isCovalent;
     int32_t out;      // index of the output map
     int32_t pad;
+    double solProbe, volProbe;  // the probe's desolvation parameters
 };
 
 struct GridGeom {
@@ -94,7 +95,7 @@ struct CutoffArgs {
     int32_t M, hydrogen, doElecNear, pad;
     int64_t P;
     const CutAtom *atoms;
-    const double *ab;                         // [N][2M] desolvation products
+    const double *vs;                         // [N][2] vol and solpar + solparQ |q| per atom
     const int32_t *brickStart, *brickList;    // all candidates (type-sorted for the typed kernel)
     const int32_t *hStart, *hList;            // DS/D1 candidates
     const int32_t *typeStart;                 // [nb][T+1] runs of one receptor type in brickList
