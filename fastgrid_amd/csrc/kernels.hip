@@ -368,11 +368,6 @@ __device__ __forceinline__ double covalentValue(const CutoffArgs &a, double gx, 
     return a.covBarrier * (1 - exp(-0.69314718055994529 * r));
 }
 
-// Waves per SIMD the cutoff kernels are register-allocated for.  The typed
-// kernel's LDS (39.4 KB with 8 maps) admits 4 workgroups of 4 waves per CU;
-// left to itself the compiler schedules the unrolled map loop for 3 waves
-// (up to 163 VGPRs with the directional terms inline).
-constexpr int kCutWaves = 4;
 constexpr int kChunk = 32;     // candidate atoms staged in LDS per round
 constexpr int kRowBins = 801;  // table bins of pairs with r <= 8 A: floor(100 r) <= 800
 constexpr int kMaxTypes = 32;  // receptor types the type-sorted lists support
@@ -396,6 +391,20 @@ struct CutoffLds {
     double row[kTyped ? kRowBins * ldsMaps<MM>() : 1];
     double eps[kTyped ? kRowBins : 1];
 };
+
+// Waves per SIMD a cutoff kernel is register-allocated for: the workgroups
+// (one wave per SIMD each) that its LDS admits in a CU's 160 KB, at most 5
+// (96 VGPRs) for up to 8 maps and 2 min/max maps, else at most 4 (128 VGPRs;
+// the larger loops spill at 96).  Left to itself the compiler schedules the
+// unrolled map loop for fewer waves than the LDS allows (up to 163 VGPRs
+// with 8 maps).
+template <int MM, int KH, bool kAffinity, bool kTyped>
+constexpr int cutWaves()
+{
+    constexpr int lds = int((160 * 1024) / sizeof(CutoffLds<MM, kAffinity, kTyped>));
+    constexpr int cap = (MM <= 8 && KH <= 2) ? 5 : 4;
+    return lds < cap ? lds : cap;
+}
 
 // One brick of the cutoff kernel; L is the workgroup's LDS.
 // kPointAng: the job has atoms whose directional terms depend on the point
@@ -615,7 +624,7 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
 }
 
 template <int MM, int KH, bool kAffinity, bool kPointAng, bool kTyped>
-__global__ __launch_bounds__(kCutThreads) __attribute__((amdgpu_waves_per_eu(kCutWaves))) void k_cutoff(
+__global__ __launch_bounds__(kCutThreads) __attribute__((amdgpu_waves_per_eu(cutWaves<MM, KH, kAffinity, kTyped>()))) void k_cutoff(
     CutoffArgs a)
 {
     __shared__ CutoffLds<MM, kAffinity, kTyped> L;
@@ -1076,7 +1085,7 @@ __global__ __launch_bounds__(kListThreads) void k_brick_lists_batch(const ListAr
 // queue order (largest jobs first), so the hardware's in-order dispatch
 // balances the load.
 template <int MM, int KH, bool kAffinity, bool kPointAng, bool kTyped>
-__global__ __launch_bounds__(kCutThreads) __attribute__((amdgpu_waves_per_eu(kCutWaves))) void k_cutoff_batch(
+__global__ __launch_bounds__(kCutThreads) __attribute__((amdgpu_waves_per_eu(cutWaves<MM, KH, kAffinity, kTyped>()))) void k_cutoff_batch(
     const CutoffArgs *jobs, const int32_t *base, int nJobs)
 {
     __shared__ CutoffLds<MM, kAffinity, kTyped> L;
