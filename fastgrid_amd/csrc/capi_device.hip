@@ -49,9 +49,16 @@ struct DevBuf {
 struct fg_ctx {
     int device = 0, numCUs = 0;
     hipStream_t stream = nullptr;
+    // The electrostatic kernels run on `side`, concurrently with the cutoff
+    // kernels on `stream` (both at the default priority): fork is recorded
+    // on stream before them, e0/e1 bracket them on side, and stream waits
+    // for e1.
+    hipStream_t side = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr, e0 = nullptr, e1 = nullptr;
     bool haveTiming = false;
-    bool splitTiming = true;  // ev[1] separates the cutoff and electrostatic phases
+    bool splitTiming = true;  // ev[0]..ev[1]: cutoff phase; e0..e1: electrostatic phase
+    bool elecTimed = false;   // e0..e1 were recorded by the last run
     std::string tuneReport;
     // live jobs and batches: fg_destroy detaches them, so destroying them
     // afterwards only frees their memory
@@ -82,6 +89,7 @@ struct fg_job {
     // cutoff inputs
     CutAtom *dAtoms = nullptr;
     double *dVS = nullptr;
+    double *dFar = nullptr;  // [P] far-field e of the compensated form
     double *dPair = nullptr, *dDesolv = nullptr, *dEps = nullptr, *dCovalent = nullptr;
     double *dXYZ = nullptr;
     int32_t *dHbond = nullptr, *dCellStart = nullptr, *dCellAtoms = nullptr;
@@ -355,6 +363,7 @@ fgk::ElecArgs elecArgs(const fg_job *j, int px, int pz, int tile, int bxShift)
     a.subBox = j->dSubBox;
     a.eps32 = j->dEps32;
     a.e = j->dMaps + int64_t(j->M) * j->P;
+    a.far = j->dFar;
     a.farCut2 = j->farCut2;
     a.r2Lo = j->r2Lo;
     a.r2Hi = j->r2Hi;
@@ -363,34 +372,35 @@ fgk::ElecArgs elecArgs(const fg_job *j, int px, int pz, int tile, int bxShift)
 }
 
 template <int PX, int PZ>
-void launchElecV(fg_job *j, const fgk::ElecArgs &a, bool comp, size_t lds)
+void launchElecV(fg_job *j, const fgk::ElecArgs &a, bool comp, size_t lds, hipStream_t s)
 {
     dim3 grid(unsigned(a.nbx * a.nby * a.nbz));
     dim3 block(fgk::kElecThreads);
-    hipStream_t s = j->ctx->stream;
     if (comp && j->clamp) hipLaunchKernelGGL((fgk::k_elec<PX, PZ, true, true>), grid, block, lds, s, a);
     else if (comp) hipLaunchKernelGGL((fgk::k_elec<PX, PZ, true, false>), grid, block, lds, s, a);
     else if (j->clamp) hipLaunchKernelGGL((fgk::k_elec<PX, PZ, false, true>), grid, block, lds, s, a);
     else hipLaunchKernelGGL((fgk::k_elec<PX, PZ, false, false>), grid, block, lds, s, a);
 }
 
-// k_elec of shape (px, pz, tile, bxShift); px 2 needs a 2D brick (bxShift > 0).
-void launchElecShape(fg_job *j, int px, int pz, int tile, int bxShift, bool comp)
+// k_elec of shape (px, pz, tile, bxShift) on stream s; px 2 needs a 2D brick
+// (bxShift > 0).
+void launchElecShape(fg_job *j, int px, int pz, int tile, int bxShift, bool comp, hipStream_t s)
 {
     const fgk::ElecArgs a = elecArgs(j, px, pz, tile, bxShift);
     const size_t lds = size_t(j->tableLen) * sizeof(float);
     if (px == 2) {
-        if (pz == 8) launchElecV<2, 8>(j, a, comp, lds);
-        else launchElecV<2, 4>(j, a, comp, lds);
+        if (pz == 8) launchElecV<2, 8>(j, a, comp, lds, s);
+        else launchElecV<2, 4>(j, a, comp, lds, s);
     } else {
-        if (pz == 8) launchElecV<1, 8>(j, a, comp, lds);
-        else launchElecV<1, 4>(j, a, comp, lds);
+        if (pz == 8) launchElecV<1, 8>(j, a, comp, lds, s);
+        else launchElecV<1, 4>(j, a, comp, lds, s);
     }
 }
 
-int launchElec(fg_job *j, int precision)
+// The electrostatic kernel of the precision on stream s (the compensated
+// form writes its far part into dFar, for k_add_far).
+int launchElec(fg_job *j, int precision, hipStream_t s)
 {
-    hipStream_t s = j->ctx->stream;
     double *e = j->dMaps + int64_t(j->M) * j->P;
     if (precision == FG_PREC_FP64) {
         dim3 grid(unsigned((j->g.nx + 15) / 16), unsigned((j->g.ny + 15) / 16), unsigned(j->g.nz));
@@ -398,7 +408,7 @@ int launchElec(fg_job *j, int precision)
         FG_HIP(hipGetLastError());
         return FG_OK;
     }
-    launchElecShape(j, j->px, j->pz, j->tile, j->bxShift, precision == FG_PREC_COMPENSATED);
+    launchElecShape(j, j->px, j->pz, j->tile, j->bxShift, precision == FG_PREC_COMPENSATED, s);
     FG_HIP(hipGetLastError());
     return FG_OK;
 }
@@ -761,9 +771,8 @@ int runPlainAffinity(fg_job *j)
     return FG_OK;
 }
 
-int runPlainElec(fg_job *j)
+int runPlainElec(fg_job *j, hipStream_t s)
 {
-    hipStream_t s = j->ctx->stream;
     fgk::PlainElecArgs a{};
     a.g = j->g;
     a.ddd = j->distDepDiel;
@@ -805,7 +814,7 @@ void destroyJob(fg_job *j)
 int timeShape(fg_job *j, int px, int pz, int tile, int bxShift, bool comp, float &ms)
 {
     fg_ctx *ctx = j->ctx;
-    auto go = [&]() { launchElecShape(j, px, pz, tile, bxShift, comp); };
+    auto go = [&]() { launchElecShape(j, px, pz, tile, bxShift, comp, ctx->stream); };
     go();  // warm-up
     FG_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
     go();
@@ -973,11 +982,11 @@ struct fg_batch {
     int MM = 4, KH = 0, px = 1, pz = 4;
     bool pointAng = false, clamp = false, typed = true;
     int sharedTableLen = 0, maxTableLen = 0;
-    int cutItems = 0, elecItems = 0, elecWGs = 0;
+    int cutItems = 0, elecItems = 0, elecWGs = 0, farItems = 0;
     fgk::ListArgs *dList = nullptr;
     fgk::CutoffArgs *dCut = nullptr;
     fgk::ElecArgs *dElec = nullptr;
-    int32_t *dCutBase = nullptr, *dElecBase = nullptr, *dCounters = nullptr;
+    int32_t *dCutBase = nullptr, *dElecBase = nullptr, *dFarBase = nullptr, *dCounters = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -995,7 +1004,7 @@ void destroyBatch(fg_batch *b)
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (void *p : {static_cast<void *>(b->dList), static_cast<void *>(b->dCut), static_cast<void *>(b->dElec),
                     static_cast<void *>(b->dCutBase), static_cast<void *>(b->dElecBase),
-                    static_cast<void *>(b->dCounters)})
+                    static_cast<void *>(b->dFarBase), static_cast<void *>(b->dCounters)})
         if (p) (void)hipFree(p);
     delete b;
 }
@@ -1046,35 +1055,54 @@ int launchCutoffBatchKH(fg_batch *b, bool launch)
 }
 
 template <int PX, int PZ, bool kComp, bool kClamp>
-int launchElecBatch(fg_batch *b, bool launch)
+int launchElecBatch(fg_batch *b, bool launch, hipStream_t s)
 {
     auto k = fgk::k_elec_batch<PX, PZ, kComp, kClamp>;
     const size_t lds = size_t(b->sharedTableLen > 0 ? b->sharedTableLen : b->maxTableLen) * sizeof(float);
     if (!b->elecWGs)
         if (int rc = persistentWGs(b, k, fgk::kElecThreads, lds, b->elecItems, b->elecWGs)) return rc;
     if (!launch) return FG_OK;
-    hipLaunchKernelGGL(k, dim3(b->elecWGs), dim3(fgk::kElecThreads), lds, b->ctx->stream, b->dElec, b->dElecBase,
+    hipLaunchKernelGGL(k, dim3(b->elecWGs), dim3(fgk::kElecThreads), lds, s, b->dElec, b->dElecBase,
                        int(b->jobs.size()), b->dCounters + 1, b->sharedTableLen);
     return FG_OK;
 }
 
 template <int PX, int PZ>
-int launchElecBatchV(fg_batch *b, bool launch)
+int launchElecBatchV(fg_batch *b, bool launch, hipStream_t s)
 {
     if (b->comp)
-        return b->clamp ? launchElecBatch<PX, PZ, true, true>(b, launch)
-                        : launchElecBatch<PX, PZ, true, false>(b, launch);
-    return b->clamp ? launchElecBatch<PX, PZ, false, true>(b, launch) : launchElecBatch<PX, PZ, false, false>(b, launch);
+        return b->clamp ? launchElecBatch<PX, PZ, true, true>(b, launch, s)
+                        : launchElecBatch<PX, PZ, true, false>(b, launch, s);
+    return b->clamp ? launchElecBatch<PX, PZ, false, true>(b, launch, s)
+                    : launchElecBatch<PX, PZ, false, false>(b, launch, s);
 }
 
-// The batch's launch sequence on the context's stream; `mid` (if any) is
-// recorded between the cutoff and electrostatic phases.  With launch false,
-// only sizes the persistent grids.
+// The batch's launch sequence.  The electrostatic kernel runs on the
+// context's side stream, forked from its stream after the counters are
+// cleared and joined before the final e = near + far sum, concurrently with
+// the cutoff phase (under capture the fork and join become graph edges).
+// `mid` (if any) is recorded after the cutoff phase and e0/e1 bracket the
+// electrostatic kernel; neither is allowed under capture.  With launch
+// false, only sizes the persistent grids.
 int enqueueBatch(fg_batch *b, hipEvent_t mid, bool launch = true)
 {
-    hipStream_t s = b->ctx->stream;
+    fg_ctx *ctx = b->ctx;
+    hipStream_t s = ctx->stream, side = ctx->side;
     const int n = int(b->jobs.size());
     if (launch) hipLaunchKernelGGL(fgk::k_fill_i32, dim3(1), dim3(256), 0, s, b->dCounters, 2, 0);
+    if (b->elec) {
+        if (launch) {
+            FG_HIP(hipEventRecord(ctx->fork, s));
+            FG_HIP(hipStreamWaitEvent(side, ctx->fork, 0));
+            if (mid) FG_HIP(hipEventRecord(ctx->e0, side));
+        }
+        int rc = b->px == 2 ? (b->pz == 8 ? launchElecBatchV<2, 8>(b, launch, side)
+                                          : launchElecBatchV<2, 4>(b, launch, side))
+                            : (b->pz == 8 ? launchElecBatchV<1, 8>(b, launch, side)
+                                          : launchElecBatchV<1, 4>(b, launch, side));
+        if (rc) return rc;
+        if (launch) FG_HIP(hipEventRecord(ctx->e1, side));
+    }
     if (b->aff || b->near || b->fmap) {
         if (launch) {
             // list offsets are fixed at job creation: fill (and sort) only
@@ -1097,10 +1125,10 @@ int enqueueBatch(fg_batch *b, hipEvent_t mid, bool launch = true)
         if (rc) return rc;
     }
     if (mid && launch) FG_HIP(hipEventRecord(mid, s));
-    if (b->elec) {
-        int rc = b->px == 2 ? (b->pz == 8 ? launchElecBatchV<2, 8>(b, launch) : launchElecBatchV<2, 4>(b, launch))
-                            : (b->pz == 8 ? launchElecBatchV<1, 8>(b, launch) : launchElecBatchV<1, 4>(b, launch));
-        if (rc) return rc;
+    if (b->elec && launch) {
+        FG_HIP(hipStreamWaitEvent(s, ctx->e1, 0));
+        if (b->near)  // e = near (k_cutoff_batch) + far (k_elec_batch)
+            hipLaunchKernelGGL(fgk::k_add_far_batch, dim3(b->farItems), dim3(256), 0, s, b->dElec, b->dFarBase, n);
     }
     FG_HIP(hipGetLastError());
     return FG_OK;
@@ -1153,7 +1181,7 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
     std::vector<fgk::ListArgs> la;
     std::vector<fgk::CutoffArgs> ca;
     std::vector<fgk::ElecArgs> ea;
-    std::vector<int32_t> cutBase(1, 0), elecBase(1, 0);
+    std::vector<int32_t> cutBase(1, 0), elecBase(1, 0), farBase(1, 0);
     for (fg_job *j : b->jobs) {
         la.push_back(listArgs(j));
         ca.push_back(cutoffArgs(j, b->near ? 1 : 0));
@@ -1168,14 +1196,19 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
         if (c > INT32_MAX || el > INT32_MAX) return fgc::fail(FG_E_ARG, "fg_batch_create: too many work items");
         cutBase.push_back(int32_t(c));
         elecBase.push_back(int32_t(el));
+        const int64_t fa = int64_t(farBase.back()) + (j->P + 255) / 256;
+        if (fa > INT32_MAX) return fgc::fail(FG_E_ARG, "fg_batch_create: too many work items");
+        farBase.push_back(int32_t(fa));
     }
     b->cutItems = cutBase.back();
     b->elecItems = elecBase.back();
+    b->farItems = farBase.back();
     if (int rc = batchUpload(b->dList, la)) return rc;
     if (int rc = batchUpload(b->dCut, ca)) return rc;
     if (int rc = batchUpload(b->dElec, ea)) return rc;
     if (int rc = batchUpload(b->dCutBase, cutBase)) return rc;
     if (int rc = batchUpload(b->dElecBase, elecBase)) return rc;
+    if (int rc = batchUpload(b->dFarBase, farBase)) return rc;
     if (int rc = batchUpload(b->dCounters, std::vector<int32_t>{0, 0})) return rc;
     // persistent grid sizes (queried outside any capture)
     if (int rc = enqueueBatch(b, nullptr, false)) return rc;
@@ -1216,13 +1249,14 @@ int fg_create(int device, fg_ctx **out)
     FG_HIP(hipSetDevice(device));
     fg_ctx *c = new fg_ctx();
     c->device = device;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) {
+        fg_destroy(c);
         return fgc::fail(FG_E_DEVICE, "fg_create: hipStreamCreate failed");
     }
-    for (hipEvent_t &e : c->ev)
-        if (hipEventCreate(&e) != hipSuccess) {
-            delete c;
+    for (hipEvent_t *e : {&c->ev[0], &c->ev[1], &c->ev[2], &c->ev[3], &c->fork, &c->e0, &c->e1})
+        if (hipEventCreate(e) != hipSuccess) {
+            fg_destroy(c);
             return fgc::fail(FG_E_DEVICE, "fg_create: hipEventCreate failed");
         }
     if (hipDeviceGetAttribute(&c->numCUs, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
@@ -1237,12 +1271,14 @@ void fg_destroy(fg_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     for (fg_job *j : ctx->jobs) j->ctx = nullptr;
     for (fg_batch *b : ctx->batches) b->ctx = nullptr;
     for (fg_mapset *m : ctx->mapsets) m->ctx = nullptr;
-    for (hipEvent_t e : ctx->ev)
+    for (hipEvent_t e : {ctx->ev[0], ctx->ev[1], ctx->ev[2], ctx->ev[3], ctx->fork, ctx->e0, ctx->e1})
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     delete ctx;
 }
 
@@ -1354,6 +1390,7 @@ int jobBuildHost(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
     if (!rc) rc = buildElecTiles(j, *d, gridMaxAbs);
     if (!rc) rc = buildPlain(j, *d);
     if (!rc) rc = devAlloc(j, j->dMaps, size_t(j->M + 2) * size_t(j->P));
+    if (!rc) rc = devAlloc(j, j->dFar, size_t(j->P));
     if (!rc && j->floating) rc = devAlloc(j, j->dFmap, size_t(j->P));
     if (rc) {
         destroyJob(j);
@@ -1471,6 +1508,19 @@ int fg_job_run(fg_ctx *ctx, fg_job *j, unsigned what, int precision)
     const bool near = elec && precision == FG_PREC_COMPENSATED;
     const bool fmap = (what & FG_RUN_FLOATING) && j->floating;
     FG_HIP(hipEventRecord(ctx->ev[0], s));
+    // the electrostatic kernels run on the side stream, concurrently with the
+    // cutoff phase: they write e (or, compensated, the far part) and read
+    // nothing the cutoff kernels write
+    if (elec) {
+        FG_HIP(hipStreamWaitEvent(ctx->side, ctx->ev[0], 0));
+        FG_HIP(hipEventRecord(ctx->e0, ctx->side));
+        if (precision == FG_PREC_PLAIN) {
+            if (int rc = runPlainElec(j, ctx->side)) return rc;
+        } else if (int rc = launchElec(j, precision, ctx->side)) {
+            return rc;
+        }
+        FG_HIP(hipEventRecord(ctx->e1, ctx->side));
+    }
     if (precision == FG_PREC_PLAIN) {
         if ((aff && j->hasGeom) || fmap)
             if (int rc = runLists(j, false)) return rc;
@@ -1478,15 +1528,7 @@ int fg_job_run(fg_ctx *ctx, fg_job *j, unsigned what, int precision)
             if (int rc = runPlainAffinity(j)) return rc;
         if (fmap)
             if (int rc = launchFmap(j)) return rc;
-        FG_HIP(hipEventRecord(ctx->ev[1], s));
-        if (elec)
-            if (int rc = runPlainElec(j)) return rc;
-        FG_HIP(hipEventRecord(ctx->ev[2], s));
-        ctx->haveTiming = true;
-        ctx->splitTiming = true;
-        return FG_OK;
-    }
-    if (aff || near || fmap) {
+    } else if (aff || near || fmap) {
         const bool affMaps = aff && j->hasGeom;
         if (int rc = runLists(j, affMaps)) return rc;
         if (aff || near)
@@ -1495,11 +1537,17 @@ int fg_job_run(fg_ctx *ctx, fg_job *j, unsigned what, int precision)
             if (int rc = launchFmap(j)) return rc;
     }
     FG_HIP(hipEventRecord(ctx->ev[1], s));
-    if (elec)
-        if (int rc = launchElec(j, precision)) return rc;
+    if (elec) {
+        FG_HIP(hipStreamWaitEvent(s, ctx->e1, 0));
+        if (near)  // e = near (k_cutoff) + far (k_elec)
+            hipLaunchKernelGGL(fgk::k_add_far, dim3(unsigned((j->P + 255) / 256)), dim3(256), 0, s, j->P, j->dFar,
+                               j->dMaps + int64_t(j->M) * j->P);
+        FG_HIP(hipGetLastError());
+    }
     FG_HIP(hipEventRecord(ctx->ev[2], s));
     ctx->haveTiming = true;
     ctx->splitTiming = true;
+    ctx->elecTimed = elec;
     return FG_OK;
 }
 
@@ -1517,8 +1565,10 @@ int fg_last_timing(fg_ctx *ctx, fg_timing *t)
     FG_HIP(hipEventSynchronize(ctx->ev[2]));
     float a = -1, b = -1, c = 0;
     if (ctx->splitTiming) {
+        // the phases overlap: e0..e1 ran on the side stream within ev[0]..ev[2]
         FG_HIP(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]));
-        FG_HIP(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
+        b = 0;
+        if (ctx->elecTimed) FG_HIP(hipEventElapsedTime(&b, ctx->e0, ctx->e1));
     }
     FG_HIP(hipEventElapsedTime(&c, ctx->ev[0], ctx->ev[2]));
     t->affinity_ms = a;
@@ -1644,6 +1694,7 @@ int fg_batch_launch(fg_ctx *ctx, fg_batch *b)
     } else {
         if (int rc = enqueueBatch(b, ctx->ev[1])) return rc;
         ctx->splitTiming = true;
+        ctx->elecTimed = b->elec;
     }
     FG_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
     ctx->haveTiming = true;

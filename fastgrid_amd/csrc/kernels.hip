@@ -995,7 +995,7 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
             if (iz >= a.g.nz) break;
             const int q = i * PZ + k;
             const int64_t p = (int64_t(iz) * a.g.ny + iy) * a.g.nx + ix + i;
-            if (kComp) a.e[p] = a.e[p] + (double(hi[q]) + double(lo[q]));
+            if (kComp) a.far[p] = double(hi[q]) + double(lo[q]);
             else a.e[p] = double(part[q]);
         }
     }
@@ -1009,6 +1009,14 @@ __global__ __launch_bounds__(kElecThreads) void k_elec(ElecArgs a)
     loadEps(eps, a.eps32, a.tableLen);
     __syncthreads();
     elecBrick<PX, PZ, kComp, kClamp>(a, int(blockIdx.x), eps);
+}
+
+// The compensated e-map: the near part k_cutoff wrote plus the far part
+// k_elec wrote (the two kernels run concurrently into separate buffers).
+__global__ __launch_bounds__(256) void k_add_far(int64_t P, const double *far, double *e)
+{
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (p < P) e[p] = e[p] + far[p];
 }
 
 // ------------------------------------------------------------------------
@@ -1110,6 +1118,16 @@ __global__ __launch_bounds__(kListThreads) void k_type_sort_batch(const ListArgs
     const int item = int(blockIdx.x);
     const int job = uniform(jobOfItem(base, nJobs, item));
     typeSortBrick(loadConst(jobs + job), item - base[job], cnt, runStart, wcnt);
+}
+
+// k_add_far for every job: items are (job, 256-point chunk) pairs.
+__global__ __launch_bounds__(256) void k_add_far_batch(const ElecArgs *jobs, const int32_t *base, int nJobs)
+{
+    const int item = int(blockIdx.x);
+    const int job = uniform(jobOfItem(base, nJobs, item));
+    const FG_CONST ElecArgs *a = constView(jobs + job);
+    const int64_t p = int64_t(item - base[job]) * 256 + threadIdx.x;
+    if (p < a->P) a->e[p] = a->e[p] + a->far[p];
 }
 
 // ------------------------------------------------------------------------

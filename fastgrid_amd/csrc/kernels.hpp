@@ -131,7 +131,9 @@ struct ElecArgs {
     const double *tiles64;   // [nAtoms][3] FP64 coordinates (for the exact mask)
     const float *subBox;     // [nAtoms/kSubTile][6] lo xyz, hi xyz (scaled)
     const float *eps32;      // [tableLen]
-    double *e;               // [P] electrostatic map (near part already there when compensated)
+    double *e;               // [P] electrostatic map (the FP32 form writes it)
+    double *far;             // [P] far-field sums of the compensated form, added to the near part
+                             // k_cutoff wrote in e by k_add_far
     float farCut2;           // (802)^2: sub-tiles farther than 8.02 A hold only far pairs
     float r2Lo, r2Hi;        // ambiguity band around 64e4 in scaled r^2
     float spacing100;        // 100 * spacing

@@ -195,7 +195,11 @@ int fg_job_create(fg_ctx *ctx, const fg_job_desc *desc, fg_job **out);
  * job.  No reference counterpart (one receptor per process there). */
 int fg_job_create_many(fg_ctx *ctx, const fg_job_desc *descs, int n, int threads, fg_job **out);
 void fg_job_destroy(fg_job *job);
-/* Enqueue the computation on the context's stream; returns immediately. */
+/* Enqueue the computation on the context's stream; returns immediately.  The
+ * electrostatic kernel runs on the context's second stream, concurrently with
+ * the cutoff kernels (the reference likewise overlaps its GPU e-map with the
+ * CPU affinity maps); the context's stream waits for it, so work enqueued
+ * after this call, fg_sync and fg_job_fetch see the finished maps. */
 int fg_job_run(fg_ctx *ctx, fg_job *job, unsigned what, int precision);
 /* Wait for everything enqueued on the context. */
 int fg_sync(fg_ctx *ctx);
@@ -263,12 +267,14 @@ void fg_batch_destroy(fg_batch *batch);
 int fg_batch_run(fg_ctx *ctx, fg_job *const *jobs, int n_jobs, unsigned what, int precision);
 
 /* Device time of the kernels of the last fg_job_run / fg_batch_launch on ctx,
- * measured with HIP events on the context's stream (ms).  A batch replayed
- * from its graph reports total_ms only (elec_ms = affinity_ms = -1). */
+ * measured with HIP events (ms).  The electrostatic kernels run on a second
+ * stream of the context, concurrently with the cutoff kernels, so elec_ms +
+ * affinity_ms exceeds total_ms when they overlap.  A batch replayed from its
+ * graph reports total_ms only (elec_ms = affinity_ms = -1). */
 typedef struct {
-    double elec_ms;      /* electrostatic kernel(s) */
+    double elec_ms;      /* electrostatic kernel(s), on the second stream; 0 if none ran */
     double affinity_ms;  /* cutoff kernel(s), incl. the per-brick atom lists */
-    double total_ms;     /* first to last event */
+    double total_ms;     /* first to last event, incl. the final e = near + far sum */
 } fg_timing;
 int fg_last_timing(fg_ctx *ctx, fg_timing *t);
 
