@@ -1018,11 +1018,12 @@ int batchUpload(T *&dst, const std::vector<T> &src)
 }
 
 template <class K>
-int persistentWGs(fg_batch *b, K kernel, int threads, size_t lds, int items, int &out)
+// reserve: workgroup slots per CU left to kernels running concurrently
+int persistentWGs(fg_batch *b, K kernel, int threads, size_t lds, int items, int &out, int reserve = 0)
 {
     int perCU = 0;
     FG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void *>(kernel), threads, lds));
-    out = std::max(1, std::min(items, std::max(perCU, 1) * b->ctx->numCUs));
+    out = std::max(1, std::min(items, std::max(perCU - reserve, 1) * b->ctx->numCUs));
     return FG_OK;
 }
 
@@ -1059,8 +1060,12 @@ int launchElecBatch(fg_batch *b, bool launch, hipStream_t s)
 {
     auto k = fgk::k_elec_batch<PX, PZ, kComp, kClamp>;
     const size_t lds = size_t(b->sharedTableLen > 0 ? b->sharedTableLen : b->maxTableLen) * sizeof(float);
+    // a persistent grid would hold every slot until the queue is empty: keep
+    // one per CU for the concurrent lists and cutoff kernels
     if (!b->elecWGs)
-        if (int rc = persistentWGs(b, k, fgk::kElecThreads, lds, b->elecItems, b->elecWGs)) return rc;
+        if (int rc = persistentWGs(b, k, fgk::kElecThreads, lds, b->elecItems, b->elecWGs,
+                                   (b->aff || b->near || b->fmap) ? 1 : 0))
+            return rc;
     if (!launch) return FG_OK;
     hipLaunchKernelGGL(k, dim3(b->elecWGs), dim3(fgk::kElecThreads), lds, s, b->dElec, b->dElecBase,
                        int(b->jobs.size()), b->dCounters + 1, b->sharedTableLen);
