@@ -73,6 +73,7 @@ struct fg_mapset {
     fg_map_grid g{};
     int nslots = 0;
     float *d = nullptr;
+    float *cells = nullptr;  // corner-packed copy (k_pack_cells), 8x the floats
 };
 
 struct fg_job {
@@ -1857,9 +1858,26 @@ int newMapset(fg_ctx *ctx, const fg_map_grid *g, int nslots, fg_mapset **out)
         delete ms;
         return fgc::fail(FG_E_DEVICE, "fg_mapset: out of device memory");
     }
+    // the corner-packed copy the interpolation reads (3 x 32 bytes per atom
+    // instead of 24 scattered floats)
+    if (hipMalloc(&ms->cells, n * 8 * sizeof(float)) != hipSuccess) {
+        (void)hipFree(ms->d);
+        delete ms;
+        return fgc::fail(FG_E_DEVICE, "fg_mapset: out of device memory (the map set and its 8x corner-packed copy)");
+    }
     ctx->mapsets.push_back(ms);
     *out = ms;
     return FG_OK;
+}
+
+// Fills the corner-packed copy from the plain layout.
+hipError_t packCells(fg_ctx *ctx, fg_mapset *ms)
+{
+    const int64_t n = int64_t(ms->g.num_points1[0]) * ms->g.num_points1[1] * ms->g.num_points1[2] * ms->nslots;
+    hipLaunchKernelGGL(fgk::k_pack_cells, dim3(unsigned((n + fgk::kInterpThreads - 1) / fgk::kInterpThreads)),
+                       dim3(fgk::kInterpThreads), 0, ctx->stream, ms->d, ms->g.num_points1[0], ms->g.num_points1[1],
+                       ms->g.num_points1[2], ms->nslots, ms->cells);
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -1874,9 +1892,12 @@ int fg_mapset_create(fg_ctx *ctx, const fg_map_grid *g, int nslots, const float 
     fg_mapset *ms = nullptr;
     if (int rc = newMapset(ctx, g, nslots, &ms)) return rc;
     const size_t n = size_t(g->num_points1[0]) * g->num_points1[1] * g->num_points1[2] * size_t(nslots);
-    if (hipMemcpy(ms->d, values, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+    hipError_t e = hipMemcpy(ms->d, values, n * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = packCells(ctx, ms);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
         fg_mapset_destroy(ms);
-        return fgc::fail(FG_E_DEVICE, "fg_mapset_create: upload failed");
+        return fgc::fail(FG_E_DEVICE, std::string("fg_mapset_create: ") + hipGetErrorString(e));
     }
     *out = ms;
     return FG_OK;
@@ -1896,6 +1917,7 @@ int fg_mapset_from_job(fg_ctx *ctx, fg_job *j, const fg_map_grid *g, fg_mapset *
     hipLaunchKernelGGL(fgk::k_export_mapset, dim3(unsigned((j->P + fgk::kInterpThreads - 1) / fgk::kInterpThreads)),
                        dim3(fgk::kInterpThreads), 0, ctx->stream, j->dMaps, j->P, V, ms->d);
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = packCells(ctx, ms);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         fg_mapset_destroy(ms);
@@ -1932,6 +1954,7 @@ void fg_mapset_destroy(fg_mapset *ms)
         forget(ms->ctx->mapsets, ms);
     }
     if (ms->d) (void)hipFree(ms->d);
+    if (ms->cells) (void)hipFree(ms->cells);
     delete ms;
 }
 
@@ -2002,7 +2025,7 @@ int fg_trilinterp(fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int np
     a.npose = npose;
     a.elecSlot = lig->elec_slot;
     a.desolvSlot = lig->desolv_slot;
-    a.map = ms->d;
+    a.cells = ms->cells;
     a.xyz = reinterpret_cast<const float *>(b + oXyz);
     a.type = reinterpret_cast<const int32_t *>(b + oType);
     a.charge = reinterpret_cast<const float *>(b + oQ);

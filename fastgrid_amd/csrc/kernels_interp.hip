@@ -7,7 +7,8 @@
 // contraction in this file), so every output equals the reference's bit for
 // bit.  Totals are the reference's running sums in atom order: one lane per
 // (pose, atom) computes the atom's two contributions into LDS, one lane per
-// pose adds them in order.
+// pose adds them in order.  The map set is read from a corner-packed copy
+// (k_pack_cells): the same floats, a cell's 8 corners of a slot side by side.
 #pragma clang fp contract(off)
 
 #include "kernels.hpp"
@@ -37,11 +38,35 @@ __global__ __launch_bounds__(kInterpThreads) void k_export_mapset(const double *
     for (int m = 0; m < V; ++m) out[p * V + m] = autodockValue(maps[int64_t(m) * P + p]);
 }
 
+// Corner-packed copy of a map set: for every cell (the grid point with the
+// lowest indices of its 8 corners) and slot, the 8 corner values in the order
+// u + 2v + 4w, 0 for a corner past the grid (as AutoDock's zero-padded array
+// reads).  One lane per (cell, slot).
+__global__ __launch_bounds__(kInterpThreads) void k_pack_cells(const float *map, int nx, int ny, int nz, int V,
+                                                               float *cells)
+{
+    const int64_t q = int64_t(blockIdx.x) * kInterpThreads + threadIdx.x;
+    const int64_t P = int64_t(nx) * ny * nz;
+    if (q >= P * V) return;
+    const int s = int(q % V);
+    const int64_t p = q / V;
+    const int i = int(p % nx), j = int((p / nx) % ny), k = int(p / (int64_t(nx) * ny));
+    float v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int u = i + (c & 1), w = j + ((c >> 1) & 1), z = k + (c >> 2);
+        v[c] = (u < nx && w < ny && z < nz) ? map[((int64_t(z) * ny + w) * nx + u) * V + s] : 0.f;
+    }
+    float4 *out = reinterpret_cast<float4 *>(cells + (q << 3));
+    out[0] = make_float4(v[0], v[1], v[2], v[3]);
+    out[1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+
 struct InterpArgs {
     double lo[3], hi[3], center[3], inv;
     int32_t nx, ny, nz, V;       // points per axis, slots
     int32_t natoms, npose, elecSlot, desolvSlot;
-    const float *map;            // [nz][ny][nx][V]
+    const float *cells;          // [nz][ny][nx][V][8] corner values per cell (k_pack_cells)
     const float *xyz;            // [npose][natoms][3]
     const int32_t *type;         // [natoms]
     const float *charge, *absq;  // [natoms]
@@ -77,53 +102,36 @@ __device__ __forceinline__ void interpAtom(const InterpArgs &a, int64_t idx, int
     }
     const int t = a.type[i];
     const double u = (double(fx) - a.lo[0]) * a.inv;
-    const int u0 = int(u), u1 = u0 + 1;
+    const int u0 = int(u);
     const double p0u = u - double(u0), p1u = 1.0 - p0u;
     const double v = (double(fy) - a.lo[1]) * a.inv;
-    const int v0 = int(v), v1 = v0 + 1;
+    const int v0 = int(v);
     const double p0v = v - double(v0), p1v = 1.0 - p0v;
     const double w = (double(fz) - a.lo[2]) * a.inv;
-    const int w0 = int(w), w1 = w0 + 1;
+    const int w0 = int(w);
     const double p0w = w - double(w0), p1w = 1.0 - p0w;
-    // slot value at a corner; 0 one past the grid's last point
-    auto val = [&](int wz, int vy, int ux, int s) -> double {
-        if (ux >= a.nx || vy >= a.ny || wz >= a.nz) return 0.0;
-        return double(a.map[((int64_t(wz) * a.ny + vy) * a.nx + ux) * a.V + s]);
-    };
+    // the cell's 8 corner values of each slot side by side (corner u + 2v +
+    // 4w, 0 one past the grid's last point): 3 x 32 bytes per atom
+    const int64_t cell = (int64_t(w0) * a.ny + v0) * a.nx + u0;
+    const float4 *c = reinterpret_cast<const float4 *>(a.cells + ((cell * a.V) << 3));
     const int es = a.elecSlot, ds = a.desolvSlot;
+    const float4 e0 = c[2 * es], e1 = c[2 * es + 1];
+    const float4 m0 = c[2 * t], m1 = c[2 * t + 1];
+    const float4 d0 = c[2 * ds], d1 = c[2 * ds + 1];
+    // corner weights as the reference forms them, p1u * p1v * p1w first
+    const double wt[8] = {p1u * p1v * p1w, p0u * p1v * p1w, p1u * p0v * p1w, p0u * p0v * p1w,
+                          p1u * p1v * p0w, p0u * p1v * p0w, p1u * p0v * p0w, p0u * p0v * p0w};
+    const float ev[8] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
+    const float mv[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+    const float dv[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+    // the three running sums over the corners in the reference's order
     double e = 0, m = 0, d = 0;
-    e += p1u * p1v * p1w * val(w0, v0, u0, es);
-    m += p1u * p1v * p1w * val(w0, v0, u0, t);
-    d += p1u * p1v * p1w * val(w0, v0, u0, ds);
-
-    d += p0u * p1v * p1w * val(w0, v0, u1, ds);
-    m += p0u * p1v * p1w * val(w0, v0, u1, t);
-    e += p0u * p1v * p1w * val(w0, v0, u1, es);
-
-    e += p1u * p0v * p1w * val(w0, v1, u0, es);
-    m += p1u * p0v * p1w * val(w0, v1, u0, t);
-    d += p1u * p0v * p1w * val(w0, v1, u0, ds);
-
-    d += p0u * p0v * p1w * val(w0, v1, u1, ds);
-    m += p0u * p0v * p1w * val(w0, v1, u1, t);
-    e += p0u * p0v * p1w * val(w0, v1, u1, es);
-
-    e += p1u * p1v * p0w * val(w1, v0, u0, es);
-    m += p1u * p1v * p0w * val(w1, v0, u0, t);
-    d += p1u * p1v * p0w * val(w1, v0, u0, ds);
-
-    d += p0u * p1v * p0w * val(w1, v0, u1, ds);
-    m += p0u * p1v * p0w * val(w1, v0, u1, t);
-    e += p0u * p1v * p0w * val(w1, v0, u1, es);
-
-    e += p1u * p0v * p0w * val(w1, v1, u0, es);
-    m += p1u * p0v * p0w * val(w1, v1, u0, t);
-    d += p1u * p0v * p0w * val(w1, v1, u0, ds);
-
-    d += p0u * p0v * p0w * val(w1, v1, u1, ds);
-    m += p0u * p0v * p0w * val(w1, v1, u1, t);
-    e += p0u * p0v * p0w * val(w1, v1, u1, es);
-
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        e += wt[q] * double(ev[q]);
+        m += wt[q] * double(mv[q]);
+        d += wt[q] * double(dv[q]);
+    }
     addE = e * double(a.charge[i]);
     addM = m + d * double(a.absq[i]);
     if (a.elec) a.elec[idx] = float(addE);
