@@ -351,7 +351,9 @@ int fg_read_map(const char *map_path, const fg_map_grid *g, int slot, int nslots
 int fg_prep_map_grid(const fg_prep *prep, fg_map_grid *out);
 
 /* A map set in HBM: floats [z][y][x][slot], AutoDock's layout
- * (AD/map_declare.h) without its padding. */
+ * (AD/map_declare.h) without its padding, plus a corner-packed copy for
+ * fg_trilinterp (a cell's 8 corner values of each slot side by side, 8x the
+ * floats) built when the set is made. */
 typedef struct fg_mapset fg_mapset;
 int fg_mapset_create(fg_ctx *ctx, const fg_map_grid *g, int nslots, const float *values, fg_mapset **out);
 /* Binary hand-off of a job's maps with no text in between: slots 0..n-1 are
