@@ -1018,8 +1018,9 @@ int batchUpload(T *&dst, const std::vector<T> &src)
     return FG_OK;
 }
 
+// Workgroups of a persistent grid: the kernel's occupancy per CU, less
+// `reserve` slots per CU left to kernels running concurrently, x CUs.
 template <class K>
-// reserve: workgroup slots per CU left to kernels running concurrently
 int persistentWGs(fg_batch *b, K kernel, int threads, size_t lds, int items, int &out, int reserve = 0)
 {
     int perCU = 0;

@@ -624,8 +624,8 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
 }
 
 template <int MM, int KH, bool kAffinity, bool kPointAng, bool kTyped>
-__global__ __launch_bounds__(kCutThreads) __attribute__((amdgpu_waves_per_eu(cutWaves<MM, KH, kAffinity, kTyped>()))) void k_cutoff(
-    CutoffArgs a)
+__global__ __launch_bounds__(kCutThreads)
+__attribute__((amdgpu_waves_per_eu(cutWaves<MM, KH, kAffinity, kTyped>()))) void k_cutoff(CutoffArgs a)
 {
     __shared__ CutoffLds<MM, kAffinity, kTyped> L;
     // XCD-aware brick order: consecutive workgroups go to different XCDs, so
@@ -1093,7 +1093,8 @@ __global__ __launch_bounds__(kListThreads) void k_brick_lists_batch(const ListAr
 // queue order (largest jobs first), so the hardware's in-order dispatch
 // balances the load.
 template <int MM, int KH, bool kAffinity, bool kPointAng, bool kTyped>
-__global__ __launch_bounds__(kCutThreads) __attribute__((amdgpu_waves_per_eu(cutWaves<MM, KH, kAffinity, kTyped>()))) void k_cutoff_batch(
+__global__ __launch_bounds__(kCutThreads)
+__attribute__((amdgpu_waves_per_eu(cutWaves<MM, KH, kAffinity, kTyped>()))) void k_cutoff_batch(
     const CutoffArgs *jobs, const int32_t *base, int nJobs)
 {
     __shared__ CutoffLds<MM, kAffinity, kTyped> L;
