@@ -110,7 +110,7 @@ struct fg_job {
     // k_elec shape: points per lane along x and z, atoms per tile, lanes
     // along x (defaults; the autotuner may change them)
     int px = 1, pz = 4, tile = 64, bxShift = 4;
-    float farCut2 = 0, r2Lo = 0, r2Hi = 0, spacing100 = 0;
+    float farCut2 = 0, lastCut2 = 0, tableCut2 = 0, r2Lo = 0, r2Hi = 0, spacing100 = 0;
     // plain form (FP32 throughout)
     float4 *dPlainQ = nullptr;          // (x, y, z, w) per atom, index order
     fgk::PlainAtom *dPlainAtoms = nullptr;
@@ -366,6 +366,8 @@ fgk::ElecArgs elecArgs(const fg_job *j, int px, int pz, int tile, int bxShift)
     a.e = j->dMaps + int64_t(j->M) * j->P;
     a.far = j->dFar;
     a.farCut2 = j->farCut2;
+    a.lastCut2 = j->lastCut2;
+    a.tableCut2 = j->tableCut2;
     a.r2Lo = j->r2Lo;
     a.r2Hi = j->r2Hi;
     a.spacing100 = j->spacing100;
@@ -519,6 +521,14 @@ int buildElecTiles(fg_job *j, const fg_job_desc &d, double gridMaxAbs)
         j->tableLen = fgk::kTableLen;
         j->clamp = true;
     }
+    // Sub-tiles whose boxes lie farther than lastCut from the brick only read
+    // the last table bin (100 r > 8191 with the same 2 + 4 Ed margin as the far
+    // cut); sub-tiles whose farthest corner lies within tableCut never index
+    // past the table (the clamped variant skips its clamp for them).
+    const double lastCut = (fgk::kTableLen + 2.0) + 4 * Ed;
+    j->lastCut2 = float(lastCut * lastCut);
+    const double tableCut = j->clamp ? (j->tableLen - 3.0) - 4 * Ed : 1e18;
+    j->tableCut2 = float(tableCut * tableCut);
     eps32.resize(size_t(j->tableLen));
     for (int k = 0; k < j->tableLen; ++k) {
         double v = d.dist_dep_diel ? eps[std::min(k, fgk::kTableLen - 1)] : 1.0;

@@ -6,6 +6,8 @@
 // are written explicitly with __builtin_fmaf.
 #pragma clang fp contract(off)
 
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace fgk {
@@ -852,8 +854,11 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
 #pragma unroll
     for (int q = 0; q < NP; ++q) hi[q] = lo[q] = part[q] = 0.f;
 
-    // far field: every lane's pair is beyond 8 A (box tests at 8.02 A)
-    auto farAtoms = [&](int c0, int c1) {
+    // far field: every lane's pair is beyond 8 A (box tests at 8.02 A).
+    // Clamp: bins past the table's end are clamped to its last entry (the
+    // clamped kernel variant, for tiles that may reach beyond the table).
+    auto farAtoms = [&](int c0, int c1, auto clampTag) {
+        constexpr bool kClampHere = kClamp && decltype(clampTag)::value;
 #pragma unroll 2
         for (int j = c0; j < c1; ++j) {
             const float4 at = make_float4(tiles[4 * j], tiles[4 * j + 1], tiles[4 * j + 2], tiles[4 * j + 3]);
@@ -879,10 +884,10 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
                     // so its bit pattern is 2^23's plus round(100 r - 1/2)
                     // (far pairs have 100 r > 800)
                     bin[q] = __float_as_uint(__builtin_fmaf(r2, ri[q], kFloorBias));
-                    if (kClamp) bin[q] = min(bin[q], clampBits);
+                    if (kClampHere) bin[q] = min(bin[q], clampBits);
                 } else {
                     bin[q] = unsigned(r2 * ri[q]);
-                    if (kClamp) bin[q] = min(bin[q], clampMax);
+                    if (kClampHere) bin[q] = min(bin[q], clampMax);
                 }
             }
 #pragma unroll
@@ -938,36 +943,85 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
             }
         }
     };
-    // sub-tile q0's box beyond farCut of the box (lo, hi)
-    auto farFrom = [&](int q0, const float *lo, const float *hi) {
+    // beyond the last table bin: every lane's pair has 100 r > 8192, where
+    // the table holds its last entry, so the bin, its address and the read
+    // drop out; the rest is the far loop's arithmetic, so the bits are the same
+    const float tLast = eps[kTableLen - 1];
+    auto lastBinAtoms = [&](int c0, int c1) {
+#pragma unroll 2
+        for (int j = c0; j < c1; ++j) {
+            const float4 at = make_float4(tiles[4 * j], tiles[4 * j + 1], tiles[4 * j + 2], tiles[4 * j + 3]);
+            const float dy = at.y - gy;
+            const float dy2 = dy * dy;
+            float dxy2[PX], dz[PZ];
+#pragma unroll
+            for (int i = 0; i < PX; ++i) {
+                const float dx = at.x - gx[i];
+                dxy2[i] = __builtin_fmaf(dx, dx, dy2);
+            }
+#pragma unroll
+            for (int k = 0; k < PZ; ++k) dz[k] = at.z - gz[k];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                const float r2 = __builtin_fmaf(dz[q % PZ], dz[q % PZ], dxy2[q / PZ]);
+                part[q] = __builtin_fmaf(at.w * __builtin_amdgcn_rsqf(r2), tLast, part[q]);
+            }
+        }
+    };
+    // squared shortest and longest distances between sub-tile q0's box and
+    // the box (lo, hi)
+    auto boxDist2 = [&](int q0, const float *lo, const float *hi, float &dmin2, float &dmax2) {
         const FG_CONST float *tb = subBox + 6 * (q0 / kSubTile);
-        float dd = 0;
+        dmin2 = dmax2 = 0;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const float d = fmaxf(fmaxf(tb[k] - hi[k], lo[k] - tb[3 + k]), 0.f);
-            dd = __builtin_fmaf(d, d, dd);
+            const float e = fmaxf(tb[3 + k] - lo[k], hi[k] - tb[k]);
+            dmin2 = __builtin_fmaf(d, d, dmin2);
+            dmax2 = __builtin_fmaf(e, e, dmax2);
         }
-        return dd > a.farCut2;
     };
+    auto farFrom = [&](int q0, const float *lo, const float *hi) {
+        float dmin2, dmax2;
+        boxDist2(q0, lo, hi, dmin2, dmax2);
+        return dmin2 > a.farCut2;
+    };
+    using kYes = std::integral_constant<bool, true>;
+    using kNo = std::integral_constant<bool, false>;
 
     for (int j0 = 0; j0 < a.nAtoms; j0 += a.tile) {
         const int j1 = min(j0 + a.tile, a.nAtoms);
-        bool far = true;
+        // the tile's class against the brick, from its sub-tile boxes: near
+        // (0), far (1), far but possibly past the table's end (2, clamped
+        // variant only), past the last table bin (3)
+        int kind = 1;
         if (kComp) {
-            // the tile is far when every sub-tile box is beyond farCut of the brick
-            for (int q = j0; q < j1; q += kSubTile) far = far && farFrom(q, blo, bhi);
-            far = uniform(far);
+            bool far = true, last = true, inTable = true;
+            for (int q = j0; q < j1; q += kSubTile) {
+                float dmin2, dmax2;
+                boxDist2(q, blo, bhi, dmin2, dmax2);
+                far = far && dmin2 > a.farCut2;
+                last = last && dmin2 > a.lastCut2;
+                inTable = inTable && dmax2 < a.tableCut2;
+            }
+            kind = uniform(last ? 3 : far ? (kClamp && !inTable ? 2 : 1) : 0);
         }
         for (int c0 = j0; c0 < j1; c0 += kFold) {
             const int c1 = min(c0 + kFold, j1);
-            if (far) {
-                farAtoms(c0, c1);
+            if (kind == 3) {
+                lastBinAtoms(c0, c1);
+            } else if (kind == 1) {
+                // the FP32 form (kComp false) clamps as its variant says
+                if (kComp) farAtoms(c0, c1, kNo());
+                else farAtoms(c0, c1, kYes());
+            } else if (kind == 2) {
+                farAtoms(c0, c1, kYes());
             } else {
                 // near tile: decide again per wave and sub-tile (a fold chunk
                 // holds whole sub-tiles)
                 for (int q = c0; q < c1; q += kSubTile) {
                     const int q1 = min(q + kSubTile, c1);
-                    if (uniform(farFrom(q, wlo, whi))) farAtoms(q, q1);
+                    if (uniform(farFrom(q, wlo, whi))) farAtoms(q, q1, kYes());
                     else nearAtoms(q, q1);
                 }
             }

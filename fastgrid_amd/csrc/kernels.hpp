@@ -135,6 +135,8 @@ struct ElecArgs {
     double *far;             // [P] far-field sums of the compensated form, added to the near part
                              // k_cutoff wrote in e by k_add_far
     float farCut2;           // (802)^2: sub-tiles farther than 8.02 A hold only far pairs
+    float lastCut2;          // sub-tiles farther than this (~81.94 A) read only the last table bin
+    float tableCut2;         // sub-tiles closer than this at their farthest stay inside the table
     float r2Lo, r2Hi;        // ambiguity band around 64e4 in scaled r^2
     float spacing100;        // 100 * spacing
 };

@@ -172,6 +172,20 @@ def test_kernels_vs_oracle_gpf_variants(ctx, tmp_path, variant):
     _kernel_check(ctx, a)
 
 
+def test_kernels_vs_oracle_long_receptor(ctx, tmp_path):
+    """A 330 A-long receptor around a small central grid: distances run from 0
+    to over 163 A, so the dielectric table no longer covers them (the clamped
+    kernel variant), and the electrostatic tiles fall in every class: near,
+    far inside the table, far possibly past its end, and wholly past its last
+    bin (81.9 A)."""
+    import fastgrid_amd as fg
+    fg.synth_write(str(tmp_path), "long", 9, 20000, "box", (330.0, 30.0, 30.0), (33, 33, 33), "A C HD")
+    a = fx.load_arrays(str(tmp_path), "long.gpf")
+    # atoms reach past 163.84 A (16,384 table bins) from the grid at the centre
+    assert np.abs(a.x).max() > 164.0
+    _kernel_check(ctx, a)
+
+
 def test_floating_grid_and_covalent_map_one_shots(ctx, tmp_path):
     """fg_floating_grid and fg_covalent_map against the oracle: the floating
     grid bit for bit (FP64 minimum, one sqrt, one cast to float), on a grid
