@@ -58,7 +58,7 @@ INTERP = dict(poses=1 << 19, atoms=32, seed=11, site=20.0, spread=5.0, check_pos
 BYTES_PER_INTERP_ATOM = (24 + 3) * 4   # 8 corners x 3 map floats read and xyz read per atom
 # HBM bytes per launch of the electrostatic kernel on L, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE scaled by a device copy of known size
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_elec_L_hbm_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_elec_{}_hbm_traffic.json")  # per workload tag
 
 
 def log(*a):
@@ -168,12 +168,12 @@ ELEC_SHAPES = [(pz, tile, bx, px) for px, pz in ((1, 4), (1, 8), (2, 4), (2, 8))
                for bx in ((8, 16, 32, 256) if px == 1 else (8, 16, 32))]
 
 
-def elec_shapes(fg, shapes):
-    """Launches the compensated electrostatic kernel on L at each listed shape
-    (pz, tile, bx, px), twice, in order, and prints the HIP-event times: the
-    dispatches a rocprofv3 counter run of this mode measures."""
+def elec_shapes(fg, shapes, w=WORKLOAD_L):
+    """Launches the compensated electrostatic kernel on the workload at each
+    listed shape (pz, tile, bx, px), twice, in order, and prints the HIP-event
+    times: the dispatches a rocprofv3 counter run of this mode measures."""
     with tempfile.TemporaryDirectory() as wd:
-        prep, a = make_inputs(fg, wd, WORKLOAD_L)
+        prep, a = make_inputs(fg, wd, w)
     ctx = fg.Context(0)
     job = fg.Job(ctx, a)
     job.run(fg.RUN_ALL, fg.PREC_COMPENSATED)
@@ -190,11 +190,11 @@ def elec_shapes(fg, shapes):
     ctx.close()
 
 
-def traffic_entry(shape):
-    """HBM bytes per electrostatic launch on L from the committed counter
-    summary, for the kernel shape (pz, tile, bx, px) the run used."""
+def traffic_entry(shape, tag="L"):
+    """HBM bytes per electrostatic launch on the workload from the committed
+    counter summary, for the kernel shape (pz, tile, bx, px) the run used."""
     try:
-        t = json.load(open(TRAFFIC_FILE))
+        t = json.load(open(TRAFFIC_FILE.format(tag)))
     except (OSError, ValueError):
         return None
     for e in t.get("shapes", []):
@@ -520,6 +520,8 @@ def main():
     ap.add_argument("--elec-shapes", default="",
                     help="only launch the electrostatic kernel on L at these shapes, e.g. '4,32,16,1;4,32,8,2' "
                          "(pz,tile,bx,px), or 'all' for the autotuner's 42 shapes (for rocprofv3 counter runs)")
+    ap.add_argument("--elec-workload", default="L", choices=["L", "X"],
+                    help="workload of --elec-shapes: L (50k atoms, 127^3) or X (the 250k-atom / 161^3 stress size)")
     ap.add_argument("--batch-only", action="store_true",
                     help="only run the 256-pocket batch and print its results (for rocprofv3 counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -531,7 +533,8 @@ def main():
     if args.elec_shapes:
         import fastgrid_amd as fg
         elec_shapes(fg, ELEC_SHAPES if args.elec_shapes == "all" else
-                    [tuple(int(v) for v in sh.split(",")) for sh in args.elec_shapes.split(";")])
+                    [tuple(int(v) for v in sh.split(",")) for sh in args.elec_shapes.split(";")],
+                    WORKLOAD_X if args.elec_workload == "X" else WORKLOAD_L)
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -591,7 +594,7 @@ def main():
                      "kernel": "k_elec (compensated far field)",
                      "flops_per_launch": FLOP_PER_ELEC_PAIR * far_pairs,
                      "hbm_gbs": round(tr["bytes_per_launch"] / (comp["elec_ms"] * 1e-3) / 1e9, 1) if tr else None,
-                     "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if tr else None},
+                     "traffic_source": os.path.relpath(TRAFFIC_FILE.format("L"), ROOT) if tr else None},
     }
     if "trilinterp" in L:
         line["trilinterp"] = L["trilinterp"]
@@ -620,6 +623,12 @@ def main():
             "error_vs_fp64": {k: X[k]["errors"] for k in ("compensated", "fp32", "plain")},
             "shape": X["shape"],
         }
+        xt_tr = traffic_entry(X["shape"], "X")
+        if xt_tr:
+            line["stress"]["hbm"] = {
+                "bytes_per_launch": xt_tr["bytes_per_launch"],
+                "gbs": round(xt_tr["bytes_per_launch"] / (X["compensated"]["elec_ms"] * 1e-3) / 1e9, 1),
+                "source": os.path.relpath(TRAFFIC_FILE.format("X"), ROOT)}
     if not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(a)
