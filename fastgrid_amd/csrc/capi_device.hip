@@ -421,6 +421,8 @@ int launchElec(fg_job *j, int precision, hipStream_t s)
 // coordinates for the exact near/far predicate and outward-rounded boxes of
 // every kSubTile atoms.  Also the guard band of the near/far predicate (see
 // ElecArgs).
+constexpr int kTableMaxNoClamp = 12288;
+
 int buildElecTiles(fg_job *j, const fg_job_desc &d, double gridMaxAbs)
 {
     const int N = j->N, TT = fgk::kSubTile;
@@ -513,8 +515,11 @@ int buildElecTiles(fg_job *j, const fg_job_desc &d, double gridMaxAbs)
     // the (possibly extended) table is the reference's min(k, 8191).
     // With the table covering every reachable index no clamp is needed: the
     // FP32 index exceeds 100 r by at most a relative 1e-6 plus ~1e-3 from
-    // coordinate rounding, inside the slack of tmax.
-    if (tmax + 2 <= 16384) {
+    // coordinate rounding, inside the slack of tmax.  Past kTableMaxNoClamp
+    // entries (4 bytes each in LDS) the table would cost occupancy, so it
+    // stays at 8192 entries and the clamped variant runs: tiles past the last
+    // bin skip the table and only tiles that may reach past it clamp.
+    if (tmax + 2 <= kTableMaxNoClamp) {
         j->tableLen = std::max(fgk::kTableLen, int(std::ceil(tmax)) + 2);
         j->clamp = false;
     } else {
