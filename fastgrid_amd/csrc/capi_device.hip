@@ -5,7 +5,6 @@
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
-#include <hipcub/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <atomic>
@@ -1974,9 +1973,6 @@ void fg_mapset_destroy(fg_mapset *ms)
     delete ms;
 }
 
-// Below this many poses the spatial sort costs more than it saves.
-static constexpr int kInterpSortMin = 2048;
-
 int fg_trilinterp(fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int npose, const float *xyz, float *elec,
                   float *emap, double *elecTotal, double *emapTotal, float *total)
 {
@@ -2006,17 +2002,6 @@ int fg_trilinterp(fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int np
     const size_t oET = off; off = align(off + size_t(npose) * sizeof(double));
     const size_t oMT = off; off = align(off + size_t(npose) * sizeof(double));
     const size_t oTot = off; off = align(off + size_t(npose) * sizeof(float));
-    // poses of a small ligand run in spatial order: sort keys and indices
-    const bool sorted = A <= fgk::kInterpThreads && npose >= kInterpSortMin;
-    size_t sortBytes = 0;
-    if (sorted)
-        FG_HIP(hipcub::DeviceRadixSort::SortPairs(
-            nullptr, sortBytes, static_cast<const uint32_t *>(nullptr), static_cast<uint32_t *>(nullptr),
-            static_cast<const int32_t *>(nullptr), static_cast<int32_t *>(nullptr), npose, 0, fgk::kPoseKeyBits,
-            ctx->stream));
-    const size_t oKey = off; off = align(off + (sorted ? size_t(npose) * 2 * sizeof(uint32_t) : 0));
-    const size_t oIdx = off; off = align(off + (sorted ? size_t(npose) * 2 * sizeof(int32_t) : 0));
-    const size_t oSort = off; off = align(off + sortBytes);
     char *b = nullptr;
     FG_HIP(hipMalloc(&b, off));
     hipStream_t s = ctx->stream;
@@ -2052,27 +2037,12 @@ int fg_trilinterp(fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int np
     a.elecTotal = reinterpret_cast<double *>(b + oET);
     a.emapTotal = reinterpret_cast<double *>(b + oMT);
     a.total = reinterpret_cast<float *>(b + oTot);
-    a.order = nullptr;
     if (e == hipSuccess) e = hipEventRecord(ctx->ev[0], s);
-    if (e == hipSuccess && sorted) {
-        uint32_t *keys = reinterpret_cast<uint32_t *>(b + oKey);
-        int32_t *idx = reinterpret_cast<int32_t *>(b + oIdx);
-        hipLaunchKernelGGL(fgk::k_pose_keys, dim3(unsigned((npose + fgk::kInterpThreads - 1) / fgk::kInterpThreads)),
-                           dim3(fgk::kInterpThreads), 0, s, a, keys, idx);
-        e = hipGetLastError();
-        size_t bytes = sortBytes;
-        if (e == hipSuccess)
-            e = hipcub::DeviceRadixSort::SortPairs(b + oSort, bytes, keys, keys + npose, idx, idx + npose, npose, 0,
-                                                   fgk::kPoseKeyBits, s);
-        a.order = idx + npose;
-    }
     if (e == hipSuccess) {
         if (A <= fgk::kInterpThreads) {
             const int ppw = fgk::kInterpThreads / A;
             const unsigned groups = unsigned((npose + ppw - 1) / ppw);
-            // with a spatial order: 8 equal stretches, one per XCD
-            const unsigned grid = a.order ? 8 * ((groups + 7) / 8) : groups;
-            hipLaunchKernelGGL(fgk::k_interp_poses, dim3(grid), dim3(fgk::kInterpThreads), 0, s, a);
+            hipLaunchKernelGGL(fgk::k_interp_poses, dim3(groups), dim3(fgk::kInterpThreads), 0, s, a);
         } else {
             hipLaunchKernelGGL(fgk::k_interp_big_poses, dim3(unsigned(npose)), dim3(fgk::kInterpThreads), 0, s, a);
         }

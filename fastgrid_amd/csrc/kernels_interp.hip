@@ -16,7 +16,6 @@
 namespace fgk {
 
 constexpr int kInterpThreads = 256;
-constexpr int kPoseKeyBits = 18;  // 6 bits per axis
 
 // The float AutoDock reads back from a printed map value: "0" below 0.0005
 // in magnitude, else "%.3f" of floor(1000 v + 0.5) * 0.001, which is the
@@ -74,7 +73,6 @@ struct InterpArgs {
     float *elec, *emap;          // [npose][natoms] or null
     double *elecTotal, *emapTotal;  // [npose] or null
     float *total;                // [npose] or null
-    const int32_t *order;        // [npose] poses in spatial order, or null
 };
 
 // Atom i of pose `pose` (idx = pose * natoms + i): what it adds to the two
@@ -145,51 +143,18 @@ __device__ __forceinline__ void writeTotals(const InterpArgs &a, int pose, doubl
     if (a.total) a.total[pose] = float(double(float(et)) + mt);
 }
 
-// Spatial sort key of each pose: Morton code of its first atom's cell in a
-// 64^3 division of the grid box (kPoseKeyBits bits; finer cells would only
-// add radix passes).
-__global__ __launch_bounds__(kInterpThreads) void k_pose_keys(InterpArgs a, uint32_t *keys, int32_t *idx)
-{
-    const int p = int(blockIdx.x) * kInterpThreads + threadIdx.x;
-    if (p >= a.npose) return;
-    const float *c = a.xyz + int64_t(p) * a.natoms * 3;
-    uint32_t key = 0;
-    for (int k = 0; k < 3; ++k) {
-        const double f = (double(c[k]) - a.lo[k]) / (a.hi[k] - a.lo[k]);
-        const uint32_t q = uint32_t(fmin(fmax(f, 0.0), 1.0) * 63.0);
-        uint32_t v = q;  // spread the bits to every third bit
-        v = (v | (v << 16)) & 0x030000FFu;
-        v = (v | (v << 8)) & 0x0300F00Fu;
-        v = (v | (v << 4)) & 0x030C30C3u;
-        v = (v | (v << 2)) & 0x09249249u;
-        key |= v << k;
-    }
-    keys[p] = key;
-    idx[p] = p;
-}
-
 // Poses of up to kInterpThreads atoms: each workgroup takes
 // kInterpThreads / natoms whole poses, one lane per atom; the contributions
 // go through LDS and one lane per pose adds them in atom order (the
-// reference's running sums), then writes the totals.  With a spatial order,
-// group g of consecutive sorted poses goes to workgroup 8 (g mod per) + g /
-// per: consecutive workgroups land on different XCDs, so each XCD works
-// through one contiguous stretch of the order and its L2 holds that
-// region's map cells.
+// reference's running sums), then writes the totals.
 __global__ __launch_bounds__(kInterpThreads) void k_interp_poses(InterpArgs a)
 {
     __shared__ double sE[kInterpThreads], sM[kInterpThreads];
     const int t = threadIdx.x, A = a.natoms;
     const int ppw = kInterpThreads / A;
-    int g = int(blockIdx.x);
-    if (a.order) {
-        const int G = (a.npose + ppw - 1) / ppw, per = (G + 7) / 8;
-        g = (g % 8) * per + g / 8;
-        if (g >= G) return;  // the whole workgroup
-    }
-    auto poseAt = [&](int sp) { return a.order ? a.order[sp] : sp; };
+    const int g = int(blockIdx.x);
     const int sp = g * ppw + t / A, i = t % A;
-    if (t < ppw * A && sp < a.npose) interpAtom(a, int64_t(poseAt(sp)) * A + i, i, sE[t], sM[t]);
+    if (t < ppw * A && sp < a.npose) interpAtom(a, int64_t(sp) * A + i, i, sE[t], sM[t]);
     __syncthreads();
     const int own = g * ppw + t;
     if (t < ppw && own < a.npose) {
@@ -198,7 +163,7 @@ __global__ __launch_bounds__(kInterpThreads) void k_interp_poses(InterpArgs a)
             et += sE[t * A + k];
             mt += sM[t * A + k];
         }
-        writeTotals(a, poseAt(own), et, mt);
+        writeTotals(a, own, et, mt);
     }
 }
 

@@ -64,7 +64,7 @@ def test_trilinterp_linear_field_and_many_poses(ctx):
     """Maps linear in the grid indices, 20,000 random poses of 24 atoms over
     the whole box and beyond it, 4,099 poses of 17 atoms, and 64 poses of 300
     atoms (more than a workgroup's lanes) with ignored atoms: bit for bit with
-    the oracle (the first two run in spatial order)."""
+    the oracle."""
     import fastgrid_amd as fg
     g0, maps = linear_case()
     g = fg.FgMapGrid()
@@ -85,7 +85,7 @@ def test_trilinterp_linear_field_and_many_poses(ctx):
     assert same(r["elec_total"], oet) and same(r["emap_total"], omt) and same(r["total"], otot)
     with pytest.raises(fg.FgError):
         fg.trilinterp(ctx, ms, xyz[:2], np.full(24, 3, np.int32), q, np.abs(q), 1, 2)
-    # spatially sorted pose order with a ragged tail: 4,099 poses of 17 atoms
+    # a ragged tail: 4,099 poses of 17 atoms
     # are 274 workgroups of 15 poses, dealt to 8 XCDs in stretches of 35
     odd = (lo - 0.5 + rng.uniform(0, 1, (4099, 17, 3)) * (hi - lo + 1.0)).astype(np.float32)
     ot, oq = types[:17], q[:17]
