@@ -492,6 +492,8 @@ def run_workload(fg, ctx, w, steps, warmup, tune=True, interp=False, cpu=True):
     report = fg.autotune(ctx, [job]) if tune else ""
     shape = job.shape
     log("tuned shape (pz, tile, bx, px):", shape)
+    for line in report.splitlines()[1:]:
+        log("  " + line)
     names = prep.map_types()[:M] + ["e", "d"]
     res = {"N": N, "P": P, "M": M, "elec_pairs": elec_pairs, "cutoff_pairs": pcut, "fp64_elec_ms": t64.elec_ms,
            "shape": shape, "tune_report": report}

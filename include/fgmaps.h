@@ -237,7 +237,10 @@ int fg_job_set_shape(fg_job *job, const fg_shape *shape);
  * throughput rather than by one small launch's latency.  The
  * points per lane along x and z are common to all the jobs (a batch launches
  * one kernel for all of them): the pair with the least total modelled time
- * wins.  precision: FG_PREC_COMPENSATED or FG_PREC_FP32.  The
+ * wins.  A single job's e-kernel runs concurrently with its cutoff kernels:
+ * the (at most 4) shapes within 5 % of its fastest e-kernel are then timed
+ * over whole steps of all its maps and the fastest step wins.
+ * precision: FG_PREC_COMPENSATED or FG_PREC_FP32.  The
  * jobs' maps are undefined until their next run.  fg_tune_report() returns
  * one text line per class of the last call on ctx. */
 int fg_autotune(fg_ctx *ctx, fg_job *const *jobs, int n_jobs, int precision);

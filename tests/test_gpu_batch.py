@@ -166,6 +166,16 @@ def test_shapes_and_autotuner(ctx, pockets):
     for pz, tile, bx, px in shapes:
         assert pz in (4, 8) and tile in (32, 64, 128) and bx in (8, 16, 32, 256) and px in (1, 2)
         assert not (px == 2 and bx == 256)
+    # one job: the e-kernel's best shapes are compared over whole steps too,
+    # and the map bits do not depend on the pick
+    before = job.fetch()
+    line = fg.autotune(ctx, [job])
+    print(line)
+    assert "whole step (all maps)" in line
+    pz, tile, bx, px = job.shape
+    assert pz in (4, 8) and tile in (32, 64, 128) and bx in (8, 16, 32, 256) and px in (1, 2)
+    job.run(fg.RUN_ALL, fg.PREC_COMPENSATED)
+    assert _same(job.fetch(), before)
     # tuned jobs still reproduce through the batch
     want = _single(jobs, fg.PREC_COMPENSATED)
     fg.batch_run(ctx, jobs)
