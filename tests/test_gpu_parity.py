@@ -223,6 +223,29 @@ def test_atom_on_grid_point(ctx, tmp_path):
     _kernel_check(ctx, a)
 
 
+@pytest.mark.parametrize("npts,natoms", [("2 2 2", 1), ("2 60 2", 1), ("60 2 4", 3), ("4 4 60", 7)])
+def test_tiny_and_thin_grids(ctx, tmp_path, npts, natoms):
+    """Edges of the launch shapes: grids of 3 points along some axes (every
+    brick, tile and z-group padded), and receptors of 1 to 7 atoms (a tile
+    and a candidate chunk mostly empty), against the oracle."""
+    rng = np.random.default_rng(natoms)
+    types = ["OA", "C", "N", "HD", "SA", "A", "NA"]
+    lines = []
+    for i in range(natoms):
+        x, y, z = rng.uniform(-3, 3, 3)
+        t = types[i % len(types)]
+        lines.append(f"ATOM  {i + 1:5d} X{i:<3d} RES A   1    {x:8.3f}{y:8.3f}{z:8.3f}  1.00  0.00    "
+                     f"{rng.uniform(-0.5, 0.5):6.3f} {t:<2s}\n")
+    open(os.path.join(tmp_path, "t.pdbqt"), "w").write("".join(lines))
+    open(os.path.join(tmp_path, "t.gpf"), "w").write(
+        f"npts {npts}\ngridfld t.maps.fld\nspacing 0.375\nreceptor_types A C HD N NA OA SA\n"
+        "ligand_types A C HD N OA\nreceptor t.pdbqt\ngridcenter 0.5 -0.25 0.0\nsmooth 0.5\nmap t.A.map\n"
+        "map t.C.map\nmap t.HD.map\nmap t.N.map\nmap t.OA.map\nelecmap t.e.map\ndsolvmap t.d.map\n"
+        "dielectric -0.1465\n")
+    a = fx.load_arrays(str(tmp_path), "t.gpf")
+    _kernel_check(ctx, a)
+
+
 def test_far_grid_has_no_cutoff_pairs(ctx, tmp_path):
     """A grid more than 8 A from every atom: affinity maps are zero, the
     electrostatic map is all far field."""
