@@ -127,6 +127,30 @@ def test_batch_against_oracle(ctx, pockets):
         j.close()
 
 
+def test_batch_with_tiny_jobs(ctx, pockets, tmp_path):
+    """Jobs of 3 to 5 atoms on grids of 3 points along some axes, batched
+    with two ordinary pockets: every job's maps equal its own run bit for
+    bit, and the tiny jobs match the oracle."""
+    import fastgrid_amd as fg
+    from oracle import oracle as orc
+    tiny = [fx.pocket(str(tmp_path), f"t{i}", 31 + i, n, npts)
+            for i, (n, npts) in enumerate([(3, (3, 3, 3)), (5, (3, 41, 3)), (4, (41, 3, 5))])]
+    arrays = [pockets[3], tiny[0], tiny[1], pockets[0], tiny[2]]
+    jobs = _jobs(ctx, arrays)
+    want = _single(jobs, fg.PREC_COMPENSATED)
+    fg.batch_run(ctx, jobs)
+    for i, j in enumerate(jobs):
+        assert _same(j.fetch(), want[i]), i
+    for k, a in zip((1, 2, 4), tiny):
+        out = jobs[k].fetch()
+        om, od = orc.affinity_maps(a)
+        M = a.n_maps
+        for m in range(M):
+            assert np.array_equal(fx.milli(out[m]), fx.milli(om[m])), (k, m)
+        assert np.array_equal(fx.milli(out[M + 1]), fx.milli(od)), k
+        assert np.abs(out[M] - orc.elec_map(a)).max() < 3e-4, k
+
+
 def test_shapes_and_autotuner(ctx, pockets):
     """Every shape gives the same bits, within the compensated bound of the
     oracle (the FP32 partials fold after every 64 atoms whatever the tile);
