@@ -88,7 +88,8 @@ def time_steps(fg, ctx, job, steps, warmup, precision):
     elec, aff = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
-        job.run(fg.RUN_ALL, precision)
+        # with the phases timed (no cost when each step is waited for)
+        job.run(fg.RUN_ALL | fg.RUN_PHASE_TIMING, precision)
         tm = ctx.timing()  # waits for this step's events
         elec.append(tm.elec_ms)
         aff.append(tm.affinity_ms)
@@ -182,7 +183,7 @@ def elec_shapes(fg, shapes, w=WORKLOAD_L):
         job.shape = sh
         ms = []
         for _ in range(2):
-            job.run(fg.RUN_ELEC, fg.PREC_COMPENSATED)
+            job.run(fg.RUN_ELEC | fg.RUN_PHASE_TIMING, fg.PREC_COMPENSATED)
             ctx.sync()
             ms.append(round(ctx.timing().elec_ms, 3))
         print(json.dumps({"shape": list(sh), "elec_ms": ms}), flush=True)

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "libfgmaps.so")
 FG_OK = 0
 PREC_PLAIN, PREC_COMPENSATED, PREC_FP64, PREC_FP32 = 0, 1, 2, 3
 RUN_ELEC, RUN_AFFINITY, RUN_FLOATING, RUN_ALL = 1, 2, 4, 7
+RUN_PHASE_TIMING = 8  # fg_timing's elec_ms/affinity_ms for compensated and FP32 runs
 BATCH_NO_GRAPH = 1
 TABLE_LEN = 8192
 

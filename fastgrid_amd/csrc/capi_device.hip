@@ -54,7 +54,7 @@ struct fg_ctx {
     // for e1.
     hipStream_t side = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    hipEvent_t fork = nullptr, e0 = nullptr, e1 = nullptr;
+    hipEvent_t fork = nullptr, e0 = nullptr, e1 = nullptr, join = nullptr;
     bool haveTiming = false;
     bool splitTiming = true;  // ev[0]..ev[1]: cutoff phase; e0..e1: electrostatic phase
     bool elecTimed = false;   // e0..e1 were recorded by the last run
@@ -90,6 +90,13 @@ struct fg_job {
     CutAtom *dAtoms = nullptr;
     double *dVS = nullptr;
     double *dFar = nullptr;  // [P] far-field e of the compensated form
+    int32_t *dCounter = nullptr;  // work counter of the persistent e-kernel
+    int elecPerCU[2][2][2] = {};  // e-kernel workgroups per CU by [px - 1][pz == 8][compensated]
+    // the graph of the last compensated or FP32 run and what it was made for
+    hipGraph_t runGraph = nullptr;
+    hipGraphExec_t runExec = nullptr;
+    unsigned runWhat = 0;
+    int runKey[5] = {-1, 0, 0, 0, 0};  // precision, px, pz, tile, bxShift
     double *dPair = nullptr, *dDesolv = nullptr, *dEps = nullptr, *dCovalent = nullptr;
     double *dXYZ = nullptr;
     int32_t *dHbond = nullptr, *dCellStart = nullptr, *dCellAtoms = nullptr;
@@ -373,35 +380,48 @@ fgk::ElecArgs elecArgs(const fg_job *j, int px, int pz, int tile, int bxShift)
     return a;
 }
 
-template <int PX, int PZ>
-void launchElecV(fg_job *j, const fgk::ElecArgs &a, bool comp, size_t lds, hipStream_t s)
+// The persistent e-kernel: the workgroups its occupancy admits per CU (found
+// when the job is made, so that nothing here queries the device), less
+// `reserve` per CU left to the kernels running concurrently (a persistent
+// workgroup keeps its slot until the bricks run out), after its counter is
+// cleared on the same stream.
+template <int PX, int PZ, bool kComp, bool kClamp>
+int launchElecK(fg_job *j, const fgk::ElecArgs &a, size_t lds, int reserve, hipStream_t s)
 {
-    dim3 grid(unsigned(a.nbx * a.nby * a.nbz));
-    dim3 block(fgk::kElecThreads);
-    if (comp && j->clamp) hipLaunchKernelGGL((fgk::k_elec<PX, PZ, true, true>), grid, block, lds, s, a);
-    else if (comp) hipLaunchKernelGGL((fgk::k_elec<PX, PZ, true, false>), grid, block, lds, s, a);
-    else if (j->clamp) hipLaunchKernelGGL((fgk::k_elec<PX, PZ, false, true>), grid, block, lds, s, a);
-    else hipLaunchKernelGGL((fgk::k_elec<PX, PZ, false, false>), grid, block, lds, s, a);
+    auto k = fgk::k_elec<PX, PZ, kComp, kClamp>;
+    const int perCU = j->elecPerCU[PX - 1][PZ == 8][kComp];
+    const int bricks = a.nbx * a.nby * a.nbz;
+    const int wgs = std::max(1, std::min(bricks, std::max(perCU - reserve, 1) * j->ctx->numCUs));
+    hipLaunchKernelGGL(fgk::k_fill_i32, dim3(1), dim3(256), 0, s, j->dCounter, 1, 0);
+    hipLaunchKernelGGL(k, dim3(unsigned(wgs)), dim3(fgk::kElecThreads), lds, s, a, j->dCounter);
+    return FG_OK;
+}
+
+template <int PX, int PZ>
+int launchElecV(fg_job *j, const fgk::ElecArgs &a, bool comp, size_t lds, int reserve, hipStream_t s)
+{
+    if (comp)
+        return j->clamp ? launchElecK<PX, PZ, true, true>(j, a, lds, reserve, s)
+                        : launchElecK<PX, PZ, true, false>(j, a, lds, reserve, s);
+    return j->clamp ? launchElecK<PX, PZ, false, true>(j, a, lds, reserve, s)
+                    : launchElecK<PX, PZ, false, false>(j, a, lds, reserve, s);
 }
 
 // k_elec of shape (px, pz, tile, bxShift) on stream s; px 2 needs a 2D brick
 // (bxShift > 0).
-void launchElecShape(fg_job *j, int px, int pz, int tile, int bxShift, bool comp, hipStream_t s)
+int launchElecShape(fg_job *j, int px, int pz, int tile, int bxShift, bool comp, int reserve, hipStream_t s)
 {
     const fgk::ElecArgs a = elecArgs(j, px, pz, tile, bxShift);
     const size_t lds = size_t(j->tableLen) * sizeof(float);
-    if (px == 2) {
-        if (pz == 8) launchElecV<2, 8>(j, a, comp, lds, s);
-        else launchElecV<2, 4>(j, a, comp, lds, s);
-    } else {
-        if (pz == 8) launchElecV<1, 8>(j, a, comp, lds, s);
-        else launchElecV<1, 4>(j, a, comp, lds, s);
-    }
+    if (px == 2)
+        return pz == 8 ? launchElecV<2, 8>(j, a, comp, lds, reserve, s) : launchElecV<2, 4>(j, a, comp, lds, reserve, s);
+    return pz == 8 ? launchElecV<1, 8>(j, a, comp, lds, reserve, s) : launchElecV<1, 4>(j, a, comp, lds, reserve, s);
 }
 
 // The electrostatic kernel of the precision on stream s (the compensated
-// form writes its far part into dFar, for k_add_far).
-int launchElec(fg_job *j, int precision, hipStream_t s)
+// form writes its far part into dFar, for k_add_far); reserve: workgroup
+// slots per CU left to the cutoff phase running concurrently.
+int launchElec(fg_job *j, int precision, int reserve, hipStream_t s)
 {
     double *e = j->dMaps + int64_t(j->M) * j->P;
     if (precision == FG_PREC_FP64) {
@@ -410,7 +430,8 @@ int launchElec(fg_job *j, int precision, hipStream_t s)
         FG_HIP(hipGetLastError());
         return FG_OK;
     }
-    launchElecShape(j, j->px, j->pz, j->tile, j->bxShift, precision == FG_PREC_COMPENSATED, s);
+    if (int rc = launchElecShape(j, j->px, j->pz, j->tile, j->bxShift, precision == FG_PREC_COMPENSATED, reserve, s))
+        return rc;
     FG_HIP(hipGetLastError());
     return FG_OK;
 }
@@ -818,6 +839,8 @@ void destroyJob(fg_job *j)
 {
     if (!j) return;
     if (j->ctx) forget(j->ctx->jobs, j);
+    if (j->runExec) (void)hipGraphExecDestroy(j->runExec);
+    if (j->runGraph) (void)hipGraphDestroy(j->runGraph);
     for (void *p : j->owned) (void)hipFree(p);
     delete j;
 }
@@ -829,11 +852,12 @@ void destroyJob(fg_job *j)
 int timeShape(fg_job *j, int px, int pz, int tile, int bxShift, bool comp, float &ms)
 {
     fg_ctx *ctx = j->ctx;
-    auto go = [&]() { launchElecShape(j, px, pz, tile, bxShift, comp, ctx->stream); };
-    go();  // warm-up
+    // one slot per CU left free, as in a step with the cutoff phase
+    auto go = [&]() { return launchElecShape(j, px, pz, tile, bxShift, comp, 1, ctx->stream); };
+    if (int rc = go()) return rc;  // warm-up
     FG_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
-    go();
-    go();
+    if (int rc = go()) return rc;
+    if (int rc = go()) return rc;
     FG_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
     FG_HIP(hipGetLastError());
     FG_HIP(hipEventSynchronize(ctx->ev[3]));
@@ -849,6 +873,21 @@ const void *elecKernelV(bool comp, bool clamp)
                            : reinterpret_cast<const void *>(fgk::k_elec<PX, PZ, true, false>);
     return clamp ? reinterpret_cast<const void *>(fgk::k_elec<PX, PZ, false, true>)
                  : reinterpret_cast<const void *>(fgk::k_elec<PX, PZ, false, false>);
+}
+
+// Workgroups per CU of every e-kernel variant at the job's table size.
+int elecOccupancy(fg_job *j)
+{
+    const size_t lds = size_t(j->tableLen) * sizeof(float);
+    for (int px = 1; px <= 2; ++px)
+        for (int z8 = 0; z8 < 2; ++z8)
+            for (int comp = 0; comp < 2; ++comp) {
+                const void *k = px == 2 ? (z8 ? elecKernelV<2, 8>(comp, j->clamp) : elecKernelV<2, 4>(comp, j->clamp))
+                                        : (z8 ? elecKernelV<1, 8>(comp, j->clamp) : elecKernelV<1, 4>(comp, j->clamp));
+                FG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&j->elecPerCU[px - 1][z8][comp], k,
+                                                                     fgk::kElecThreads, lds));
+            }
+    return FG_OK;
 }
 
 // Share of the device's workgroup slots one launch of shape (px, pz, tile,
@@ -1329,7 +1368,7 @@ int fg_create(int device, fg_ctx **out)
         fg_destroy(c);
         return fgc::fail(FG_E_DEVICE, "fg_create: hipStreamCreate failed");
     }
-    for (hipEvent_t *e : {&c->ev[0], &c->ev[1], &c->ev[2], &c->ev[3], &c->fork, &c->e0, &c->e1})
+    for (hipEvent_t *e : {&c->ev[0], &c->ev[1], &c->ev[2], &c->ev[3], &c->fork, &c->e0, &c->e1, &c->join})
         if (hipEventCreate(e) != hipSuccess) {
             fg_destroy(c);
             return fgc::fail(FG_E_DEVICE, "fg_create: hipEventCreate failed");
@@ -1350,7 +1389,7 @@ void fg_destroy(fg_ctx *ctx)
     for (fg_job *j : ctx->jobs) j->ctx = nullptr;
     for (fg_batch *b : ctx->batches) b->ctx = nullptr;
     for (fg_mapset *m : ctx->mapsets) m->ctx = nullptr;
-    for (hipEvent_t e : {ctx->ev[0], ctx->ev[1], ctx->ev[2], ctx->ev[3], ctx->fork, ctx->e0, ctx->e1})
+    for (hipEvent_t e : {ctx->ev[0], ctx->ev[1], ctx->ev[2], ctx->ev[3], ctx->fork, ctx->e0, ctx->e1, ctx->join})
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
@@ -1466,6 +1505,7 @@ int jobBuildHost(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
     if (!rc) rc = buildPlain(j, *d);
     if (!rc) rc = devAlloc(j, j->dMaps, size_t(j->M + 2) * size_t(j->P));
     if (!rc) rc = devAlloc(j, j->dFar, size_t(j->P));
+    if (!rc) rc = devAlloc(j, j->dCounter, size_t(1));
     if (!rc && j->floating) rc = devAlloc(j, j->dFmap, size_t(j->P));
     if (rc) {
         destroyJob(j);
@@ -1483,6 +1523,7 @@ int jobBuildHost(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
 int jobFinishDevice(fg_ctx *ctx, fg_job *j)
 {
     int rc = flushArena(j);
+    if (!rc) rc = elecOccupancy(j);
     if (!rc) rc = buildLists(j);
     if (!rc) {
         hipError_t e = hipMemsetAsync(j->dMaps, 0, size_t(j->M + 2) * size_t(j->P) * sizeof(double), ctx->stream);
@@ -1571,18 +1612,21 @@ void fg_job_destroy(fg_job *job)
     destroyJob(job);
 }
 
-int fg_job_run(fg_ctx *ctx, fg_job *j, unsigned what, int precision)
+// The launches of one run on the two streams.  Under capture the timing
+// events become event-record nodes of the graph (kGraphTimed) or are left out
+// (kGraphUntimed); the fork and join events only order the two streams.
+enum RunMode { kStreams, kGraphTimed, kGraphUntimed };
+static int enqueueRun(fg_ctx *ctx, fg_job *j, unsigned what, int precision, RunMode mode)
 {
-    if (!ctx || !j || j->ctx != ctx) return fgc::fail(FG_E_ARG, "fg_job_run: job does not belong to this context");
-    if (precision < FG_PREC_PLAIN || precision > FG_PREC_FP32) return fgc::fail(FG_E_ARG, "fg_job_run: bad precision");
-    if ((what & FG_RUN_AFFINITY) && j->M > 0 && !j->hasGeom)
-        return fgc::fail(FG_E_ARG, "fg_job_run: job has no affinity inputs");
-    FG_HIP(hipSetDevice(ctx->device));
+    auto rec = [&](hipEvent_t ev, hipStream_t st) {
+        if (mode == kGraphUntimed) return hipSuccess;
+        return mode == kGraphTimed ? hipEventRecordWithFlags(ev, st, hipEventRecordExternal) : hipEventRecord(ev, st);
+    };
     hipStream_t s = ctx->stream;
     const bool elec = what & FG_RUN_ELEC, aff = (what & FG_RUN_AFFINITY) != 0;
     const bool near = elec && precision == FG_PREC_COMPENSATED;
     const bool fmap = (what & FG_RUN_FLOATING) && j->floating;
-    FG_HIP(hipEventRecord(ctx->ev[0], s));
+    FG_HIP(rec(ctx->ev[0], s));
     // the electrostatic kernels run on the side stream, concurrently with the
     // cutoff phase: they write e (or, compensated, the far part) and read
     // nothing the cutoff kernels write.  They are enqueued first: theirs is
@@ -1591,14 +1635,16 @@ int fg_job_run(fg_ctx *ctx, fg_job *j, unsigned what, int precision)
     // the lists, the cutoff kernels took the slots first and L's step grew
     // from 23.5 to 33.5 ms.
     if (elec) {
-        FG_HIP(hipStreamWaitEvent(ctx->side, ctx->ev[0], 0));
-        FG_HIP(hipEventRecord(ctx->e0, ctx->side));
+        FG_HIP(hipEventRecord(ctx->fork, s));
+        FG_HIP(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+        FG_HIP(rec(ctx->e0, ctx->side));
         if (precision == FG_PREC_PLAIN) {
             if (int rc = runPlainElec(j, ctx->side)) return rc;
-        } else if (int rc = launchElec(j, precision, ctx->side)) {
+        } else if (int rc = launchElec(j, precision, (aff || near || fmap) ? 1 : 0, ctx->side)) {
             return rc;
         }
-        FG_HIP(hipEventRecord(ctx->e1, ctx->side));
+        FG_HIP(rec(ctx->e1, ctx->side));
+        FG_HIP(hipEventRecord(ctx->join, ctx->side));
     }
     if (precision == FG_PREC_PLAIN) {
         if ((aff && j->hasGeom) || fmap)
@@ -1615,18 +1661,66 @@ int fg_job_run(fg_ctx *ctx, fg_job *j, unsigned what, int precision)
         if (fmap)
             if (int rc = launchFmap(j)) return rc;
     }
-    FG_HIP(hipEventRecord(ctx->ev[1], s));
+    FG_HIP(rec(ctx->ev[1], s));
     if (elec) {
-        FG_HIP(hipStreamWaitEvent(s, ctx->e1, 0));
+        FG_HIP(hipStreamWaitEvent(s, ctx->join, 0));
         if (near)  // e = near (k_cutoff) + far (k_elec)
             hipLaunchKernelGGL(fgk::k_add_far, dim3(unsigned((j->P + 255) / 256)), dim3(256), 0, s, j->P, j->dFar,
                                j->dMaps + int64_t(j->M) * j->P);
         FG_HIP(hipGetLastError());
     }
-    FG_HIP(hipEventRecord(ctx->ev[2], s));
+    FG_HIP(rec(ctx->ev[2], s));
+    return FG_OK;
+}
+
+int fg_job_run(fg_ctx *ctx, fg_job *j, unsigned what, int precision)
+{
+    if (!ctx || !j || j->ctx != ctx) return fgc::fail(FG_E_ARG, "fg_job_run: job does not belong to this context");
+    if (precision < FG_PREC_PLAIN || precision > FG_PREC_FP32) return fgc::fail(FG_E_ARG, "fg_job_run: bad precision");
+    if ((what & FG_RUN_AFFINITY) && j->M > 0 && !j->hasGeom)
+        return fgc::fail(FG_E_ARG, "fg_job_run: job has no affinity inputs");
+    FG_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    if (precision == FG_PREC_COMPENSATED || precision == FG_PREC_FP32) {
+        // One graph per (maps, precision, shape, phase timing), captured at
+        // its first run and replayed: launched as one graph without event
+        // nodes, the e-kernel and the cutoff phase share the device the same
+        // way every time (L: 23.4 ms per run back to back), while launched
+        // on the two streams run after run they did not (28.6 ms; 23.4-25 ms
+        // with a wait after each run).  Event-record nodes bring back the
+        // 28 ms when runs follow each other without a wait, so the phases are
+        // timed only when asked for; the run's total is timed around it.
+        const bool phases = (what & FG_RUN_PHASE_TIMING) != 0;
+        const int key[5] = {precision, j->px, j->pz, j->tile, j->bxShift};
+        if (!j->runExec || j->runWhat != what || std::memcmp(key, j->runKey, sizeof key) != 0) {
+            if (j->runExec) (void)hipGraphExecDestroy(j->runExec);
+            if (j->runGraph) (void)hipGraphDestroy(j->runGraph);
+            j->runExec = nullptr;
+            j->runGraph = nullptr;
+            FG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueueRun(ctx, j, what, precision, phases ? kGraphTimed : kGraphUntimed);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(s, &g);
+            if (rc || e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc ? rc : fgc::fail(FG_E_DEVICE, std::string("fg_job_run: capture: ") + hipGetErrorString(e));
+            }
+            j->runGraph = g;
+            FG_HIP(hipGraphInstantiate(&j->runExec, g, nullptr, nullptr, 0));
+            j->runWhat = what;
+            std::memcpy(j->runKey, key, sizeof key);
+        }
+        if (!phases) FG_HIP(hipEventRecord(ctx->ev[0], s));
+        FG_HIP(hipGraphLaunch(j->runExec, s));
+        if (!phases) FG_HIP(hipEventRecord(ctx->ev[2], s));
+        ctx->splitTiming = phases;
+    } else if (int rc = enqueueRun(ctx, j, what, precision, kStreams)) {
+        return rc;
+    } else {
+        ctx->splitTiming = true;
+    }
     ctx->haveTiming = true;
-    ctx->splitTiming = true;
-    ctx->elecTimed = elec;
+    ctx->elecTimed = (what & FG_RUN_ELEC) != 0;
     return FG_OK;
 }
 

@@ -177,7 +177,8 @@ int main(int argc, char **argv)
     fg_job *job = nullptr;
     if (fg_job_create(ctx, &desc, &job) != FG_OK) return fatal(0xbad, fg_last_error());
     const auto t2 = std::chrono::steady_clock::now();
-    if (fg_job_run(ctx, job, FG_RUN_ALL, o.precision) != FG_OK || fg_sync(ctx) != FG_OK)
+    if (fg_job_run(ctx, job, FG_RUN_ALL | (o.benchmark ? FG_RUN_PHASE_TIMING : 0u), o.precision) != FG_OK ||
+        fg_sync(ctx) != FG_OK)
         return fatal(0xbad, fg_last_error());
     fg_timing tm{};
     fg_last_timing(ctx, &tm);

@@ -1055,16 +1055,6 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
     }
 }
 
-// One job: one workgroup per brick.
-template <int PX, int PZ, bool kComp, bool kClamp>
-__global__ __launch_bounds__(kElecThreads) void k_elec(ElecArgs a)
-{
-    extern __shared__ float eps[];
-    loadEps(eps, a.eps32, a.tableLen);
-    __syncthreads();
-    elecBrick<PX, PZ, kComp, kClamp>(a, int(blockIdx.x), eps);
-}
-
 // The compensated e-map: the near part k_cutoff wrote plus the far part
 // k_elec wrote (the two kernels run concurrently into separate buffers).
 __global__ __launch_bounds__(256) void k_add_far(int64_t P, const double *far, double *e)
@@ -1103,6 +1093,25 @@ __global__ __launch_bounds__(256) void k_fill_i32(int32_t *p, int n, int32_t v)
 {
     const int i = int(blockIdx.x) * 256 + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// One job as a persistent grid: workgroups take bricks from a counter
+// (cleared by k_fill_i32 before the launch) until none are left.  A
+// workgroup that starts late, its slot held by a kernel running
+// concurrently, takes fewer bricks instead of stretching the launch by a
+// whole brick's time; which workgroup computes a brick changes no bits.
+template <int PX, int PZ, bool kComp, bool kClamp>
+__global__ __launch_bounds__(kElecThreads) void k_elec(ElecArgs a, int *counter)
+{
+    extern __shared__ float eps[];
+    __shared__ int slot;
+    loadEps(eps, a.eps32, a.tableLen);
+    const int total = a.nbx * a.nby * a.nbz;
+    for (;;) {
+        const int item = nextItem(counter, &slot);  // its barrier also orders the table load
+        if (item >= total) break;
+        elecBrick<PX, PZ, kComp, kClamp>(a, item, eps);
+    }
 }
 
 // sharedTableLen > 0: every job uses the same table (entries past 8191

@@ -64,6 +64,12 @@ enum {
 #define FG_RUN_AFFINITY 2u /* affinity maps + desolvation map */
 #define FG_RUN_FLOATING 4u /* the floating grid (jobs created with floating_grid = 1) */
 #define FG_RUN_ALL 7u
+/* Also time the phases (fg_timing's elec_ms and affinity_ms) of a compensated
+ * or FP32 run.  Those runs replay a graph of their launches; with phase
+ * timing it holds the timing events as graph nodes, which cost ~20 % when
+ * runs follow each other without a wait (L: 28 ms instead of 23 ms per run)
+ * and nothing when each run is waited for. */
+#define FG_RUN_PHASE_TIMING 8u
 
 typedef struct fg_ctx fg_ctx;
 typedef struct fg_job fg_job;
@@ -273,7 +279,8 @@ int fg_batch_run(fg_ctx *ctx, fg_job *const *jobs, int n_jobs, unsigned what, in
  * measured with HIP events (ms).  The electrostatic kernels run on a second
  * stream of the context, concurrently with the cutoff kernels, so elec_ms +
  * affinity_ms exceeds total_ms when they overlap.  A batch replayed from its
- * graph reports total_ms only (elec_ms = affinity_ms = -1). */
+ * graph, and a compensated or FP32 job run without FG_RUN_PHASE_TIMING,
+ * report total_ms only (elec_ms = affinity_ms = -1). */
 typedef struct {
     double elec_ms;      /* electrostatic kernel(s), on the second stream; 0 if none ran */
     double affinity_ms;  /* cutoff kernel(s), incl. the per-brick atom lists */

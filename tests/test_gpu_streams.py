@@ -57,7 +57,7 @@ def test_partial_and_interleaved_runs_match_a_full_run(ctx, job_arrays):
 def test_timing_events_bracket_the_concurrent_phases(ctx, job_arrays, prec):
     import fastgrid_amd as fg
     job = fg.Job(ctx, job_arrays)
-    job.run(fg.RUN_ALL, getattr(fg, prec))
+    job.run(fg.RUN_ALL | fg.RUN_PHASE_TIMING, getattr(fg, prec))
     t = ctx.timing()
     # ev0 .. ev2 encloses both the cutoff phase (ev0 .. ev1) and the
     # electrostatic kernel on the second stream (e0 .. e1, e0 after ev0, e1
@@ -65,6 +65,34 @@ def test_timing_events_bracket_the_concurrent_phases(ctx, job_arrays, prec):
     assert t.elec_ms > 0 and t.affinity_ms > 0
     assert t.total_ms + 1e-3 >= t.elec_ms and t.total_ms + 1e-3 >= t.affinity_ms
     # a run without the electrostatic map reports none
-    job.run(fg.RUN_AFFINITY, getattr(fg, prec))
+    job.run(fg.RUN_AFFINITY | fg.RUN_PHASE_TIMING, getattr(fg, prec))
     assert ctx.timing().elec_ms == 0
+    # without phase timing a graph-replayed run (compensated, FP32) reports
+    # its total only; the plain form always times its phases
+    job.run(fg.RUN_ALL, getattr(fg, prec))
+    t = ctx.timing()
+    assert t.total_ms > 0
+    if prec == "PREC_PLAIN":
+        assert t.elec_ms > 0 and t.affinity_ms > 0
+    else:
+        assert t.elec_ms == -1 and t.affinity_ms == -1
+    job.close()
+
+
+def test_back_to_back_runs_and_graph_rebuilds(ctx, job_arrays):
+    """Runs enqueued without waits between them, with shape changes (a new
+    graph each) and phase timing switched on and off in between, give the
+    bits of a lone run."""
+    import fastgrid_amd as fg
+    job = fg.Job(ctx, job_arrays)
+    job.run(fg.RUN_ALL, fg.PREC_COMPENSATED)
+    want = job.fetch()
+    for shape in [(4, 64, 8, 2), (8, 32, 16, 1), (4, 128, 256, 1)]:
+        job.shape = shape
+        for flags in (0, fg.RUN_PHASE_TIMING, 0):
+            for _ in range(3):
+                job.run(fg.RUN_ALL | flags, fg.PREC_COMPENSATED)
+        got = job.fetch()
+        for m in range(len(want)):
+            assert np.array_equal(got[m], want[m]), (shape, m)
     job.close()
