@@ -103,6 +103,13 @@ SIGNATURES = {
     "fg_affinity_d_maps": (C.c_int, [C.c_void_p, C.POINTER(FgGrid), C.POINTER(FgAtoms), C.POINTER(FgHbondGeom),
                                      C.POINTER(FgTables), C.POINTER(FgMapParams), C.c_int, C.c_double, C.c_int,
                                      C.POINTER(_dp), _dp]),
+    "fg_elec_map_async": (C.c_int, [C.c_void_p, C.POINTER(FgGrid), C.POINTER(FgAtoms), _dp, C.c_int, C.c_int, _dp,
+                                    C.POINTER(C.c_void_p)]),
+    "fg_affinity_d_maps_async": (C.c_int, [C.c_void_p, C.POINTER(FgGrid), C.POINTER(FgAtoms),
+                                           C.POINTER(FgHbondGeom), C.POINTER(FgTables), C.POINTER(FgMapParams),
+                                           C.c_int, C.c_double, C.c_int, C.POINTER(_dp), _dp,
+                                           C.POINTER(C.c_void_p)]),
+    "fg_wait": (C.c_int, [C.c_void_p]),
     "fg_job_create": (C.c_int, [C.c_void_p, C.POINTER(FgJobDesc), C.POINTER(C.c_void_p)]),
     "fg_job_create_many": (C.c_int, [C.c_void_p, C.POINTER(FgJobDesc), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "fg_job_destroy": (None, [C.c_void_p]),
@@ -152,6 +159,7 @@ class FgError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"libfgmaps error {code}: {msg}")
         self.code = code
+        self.exit_code = None  # set by the GPF loaders: the reference's exit status
 
 
 def _load() -> C.CDLL:
@@ -293,8 +301,10 @@ class Prepared:
         opts = FgLoadOpts(1 if v4 else 0, 1 if ag421_param_order else 0)
         rc = LIB.fg_prep_load(gpf.encode(), C.byref(opts), C.byref(self._h))
         if rc != FG_OK:
-            self.exit_code = LIB.fg_prep_exit_code()
-            _check(rc)
+            # the reference's exit status for the failure (fg_prep_exit_code)
+            err = FgError(rc, LIB.fg_last_error().decode(errors="replace"))
+            err.exit_code = LIB.fg_prep_exit_code()
+            raise err
 
     @property
     def n_atoms(self) -> int:
@@ -455,11 +465,21 @@ class Context:
         return t
 
     def elec_map(self, arrays: JobArrays, precision: int = PREC_COMPENSATED) -> np.ndarray:
+        """The electrostatic map (fg_elec_map)."""
         d, keep = arrays.desc()
         out = np.zeros(arrays.P, np.float64)
         _check(LIB.fg_elec_map(self._h, C.byref(d.grid), C.byref(d.atoms), d.tables.eps, d.dist_dep_diel, precision,
                                _ptr(out, C.c_double)))
         return out
+
+    def elec_map_async(self, arrays: JobArrays, precision: int = PREC_COMPENSATED) -> "Pending":
+        """fg_elec_map_async: returns at once; .wait() gives the map."""
+        d, keep = arrays.desc()
+        out = np.zeros(arrays.P, np.float64)
+        h = C.c_void_p()
+        _check(LIB.fg_elec_map_async(self._h, C.byref(d.grid), C.byref(d.atoms), d.tables.eps, d.dist_dep_diel,
+                                     precision, _ptr(out, C.c_double), C.byref(h)))
+        return Pending(h, out)
 
     def covalent_map(self, arrays: JobArrays) -> np.ndarray:
         """Initial values of the covalent maps (fg_covalent_map)."""
@@ -477,9 +497,11 @@ class Context:
         _check(LIB.fg_floating_grid(self._h, C.byref(d.grid), C.byref(d.atoms), _ptr(out, C.c_float)))
         return out
 
-    def affinity_d_maps(self, arrays: JobArrays) -> tuple[list[np.ndarray], np.ndarray]:
-        """Affinity and desolvation maps (fg_affinity_d_maps).  Covalent maps
-        start from arrays.covalent, else from fg_covalent_map."""
+    def affinity_d_maps(self, arrays: JobArrays, asynchronous: bool = False):
+        """Affinity and desolvation maps (fg_affinity_d_maps): (maps, d).
+        Covalent maps start from arrays.covalent, else from fg_covalent_map.
+        asynchronous: fg_affinity_d_maps_async, returns a Pending whose
+        .wait() gives (maps, d)."""
         d, keep = arrays.desc()
         P, M = arrays.P, arrays.n_maps
         cov = None
@@ -490,9 +512,34 @@ class Context:
                 for m in range(M)]
         dmap = np.zeros(P)
         ptrs = (_dp * max(M, 1))(*[_ptr(o, C.c_double) for o in outs])
-        _check(LIB.fg_affinity_d_maps(self._h, C.byref(d.grid), C.byref(d.atoms), C.byref(d.geom), C.byref(d.tables),
-                                      d.maps, M, d.solpar_q, d.hydrogen, ptrs, _ptr(dmap, C.c_double)))
+        args = (self._h, C.byref(d.grid), C.byref(d.atoms), C.byref(d.geom), C.byref(d.tables), d.maps, M, d.solpar_q,
+                d.hydrogen, ptrs, _ptr(dmap, C.c_double))
+        if asynchronous:
+            h = C.c_void_p()
+            _check(LIB.fg_affinity_d_maps_async(*args, C.byref(h)))
+            return Pending(h, (outs, dmap))
+        _check(LIB.fg_affinity_d_maps(*args))
         return outs, dmap
+
+
+class Pending:
+    """An asynchronous one-shot computation (fg_handle): wait() returns its
+    outputs once fg_wait has written them."""
+
+    def __init__(self, handle, result):
+        self._h, self._result = handle, result
+
+    def wait(self):
+        if self._h:
+            h, self._h = self._h, C.c_void_p()
+            _check(LIB.fg_wait(h))
+        return self._result
+
+    def __del__(self):
+        try:
+            self.wait()
+        except Exception:
+            pass
 
 
 class Job:

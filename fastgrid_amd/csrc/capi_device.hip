@@ -66,6 +66,14 @@ struct fg_ctx {
     std::vector<fg_mapset *> mapsets;
 };
 
+// After a failure inside a capture that may have forked the side stream:
+// join it, so that ending the capture takes both streams out of capture mode.
+static void joinSide(fg_ctx *ctx)
+{
+    (void)hipEventRecord(ctx->join, ctx->side);
+    (void)hipStreamWaitEvent(ctx->stream, ctx->join, 0);
+}
+
 // An AutoDock map set in HBM: floats [z][y][x][slot].
 struct fg_mapset {
     fg_ctx *ctx = nullptr;
@@ -289,7 +297,11 @@ void launchCutoffKH(fg_job *j, const fgk::CutoffArgs &a, dim3 grid, dim3 block)
     hipStream_t s = j->ctx->stream;
     if (j->KH == 0) hipLaunchKernelGGL((fgk::k_cutoff<MM, 0, true, PA, TY>), grid, block, 0, s, a);
     else if (j->KH <= 2) hipLaunchKernelGGL((fgk::k_cutoff<MM, 2, true, PA, TY>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((fgk::k_cutoff<MM, 4, true, PA, TY>), grid, block, 0, s, a);
+    else if (MM < 16 || j->KH <= 4) hipLaunchKernelGGL((fgk::k_cutoff<MM, 4, true, PA, TY>), grid, block, 0, s, a);
+    // more than 4 maps with a min/max (repeated donor or A1 ligand types):
+    // a slot for every map (register spills; no GPF of AutoDock's own types
+    // needs it)
+    else if constexpr (MM == 16) hipLaunchKernelGGL((fgk::k_cutoff<16, 16, true, PA, TY>), grid, block, 0, s, a);
 }
 
 template <int MM, bool TY>
@@ -332,10 +344,10 @@ int launchCutoff(fg_job *j, bool affinity, int doElecNear)
     } else if (j->M <= 4) {
         if (j->typed) launchCutoffM<4, true>(j, a, grid, block);
         else launchCutoffM<4, false>(j, a, grid, block);
-    } else if (j->M <= 6) {
+    } else if (j->M <= 6 && j->KH <= 4) {
         if (j->typed) launchCutoffM<6, true>(j, a, grid, block);
         else launchCutoffM<6, false>(j, a, grid, block);
-    } else if (j->M <= 8) {
+    } else if (j->M <= 8 && j->KH <= 4) {
         if (j->typed) launchCutoffM<8, true>(j, a, grid, block);
         else launchCutoffM<8, false>(j, a, grid, block);
     } else {
@@ -902,7 +914,9 @@ int elecFill(const fg_job *j, int px, int pz, int bxShift, bool comp, double &fi
     int perCU = 0;
     FG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k, fgk::kElecThreads,
                                                          size_t(j->tableLen) * sizeof(float)));
-    const double slots = double(std::max(perCU, 1)) * double(j->ctx->numCUs);
+    // the persistent grid a batch launches: one slot per CU is left to its
+    // concurrent cutoff phase (launchElecBatch)
+    const double slots = double(std::max(perCU - 1, 1)) * double(j->ctx->numCUs);
     const double bricks = double(a.nbx) * a.nby * a.nbz;
     fill = bricks / (slots * std::ceil(bricks / slots));
     return FG_OK;
@@ -1161,7 +1175,9 @@ int launchCutoffBatchKH(fg_batch *b, bool launch)
 {
     if (b->KH == 0) return launchCutoffBatchPA<MM, 0>(b, launch);
     if (b->KH <= 2) return launchCutoffBatchPA<MM, 2>(b, launch);
-    return launchCutoffBatchPA<MM, 4>(b, launch);
+    if (MM < 16 || b->KH <= 4) return launchCutoffBatchPA<MM, 4>(b, launch);
+    if constexpr (MM == 16) return launchCutoffBatchPA<16, 16>(b, launch);
+    return FG_OK;
 }
 
 template <int PX, int PZ, bool kComp, bool kClamp>
@@ -1279,7 +1295,7 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
         b->px = v >= 2 ? 2 : 1;
         b->pz = v & 1 ? 8 : 4;
     }
-    b->MM = maxM <= 4 ? 4 : maxM <= 6 ? 6 : maxM <= 8 ? 8 : 16;
+    b->MM = maxM <= 4 ? 4 : b->KH > 4 ? 16 : maxM <= 6 ? 6 : maxM <= 8 ? 8 : 16;
     if (b->aff && maxM == 0) b->aff = false;
     // one LDS table for all jobs when their dielectric tables agree
     {
@@ -1330,6 +1346,7 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
     hipStream_t s = ctx->stream;
     FG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int rc = enqueueBatch(b, nullptr);
+    if (rc) joinSide(ctx);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s, &g);
     if (rc) {
@@ -1478,10 +1495,6 @@ int jobBuildHost(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
         bool add = true;
         for (int m = 0; m < j->M; ++m) add = add && ((actions[size_t(m)] >> (2 * h)) & 3) == fgk::kActAdd;
         if (add) j->addOnlyClasses |= 1 << h;
-    }
-    if (j->KH > fgk::kMaxHbSlots) {
-        destroyJob(j);
-        return fgc::fail(FG_E_ARG, "fg_job_create: at most 4 maps of donor (DS/D1) or A1 probes are supported");
     }
     j->mp.resize(size_t(j->M));
     for (int k = 0; k < j->M; ++k) {
@@ -1693,12 +1706,15 @@ int fg_job_run(fg_ctx *ctx, fg_job *j, unsigned what, int precision)
         const bool phases = (what & FG_RUN_PHASE_TIMING) != 0;
         const int key[5] = {precision, j->px, j->pz, j->tile, j->bxShift};
         if (!j->runExec || j->runWhat != what || std::memcmp(key, j->runKey, sizeof key) != 0) {
+            // earlier replays of the old graph may still be queued
+            if (j->runExec) FG_HIP(hipStreamSynchronize(s));
             if (j->runExec) (void)hipGraphExecDestroy(j->runExec);
             if (j->runGraph) (void)hipGraphDestroy(j->runGraph);
             j->runExec = nullptr;
             j->runGraph = nullptr;
             FG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             const int rc = enqueueRun(ctx, j, what, precision, phases ? kGraphTimed : kGraphUntimed);
+            if (rc) joinSide(ctx);
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(s, &g);
             if (rc || e != hipSuccess) {
@@ -1886,10 +1902,48 @@ int fg_batch_run(fg_ctx *ctx, fg_job *const *jobs, int n, unsigned what, int pre
     return rc;
 }
 
-int fg_elec_map(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, const double *eps, int ddd, int precision,
-                double *e_out)
+// The one-shot entry points run a job: created from the caller's host
+// buffers (copied before the call returns), run on `streams` and fetched into
+// `outs` by fg_wait.  The asynchronous forms give the job a context of its own
+// (streams on the caller's device), so that work the caller enqueues on its
+// context meanwhile runs concurrently with it.
+struct fg_handle {
+    fg_ctx *own = nullptr;  // the handle's context (asynchronous forms), destroyed by fg_wait
+    fg_ctx *ctx = nullptr;  // the context the job runs on
+    fg_job *job = nullptr;
+    std::vector<double *> outs;
+};
+
+namespace {
+
+int startOneShot(fg_ctx *ctx, bool async, const fg_job_desc &d, unsigned what, int precision,
+                 std::vector<double *> outs, fg_handle **out)
 {
-    if (!ctx || !grid || !atoms || !e_out) return fgc::fail(FG_E_ARG, "fg_elec_map: NULL argument");
+    *out = nullptr;
+    fg_handle *h = new fg_handle();
+    h->ctx = ctx;
+    if (async) {
+        if (int rc = fg_create(ctx->device, &h->own)) {
+            delete h;
+            return rc;
+        }
+        h->ctx = h->own;
+    }
+    int rc = fg_job_create(h->ctx, &d, &h->job);
+    if (!rc) rc = fg_job_run(h->ctx, h->job, what, precision);
+    if (rc) {
+        if (h->job) fg_job_destroy(h->job);
+        if (h->own) fg_destroy(h->own);
+        delete h;
+        return rc;
+    }
+    h->outs = std::move(outs);
+    *out = h;
+    return FG_OK;
+}
+
+fg_job_desc elecDesc(const fg_grid *grid, const fg_atoms *atoms, const double *eps, int ddd)
+{
     fg_job_desc d{};
     d.grid = *grid;
     d.atoms.n = atoms->n;
@@ -1900,14 +1954,38 @@ int fg_elec_map(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, const d
     d.tables.eps = eps;
     d.dist_dep_diel = ddd;
     d.hydrogen = -1;
-    fg_job *j = nullptr;
-    if (int rc = fg_job_create(ctx, &d, &j)) return rc;
-    std::vector<double *> outs(2, nullptr);
-    outs[0] = e_out;
-    int rc = fg_job_run(ctx, j, FG_RUN_ELEC, precision);
-    if (!rc) rc = fg_job_fetch(ctx, j, outs.data());
-    fg_job_destroy(j);
+    return d;
+}
+
+}  // namespace
+
+int fg_wait(fg_handle *h)
+{
+    if (!h) return FG_OK;
+    int rc = fg_job_fetch(h->ctx, h->job, h->outs.data());
+    fg_job_destroy(h->job);
+    if (h->own) fg_destroy(h->own);
+    delete h;
     return rc;
+}
+
+int fg_elec_map_async(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, const double *eps, int ddd,
+                      int precision, double *e_out, fg_handle **out)
+{
+    if (!out) return fgc::fail(FG_E_ARG, "fg_elec_map_async: out is NULL");
+    *out = nullptr;
+    if (!ctx || !grid || !atoms || !e_out) return fgc::fail(FG_E_ARG, "fg_elec_map_async: NULL argument");
+    return startOneShot(ctx, true, elecDesc(grid, atoms, eps, ddd), FG_RUN_ELEC, precision, {nullptr, e_out}, out);
+}
+
+int fg_elec_map(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, const double *eps, int ddd, int precision,
+                double *e_out)
+{
+    if (!ctx || !grid || !atoms || !e_out) return fgc::fail(FG_E_ARG, "fg_elec_map: NULL argument");
+    fg_handle *h = nullptr;
+    if (int rc = startOneShot(ctx, false, elecDesc(grid, atoms, eps, ddd), FG_RUN_ELEC, precision, {nullptr, e_out}, &h))
+        return rc;
+    return fg_wait(h);
 }
 
 int fg_covalent_map(fg_ctx *ctx, const fg_grid *grid, const double cov_point[3], double hw_sq_inv, double barrier,
@@ -1958,12 +2036,16 @@ int fg_floating_grid(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, fl
     return rc;
 }
 
-int fg_affinity_d_maps(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, const fg_hbond_geom *geom,
-                       const fg_tables *tables, const fg_map_params *maps, int n_maps, double solpar_q, int hydrogen,
-                       double *const *maps_out, double *d_out)
+namespace {
+
+int affinityOneShot(fg_ctx *ctx, bool async, const fg_grid *grid, const fg_atoms *atoms, const fg_hbond_geom *geom,
+                    const fg_tables *tables, const fg_map_params *maps, int n_maps, double solpar_q, int hydrogen,
+                    double *const *maps_out, double *d_out, fg_handle **out)
 {
-    if (!ctx || !grid || !atoms || !geom || !tables || (!maps && n_maps > 0) || !maps_out || !d_out)
-        return fgc::fail(FG_E_ARG, "fg_affinity_d_maps: NULL argument");
+    const char *who = async ? "fg_affinity_d_maps_async" : "fg_affinity_d_maps";
+    if (!ctx || !grid || !atoms || !geom || !tables || (!maps && n_maps > 0) || (!maps_out && n_maps > 0) || !d_out)
+        return fgc::fail(FG_E_ARG, std::string(who) + ": NULL argument");
+    if (n_maps < 0 || n_maps > 14) return fgc::fail(FG_E_ARG, std::string(who) + ": n_maps must be 0..14");
     fg_job_desc d{};
     d.grid = *grid;
     d.atoms = *atoms;
@@ -1974,26 +2056,45 @@ int fg_affinity_d_maps(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, 
     d.solpar_q = solpar_q;
     d.hydrogen = hydrogen;
     d.dist_dep_diel = 0;
-    // covalent maps arrive as initial values in maps_out
+    // covalent maps arrive as initial values in maps_out (copied into the job
+    // before the call returns)
     const int64_t P = int64_t(grid->nx) * grid->ny * grid->nz;
     std::vector<double> cov;
     bool anyCov = false;
     for (int m = 0; m < n_maps; ++m) anyCov = anyCov || maps[m].is_covalent;
-    if (anyCov) {
+    if (anyCov && P > 0) {
         cov.assign(size_t(n_maps) * size_t(P), 0.0);
         for (int m = 0; m < n_maps; ++m)
             if (maps[m].is_covalent && maps_out[m]) std::copy(maps_out[m], maps_out[m] + P, cov.begin() + int64_t(m) * P);
         d.covalent = cov.data();
     }
-    fg_job *j = nullptr;
-    if (int rc = fg_job_create(ctx, &d, &j)) return rc;
     std::vector<double *> outs(size_t(n_maps) + 2, nullptr);
     for (int m = 0; m < n_maps; ++m) outs[size_t(m)] = maps_out[m];
     outs[size_t(n_maps) + 1] = d_out;
-    int rc = fg_job_run(ctx, j, FG_RUN_AFFINITY, FG_PREC_FP64);
-    if (!rc) rc = fg_job_fetch(ctx, j, outs.data());
-    fg_job_destroy(j);
-    return rc;
+    return startOneShot(ctx, async, d, FG_RUN_AFFINITY, FG_PREC_FP64, std::move(outs), out);
+}
+
+}  // namespace
+
+int fg_affinity_d_maps(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, const fg_hbond_geom *geom,
+                       const fg_tables *tables, const fg_map_params *maps, int n_maps, double solpar_q, int hydrogen,
+                       double *const *maps_out, double *d_out)
+{
+    fg_handle *h = nullptr;
+    if (int rc = affinityOneShot(ctx, false, grid, atoms, geom, tables, maps, n_maps, solpar_q, hydrogen, maps_out,
+                                 d_out, &h))
+        return rc;
+    return fg_wait(h);
+}
+
+int fg_affinity_d_maps_async(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, const fg_hbond_geom *geom,
+                             const fg_tables *tables, const fg_map_params *maps, int n_maps, double solpar_q,
+                             int hydrogen, double *const *maps_out, double *d_out, fg_handle **out)
+{
+    if (!out) return fgc::fail(FG_E_ARG, "fg_affinity_d_maps_async: out is NULL");
+    *out = nullptr;
+    return affinityOneShot(ctx, true, grid, atoms, geom, tables, maps, n_maps, solpar_q, hydrogen, maps_out, d_out,
+                           out);
 }
 
 // ------------------------------------------------------------------------

@@ -159,8 +159,14 @@ int main(int argc, char **argv)
     fg_load_opts lo{o.v4 ? 1 : 0, o.agOrder ? 1 : 0};
     fg_prep *prep = nullptr;
     if (fg_prep_load(o.gpf.c_str(), &lo, &prep) != FG_OK) {
-        int code = fg_prep_exit_code();
-        return fatal(code ? code : -2, fg_last_error());
+        const int code = fg_prep_exit_code();
+        const std::string what = fg_last_error();
+        if (code && code != -2) return fatal(code, what.c_str());
+        // a fatal GPF error: the message, then the reference's closing line
+        // (FG/InputDataLoader.cpp:757-780, FG/LogFile.cpp:147-149)
+        std::fprintf(log, "\n%s: ERROR:  %s\n", prog, what.c_str());
+        std::fprintf(stderr, "\n%s: ERROR:  %s\n", prog, what.c_str());
+        return fatal(-2, "Unsuccessful completion.");
     }
     std::fputs(fg_prep_gpf_echo(prep), log);
     const auto t1 = std::chrono::steady_clock::now();

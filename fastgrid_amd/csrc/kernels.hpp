@@ -65,7 +65,6 @@ constexpr int kAngNone = 0, kAngConst = 1, kAngPoint = 2;
 //   kActMinRacc keep min/max of E * (racc + (1 - racc) * rsph)  (A1 probe, donor atom)
 //   kActMinRdon keep min/max of E * (rdon + (1 - rdon) * rsph)  (donor probe, acceptor atom)
 constexpr int kActAdd = 0, kActDonor = 1, kActMinRacc = 2, kActMinRdon = 3;
-constexpr int kMaxHbSlots = 4;  // maps that keep an H-bond min/max per point
 
 // Per affinity map, uniform across the kernel, in kernel order: maps that keep
 // an H-bond min/max come first.

@@ -177,6 +177,27 @@ int fg_affinity_d_maps(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms,
                        const fg_map_params *maps, int n_maps, double solpar_q, int hydrogen,
                        double *const *maps_out, double *d_out);
 
+/* Asynchronous forms.  The reference starts its e-map on a thread of its own
+ * and computes the affinity maps meanwhile (FG/main.cpp:215-222):
+ * calculateElectrostaticMapAsync returns a handle and synchronizeCalculation
+ * waits for it (FG/electrostatics/Electrostatics.h:28-36,
+ * FG/electrostatics/Cuda.cpp:222-249).  fg_elec_map_async and
+ * fg_affinity_d_maps_async copy their inputs to the device, enqueue the
+ * computation on HIP streams of the handle's own (on ctx's device) and return
+ * at once, so work enqueued on ctx meanwhile (for instance fg_affinity_d_maps
+ * while an e-map runs) overlaps with it on the GPU.  The input buffers may be
+ * reused as soon as the call returns; the output buffers are written by
+ * fg_wait, which also frees the handle.  fg_wait(NULL) does nothing, as
+ * synchronizeCalculation(0) does.  Every handle must be waited for. */
+typedef struct fg_handle fg_handle;
+int fg_elec_map_async(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, const double *eps,
+                      int dist_dep_diel, int precision, double *e_out, fg_handle **out);
+int fg_affinity_d_maps_async(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms,
+                             const fg_hbond_geom *geom, const fg_tables *tables,
+                             const fg_map_params *maps, int n_maps, double solpar_q, int hydrogen,
+                             double *const *maps_out, double *d_out, fg_handle **out);
+int fg_wait(fg_handle *handle);
+
 /* Initial values of a covalent map.  Replaces initCovalentMaps
  * (FG/main.cpp:32-52): out[p] = barrier * (1 - exp(ln(1/2) * max(r, 1e-12))),
  * r = |cov_point - p|^2 * hw_sq_inv, cov_point grid-centred. */

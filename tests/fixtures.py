@@ -130,3 +130,50 @@ def value_digest(path):
 
 def text_digests(name, which):
     return json.load(open(os.path.join(GOLDEN, name, which.replace(".npz", ".text_sha256.json"))))
+
+
+AGTESTS = os.path.join(GOLDEN, "agtests")
+
+
+def agtests(tmp):
+    """AutoGrid 4.2.1's own test inputs (receptors, GPFs, parameter file) in
+    tmp; returns tmp and the fixture metadata (map names per x1hpv GPF, the
+    reference's exit status for the GPFs it refuses)."""
+    for f in os.listdir(AGTESTS):
+        if f.endswith((".gpf", ".pdbqt", ".dat")):
+            _copy(os.path.join(AGTESTS, f), os.path.join(tmp, f))
+    with gzip.open(os.path.join(AGTESTS, "x1hpv.pdbqt.gz"), "rb") as g:
+        open(os.path.join(tmp, "x1hpv.pdbqt"), "wb").write(g.read())
+    return tmp, json.load(open(os.path.join(AGTESTS, "agtests.json")))
+
+
+X1HPV = ["x1hpv_CASE0", "x1hpv_31", "x1hpv_hb4", "x1hpv_hb7"]
+HSG1 = ["hsg1_sm", "hsg1_sm_no_parameter_file", "hsg1_no_receptor_types"]
+HSG1_MAPS = ["A", "C", "HD", "N", "NA", "OA", "e", "d"]
+
+
+def hsg1_close(got, want):
+    """test_autogrid4.py:149-180: within 0.5 % where |value| > 0.3, else
+    within 0.01 (values in thousandths)."""
+    g, w = np.asarray(got) / 1000.0, np.asarray(want) / 1000.0
+    big = np.abs(g) > 0.30
+    rel = np.abs(g - w) / np.where(big, np.abs(g), 1.0) * 100.0
+    return bool(np.all(np.where(big, rel < 0.5, np.abs(g - w) < 0.01)))
+
+
+def z_slab(a, z0, nz):
+    """The same job restricted to the z-planes z0 .. z0+nz-1 (the points keep
+    their positions: div2 moves with the first plane); for oracle checks of
+    sampled planes of grids too large for the oracle."""
+    import copy
+    b = copy.copy(a)
+    b.n = (a.n[0], a.n[1], nz)
+    b.div2 = (a.div2[0], a.div2[1], a.div2[2] - z0)
+    assert a.covalent is None
+    return b
+
+
+def slab_of(v, a, z0, nz):
+    """Values of the planes z0 .. z0+nz-1 of a map of job a (x fastest)."""
+    plane = a.n[0] * a.n[1]
+    return v[z0 * plane:(z0 + nz) * plane]

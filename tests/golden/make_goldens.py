@@ -10,6 +10,10 @@ Fixtures (all data: inputs and expected outputs):
              AD4_parameters.dat: AutoGrid 4.2.1's shipped maps, and the maps the
              reference FastGrid binary writes for the same GPF (--v4), which
              differ on HD/OA because FastGrid reads parameter_file after the GPF.
+  agtests/   AutoGrid 4.2.1's own test cases (autogrid/Tests): x1hpv_CASE0-3 and the
+             hsg1_sm GPFs with their receptors and parameter file, PyAutoDock's
+             hsg1_sm maps, and the reference binary's maps for x1hpv_CASE0 and
+             edits of it (larger grid; 4 and 7 maps with an H-bond min/max).
   synth/     seeded synthetic receptors from the project generator and the maps
              the reference FastGrid binary (oracle/_ref/fastgrid4_ref, CPU path,
              MALLOC_PERTURB_=255 so its uninitialised bond vectors are zero)
@@ -177,6 +181,89 @@ def pgp():
     np.savez_compressed(os.path.join(out, "goldens_autogrid421_minus_fastgrid.npz"), **ag)
 
 
+AGT = os.path.join(REF, "autogrid_original_4_2_1_ref", "autogrid", "Tests")
+AGT_INPUTS = ["x1hpv_CASE0.gpf", "x1hpv_CASE1.gpf", "x1hpv_CASE2.gpf", "x1hpv_CASE3.gpf", "hsg1_sm.pdbqt",
+              "hsg1_sm.gpf", "hsg1_sm_no_parameter_file.gpf", "hsg1_no_receptor_types.gpf",
+              "hsg1_sm_minus_two_types.gpf", "AD4.1_bound.dat"]
+HSG1_MAPS = ["A", "C", "HD", "N", "NA", "OA", "e", "d"]
+# x1hpv_CASE0 (11 affinity maps) and edits of it: a larger grid, and ligand
+# types repeated so that 4 and 7 maps keep an H-bond min/max (HD, HS: donor
+# probes; NA: A1 acceptor probe).  Each edit list applies to CASE0's text.
+X1HPV_MAPS = "A C HD N NA OA SA CL F S BR".split()
+
+
+X1HPV = {
+    "x1hpv_CASE0": ([], X1HPV_MAPS),
+    "x1hpv_31": ([("npts 6 6 6", "npts 30 30 30")], X1HPV_MAPS),
+    "x1hpv_hb4": ([("npts 6 6 6", "npts 24 24 24")], "A C HD HS N NA OA SA HD".split()),
+    "x1hpv_hb7": ([("npts 6 6 6", "npts 20 20 20")], "A C HD HS N NA OA SA HD NA HS HD".split()),
+}
+# GPFs the reference refuses: exit status (FG/LogFile.cpp:147-149, FG/Exceptions.h:28-39)
+AGT_FAILING = ["x1hpv_CASE1.gpf", "x1hpv_CASE2.gpf", "x1hpv_CASE3.gpf", "hsg1_sm_minus_two_types.gpf"]
+
+
+def x1hpv_gpf(name):
+    """The text of an x1hpv GPF of X1HPV (CASE0 with its edits)."""
+    text = open(os.path.join(HERE, "agtests", "x1hpv_CASE0.gpf")).read()
+    edits, types = X1HPV[name]
+    for a, b in edits:
+        assert a in text, a
+        text = text.replace(a, b)
+    if types != X1HPV_MAPS:
+        # the new ligand_types line in place of the old one, the map lines
+        # in place of the old ones (before elecmap)
+        lines = [l for l in text.split("\n") if not l.startswith("map ")]
+        lines = [f"ligand_types {' '.join(types)}" if l.startswith("ligand_types") else l for l in lines]
+        i = next(k for k, l in enumerate(lines) if l.startswith("elecmap"))
+        lines[i:i] = [f"map x1hpv.{n}.map" for n in x1hpv_names(name)[:-2]]
+        text = "\n".join(lines)
+    return text
+
+
+def x1hpv_names(name):
+    """Map names: the ligand types, a repeated type numbered from 2."""
+    types = X1HPV[name][1]
+    return [f"{t}{'' if types[:i].count(t) == 0 else types[:i].count(t) + 1}" for i, t in enumerate(types)] + ["e", "d"]
+
+
+def agtests():
+    """AutoGrid 4.2.1's own test cases (autogrid/Tests/test_autogrid4.py):
+    inputs, PyAutoDock's hsg1_sm maps (the expected values of its hsg1 tests,
+    :149-180), the reference binary's maps for x1hpv_CASE0 and the edits in
+    X1HPV, and the exit status it gives the GPFs it refuses (:294-311)."""
+    out = os.path.join(HERE, "agtests")
+    os.makedirs(out, exist_ok=True)
+    for f in AGT_INPUTS:
+        shutil.copy(os.path.join(AGT, f), out)
+    with open(os.path.join(AGT, "x1hpv.pdbqt"), "rb") as f, \
+            gzip.GzipFile(os.path.join(out, "x1hpv.pdbqt.gz"), "wb", compresslevel=9, mtime=0) as g:
+        g.write(f.read())
+    py = {n: read_map(os.path.join(AGT, f"py_hsg1_sm.{n}.map"))[1] for n in HSG1_MAPS}
+    np.savez_compressed(os.path.join(out, "py_hsg1_sm.npz"), **py)
+    meta = {"failing": {}}
+    with tempfile.TemporaryDirectory() as d:
+        shutil.copy(os.path.join(AGT, "x1hpv.pdbqt"), d)
+        for f in ("hsg1_sm.pdbqt", "AD4.1_bound.dat"):
+            shutil.copy(os.path.join(AGT, f), d)
+        for name in X1HPV:
+            text = x1hpv_gpf(name)
+            open(os.path.join(d, name + ".gpf"), "w").write(text)
+            if name != "x1hpv_CASE0":
+                open(os.path.join(out, name + ".gpf"), "w").write(text)
+            run_ref(name + ".gpf", d)
+            names = x1hpv_names(name)
+            save_maps(os.path.join(out, f"{name}.npz"), [os.path.join(d, f"x1hpv.{n}.map") for n in names], names)
+            meta[name] = {"names": names}
+        for f in AGT_FAILING:
+            shutil.copy(os.path.join(AGT, f), d)
+            env = dict(os.environ, MALLOC_PERTURB_="255")
+            p = subprocess.run([REF_BIN, "-p", f, "-l", "fail.glg"], cwd=d, env=env, stdout=subprocess.DEVNULL,
+                               stderr=subprocess.DEVNULL)
+            last = [l for l in open(os.path.join(d, "fail.glg")).read().split("\n") if l.strip()][-1]
+            meta["failing"][f] = {"exit_status": p.returncode, "last_log_line": last.split(": ", 1)[1]}
+    json.dump(meta, open(os.path.join(out, "agtests.json"), "w"), indent=1)
+
+
 def synth():
     import fastgrid_amd as fg
     out = os.path.join(HERE, "synth")
@@ -202,4 +289,5 @@ if __name__ == "__main__":
     variants()
     pgp()
     synth()
+    agtests()
     print("goldens written under", HERE)
