@@ -271,6 +271,7 @@ fgk::CutoffArgs cutoffArgs(const fg_job *j, int doElecNear)
     a.vs = j->dVS;
     a.brickStart = j->dStart;
     a.brickList = j->typed ? j->dSorted : j->dList;
+    a.typed = j->typed ? 1 : 0;
     a.hStart = j->dHStart;
     a.hList = j->dHList;
     a.typeStart = j->dTypeStart;
@@ -351,7 +352,8 @@ int launchCutoff(fg_job *j, bool affinity, int doElecNear)
         if (j->typed) launchCutoffM<8, true>(j, a, grid, block);
         else launchCutoffM<8, false>(j, a, grid, block);
     } else {
-        launchCutoffM<16, false>(j, a, grid, block);
+        if (j->typed) launchCutoffM<16, true>(j, a, grid, block);
+        else launchCutoffM<16, false>(j, a, grid, block);
     }
     FG_HIP(hipGetLastError());
     return FG_OK;
@@ -685,9 +687,9 @@ int buildCutoff(fg_job *j, const fg_job_desc &d)
     if (int rc = devAlloc(j, j->dCounts, size_t(2) * size_t(j->nb))) return rc;
     if (int rc = devAlloc(j, j->dStart, size_t(j->nb) + 1)) return rc;
     if (int rc = devAlloc(j, j->dHStart, size_t(j->nb) + 1)) return rc;
-    // the typed cutoff kernel: up to 8 maps (its LDS holds one type's rows for
-    // 8 maps) and up to kMaxTypes receptor types
-    j->typed = j->hasGeom && j->M > 0 && j->M <= 8 && j->T >= 1 && j->T <= fgk::kMaxTypes;
+    // the typed cutoff kernel: up to kMaxTypes receptor types (its LDS holds
+    // one type's rows for up to 4 maps)
+    j->typed = j->hasGeom && j->M > 0 && j->T >= 1 && j->T <= fgk::kMaxTypes;
     if (j->typed) {
         std::vector<int32_t> ty(size_t(std::max(N, 1)), 0);
         for (int i = 0; i < N; ++i) ty[size_t(i)] = A.type[i];
@@ -1102,7 +1104,7 @@ struct fg_batch {
     std::vector<fg_job *> jobs;  // largest first
     bool aff = false, near = false, elec = false, comp = true, fmap = false;
     int MM = 4, KH = 0, px = 1, pz = 4;
-    bool pointAng = false, clamp = false, typed = true;
+    bool pointAng = false, clamp = false, typed = false;  // typed: some job has type-sorted lists
     int sharedTableLen = 0, maxTableLen = 0;
     int cutItems = 0, elecItems = 0, elecWGs = 0, farItems = 0;
     fgk::ListArgs *dList = nullptr;
@@ -1163,9 +1165,9 @@ int launchCutoffBatch(fg_batch *b, bool launch)
 template <int MM, int KH>
 int launchCutoffBatchPA(fg_batch *b, bool launch)
 {
-    if (MM <= 8 && b->typed)
-        return b->pointAng ? launchCutoffBatch<MM, KH, true, true, MM <= 8>(b, launch)
-                           : launchCutoffBatch<MM, KH, true, false, MM <= 8>(b, launch);
+    if (b->typed)
+        return b->pointAng ? launchCutoffBatch<MM, KH, true, true, true>(b, launch)
+                           : launchCutoffBatch<MM, KH, true, false, true>(b, launch);
     return b->pointAng ? launchCutoffBatch<MM, KH, true, true, false>(b, launch)
                        : launchCutoffBatch<MM, KH, true, false, false>(b, launch);
 }
@@ -1286,7 +1288,7 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
         b->pointAng = b->pointAng || j->pointAng;
         b->clamp = b->clamp || j->clamp;
         b->maxTableLen = std::max(b->maxTableLen, j->tableLen);
-        b->typed = b->typed && j->typed;
+        b->typed = b->typed || j->typed;
     }
     {
         int v = 0;
@@ -1315,7 +1317,12 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
     for (fg_job *j : b->jobs) {
         la.push_back(listArgs(j));
         ca.push_back(cutoffArgs(j, b->near ? 1 : 0));
-        if (!(b->typed && b->aff)) ca.back().brickList = j->dList;
+        // the typed kernel takes a job without type-sorted lists in list
+        // order, as that job's own (untyped) kernel does: same bits
+        if (!(b->typed && b->aff && j->typed)) {
+            ca.back().brickList = j->dList;
+            ca.back().typed = 0;
+        }
         // two points per lane along x need a 2D brick (the shape does not
         // change the results)
         const int shift = b->px == 2 && j->bxShift == 0 ? 4 : j->bxShift;
@@ -1942,6 +1949,7 @@ int startOneShot(fg_ctx *ctx, bool async, const fg_job_desc &d, unsigned what, i
     return FG_OK;
 }
 
+// An e-map job has no affinity maps: its outputs are [e, d].
 fg_job_desc elecDesc(const fg_grid *grid, const fg_atoms *atoms, const double *eps, int ddd)
 {
     fg_job_desc d{};
@@ -1975,7 +1983,7 @@ int fg_elec_map_async(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, c
     if (!out) return fgc::fail(FG_E_ARG, "fg_elec_map_async: out is NULL");
     *out = nullptr;
     if (!ctx || !grid || !atoms || !e_out) return fgc::fail(FG_E_ARG, "fg_elec_map_async: NULL argument");
-    return startOneShot(ctx, true, elecDesc(grid, atoms, eps, ddd), FG_RUN_ELEC, precision, {nullptr, e_out}, out);
+    return startOneShot(ctx, true, elecDesc(grid, atoms, eps, ddd), FG_RUN_ELEC, precision, {e_out, nullptr}, out);
 }
 
 int fg_elec_map(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, const double *eps, int ddd, int precision,
@@ -1983,7 +1991,7 @@ int fg_elec_map(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, const d
 {
     if (!ctx || !grid || !atoms || !e_out) return fgc::fail(FG_E_ARG, "fg_elec_map: NULL argument");
     fg_handle *h = nullptr;
-    if (int rc = startOneShot(ctx, false, elecDesc(grid, atoms, eps, ddd), FG_RUN_ELEC, precision, {nullptr, e_out}, &h))
+    if (int rc = startOneShot(ctx, false, elecDesc(grid, atoms, eps, ddd), FG_RUN_ELEC, precision, {e_out, nullptr}, &h))
         return rc;
     return fg_wait(h);
 }

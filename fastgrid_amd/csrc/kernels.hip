@@ -383,7 +383,7 @@ constexpr int kMaxTypes = 32;  // receptor types the type-sorted lists support
 template <int MM>
 constexpr int ldsMaps()
 {
-    return MM / 2 > 0 ? MM / 2 : 1;
+    return MM / 2 > 4 ? 4 : MM / 2 > 0 ? MM / 2 : 1;
 }
 template <int MM, bool kAffinity, bool kTyped>
 struct CutoffLds {
@@ -478,11 +478,14 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
     double dmap = 0, enear = 0, sv = 0, sq = 0;
 
     // candidates: one run over the brick's list, or one run per receptor
-    // type over the type-sorted list with that type's rows in LDS
-    const int nRuns = kTyped ? a.T : 1;
+    // type over the type-sorted list with that type's rows in LDS (a batch's
+    // typed kernel also serves jobs without sorted lists: one run, rows read
+    // from the table, the same values)
+    const bool typedRun = kTyped && a.typed;
+    const int nRuns = typedRun ? a.T : 1;
     for (int run = 0; run < nRuns; ++run) {
         int c0, c1;
-        if (kTyped) {
+        if (typedRun) {
             c0 = constView(a.typeStart)[b * (a.T + 1) + run];
             c1 = constView(a.typeStart)[b * (a.T + 1) + run + 1];
             if (c0 == c1) continue;
@@ -553,7 +556,7 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                 }
                 const double S = a.desolv[idx];
                 const double *grow = a.pair + (int64_t(type) * kTableLen + idx) * M;
-                const double *lrow = kTyped ? &L.row[idx * ldsMaps<MM>()] : grow;
+                const double *lrow = typedRun ? &L.row[idx * ldsMaps<MM>()] : grow;
                 // map m's entry: LDS for the first MM/2 maps of the typed kernel
 #define FG_ROW(m) ((kTyped && (m) < ldsMaps<MM>()) ? lrow[(m)] : grow[(m)])
                 const double *row = grow;
@@ -1181,7 +1184,9 @@ __global__ __launch_bounds__(kListThreads) void k_type_sort_batch(const ListArgs
     __shared__ int32_t cnt[kMaxTypes], runStart[kMaxTypes], wcnt[4 * kMaxTypes];
     const int item = int(blockIdx.x);
     const int job = uniform(jobOfItem(base, nJobs, item));
-    typeSortBrick(loadConst(jobs + job), item - base[job], cnt, runStart, wcnt);
+    const ListArgs a = loadConst(jobs + job);
+    if (!a.sorted) return;  // a job without type-sorted lists
+    typeSortBrick(a, item - base[job], cnt, runStart, wcnt);
 }
 
 // k_add_far for every job: items are (job, 256-point chunk) pairs.
