@@ -21,6 +21,7 @@ RUN_ELEC, RUN_AFFINITY, RUN_FLOATING, RUN_ALL = 1, 2, 4, 7
 RUN_PHASE_TIMING = 8  # fg_timing's elec_ms/affinity_ms for compensated and FP32 runs
 BATCH_NO_GRAPH = 1
 TABLE_LEN = 8192
+NOALLOC_SCRATCH_BYTES = 32 << 20  # FG_NOALLOC_SCRATCH_BYTES
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -152,6 +153,8 @@ SIGNATURES = {
     "fg_mapset_destroy": (None, [C.c_void_p]),
     "fg_trilinterp": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FgLigand), C.c_int, C.POINTER(C.c_float),
                                 C.POINTER(C.c_float), C.POINTER(C.c_float), _dp, _dp, C.POINTER(C.c_float)]),
+    "fg_trilinterp_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FgLigand), C.c_int, C.POINTER(C.c_float),
+                                        C.POINTER(C.c_float), C.POINTER(C.c_float), _dp, _dp, C.POINTER(C.c_float)]),
 }
 
 
@@ -729,10 +732,11 @@ class MapSet:
 
 
 def trilinterp(ctx: Context, ms: MapSet, xyz: np.ndarray, types, charge, abs_charge, elec_slot: int,
-               desolv_slot: int, ignore=None, per_atom: bool = True) -> dict:
+               desolv_slot: int, ignore=None, per_atom: bool = True, noalloc: bool = False) -> dict:
     """AutoDock's trilinterp of every pose of xyz [pose][atom][3] on the GPU
-    (fg_trilinterp).  Returns elec/emap per atom (float32; None unless
-    per_atom), the two double totals and the returned float per pose."""
+    (fg_trilinterp, or fg_trilinterp_noalloc with noalloc).  Returns
+    elec/emap per atom (float32; None unless per_atom), the two double totals
+    and the returned float per pose."""
     xyz = np.ascontiguousarray(xyz, dtype=np.float32)
     npose, natoms = xyz.shape[0], xyz.shape[1]
     t = np.ascontiguousarray(types, dtype=np.int32)
@@ -745,7 +749,8 @@ def trilinterp(ctx: Context, ms: MapSet, xyz: np.ndarray, types, charge, abs_cha
     out = {"elec": np.zeros((npose, natoms), np.float32) if per_atom else None,
            "emap": np.zeros((npose, natoms), np.float32) if per_atom else None,
            "elec_total": np.zeros(npose), "emap_total": np.zeros(npose), "total": np.zeros(npose, np.float32)}
-    _check(LIB.fg_trilinterp(ctx.handle, ms.handle, C.byref(lig), npose, _ptr(xyz, C.c_float),
+    fn = LIB.fg_trilinterp_noalloc if noalloc else LIB.fg_trilinterp
+    _check(fn(ctx.handle, ms.handle, C.byref(lig), npose, _ptr(xyz, C.c_float),
                              _ptr(out["elec"], C.c_float), _ptr(out["emap"], C.c_float),
                              _ptr(out["elec_total"], C.c_double), _ptr(out["emap_total"], C.c_double),
                              _ptr(out["total"], C.c_float)))

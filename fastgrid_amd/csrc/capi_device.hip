@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <system_error>
@@ -42,6 +43,42 @@ struct DevBuf {
         if (e_ != hipSuccess)                                                                          \
             return fgc::fail(FG_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));          \
     } while (0)
+
+}  // namespace
+
+// Staging buffer of fg_trilinterp_noalloc: one fixed array per GPU, held
+// for the life of the process, so that the call allocates nothing.  On the
+// host this name only stands for the symbol: host code takes the array's
+// device address from hipGetSymbolAddress, once per device (scratchSetUp,
+// from fg_create with that device current), and uses that address only.
+static __device__ __align__(256) unsigned char scratch[FG_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+void *g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (which is `device`).
+int scratchSetUp(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return FG_OK;  // fg_trilinterp_noalloc then fails on it
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (g_scratch[device]) return FG_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fgc::fail(FG_E_DEVICE, std::string("fg_create: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = p;
+    return FG_OK;
+}
+
+void *scratchOf(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    return g_scratch[device];
+}
 
 }  // namespace
 
@@ -1400,6 +1437,10 @@ int fg_create(int device, fg_ctx **out)
     if (hipDeviceGetAttribute(&c->numCUs, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
         c->numCUs <= 0)
         c->numCUs = 256;
+    if (int rc = scratchSetUp(device)) {
+        fg_destroy(c);
+        return rc;
+    }
     *out = c;
     return FG_OK;
 }
@@ -2230,44 +2271,66 @@ void fg_mapset_destroy(fg_mapset *ms)
     delete ms;
 }
 
-int fg_trilinterp(fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int npose, const float *xyz, float *elec,
-                  float *emap, double *elecTotal, double *emapTotal, float *total)
+namespace {
+
+// Device staging of one trilinterp call, one buffer: the poses and the
+// ligand's arrays going in, the per-atom and per-pose outputs coming back.
+struct InterpLayout {
+    size_t xyz, type, q, aq, ign, elec, emap, et, mt, tot, bytes;
+};
+
+InterpLayout interpLayout(int64_t n, int A, int npose, bool elec, bool emap)
 {
-    if (!ctx || !ms || !lig || (npose > 0 && !xyz)) return fgc::fail(FG_E_ARG, "fg_trilinterp: NULL argument");
-    if (ms->ctx != ctx) return fgc::fail(FG_E_ARG, "fg_trilinterp: map set does not belong to this context");
+    auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
+    InterpLayout L{};
+    size_t off = 0;
+    L.xyz = off; off = align(off + size_t(n) * 3 * sizeof(float));
+    L.type = off; off = align(off + size_t(A) * sizeof(int32_t));
+    L.q = off; off = align(off + size_t(A) * sizeof(float));
+    L.aq = off; off = align(off + size_t(A) * sizeof(float));
+    L.ign = off; off = align(off + size_t(A) * sizeof(int32_t));
+    L.elec = off; off = align(off + (elec ? size_t(n) * sizeof(float) : 0));
+    L.emap = off; off = align(off + (emap ? size_t(n) * sizeof(float) : 0));
+    L.et = off; off = align(off + size_t(npose) * sizeof(double));
+    L.mt = off; off = align(off + size_t(npose) * sizeof(double));
+    L.tot = off; off = align(off + size_t(npose) * sizeof(float));
+    L.bytes = off;
+    return L;
+}
+
+int interpCheck(const char *who, fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int npose, const float *xyz)
+{
+    const std::string w(who);
+    if (!ctx || !ms || !lig || (npose > 0 && !xyz)) return fgc::fail(FG_E_ARG, w + ": NULL argument");
+    if (ms->ctx != ctx) return fgc::fail(FG_E_ARG, w + ": map set does not belong to this context");
     const int A = lig->natoms, V = ms->nslots;
     if (A <= 0 || npose < 0 || !lig->type || !lig->charge || !lig->abs_charge)
-        return fgc::fail(FG_E_ARG, "fg_trilinterp: invalid ligand");
+        return fgc::fail(FG_E_ARG, w + ": invalid ligand");
     if (lig->elec_slot < 0 || lig->elec_slot >= V || lig->desolv_slot < 0 || lig->desolv_slot >= V)
-        return fgc::fail(FG_E_ARG, "fg_trilinterp: e/d slot out of range");
+        return fgc::fail(FG_E_ARG, w + ": e/d slot out of range");
     for (int i = 0; i < A; ++i)
-        if (lig->type[i] < 0 || lig->type[i] >= V) return fgc::fail(FG_E_ARG, "fg_trilinterp: atom type slot out of range");
-    if (npose == 0) return FG_OK;
+        if (lig->type[i] < 0 || lig->type[i] >= V) return fgc::fail(FG_E_ARG, w + ": atom type slot out of range");
+    if (int64_t(npose) * A > (int64_t(1) << 31)) return fgc::fail(FG_E_ARG, w + ": too many (pose, atom) pairs");
+    return FG_OK;
+}
+
+// Uploads the poses and the ligand into the staging buffer b (device memory
+// laid out as L), interpolates, and copies the outputs back; returns when
+// they are on the host.  Every output the kernels write is written whole, so
+// b needs no clearing.
+int interpRun(const char *who, fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int npose, const float *xyz,
+              float *elec, float *emap, double *elecTotal, double *emapTotal, float *total, char *b,
+              const InterpLayout &L)
+{
+    const int A = lig->natoms, V = ms->nslots;
     const int64_t n = int64_t(npose) * A;
-    if (n > (int64_t(1) << 31)) return fgc::fail(FG_E_ARG, "fg_trilinterp: too many (pose, atom) pairs");
-    FG_HIP(hipSetDevice(ctx->device));
-    // one allocation: xyz, per-atom inputs, per-atom outputs, totals
-    auto align = [](size_t b) { return (b + 255) & ~size_t(255); };
-    size_t off = 0;
-    const size_t oXyz = off; off = align(off + size_t(n) * 3 * sizeof(float));
-    const size_t oType = off; off = align(off + size_t(A) * sizeof(int32_t));
-    const size_t oQ = off; off = align(off + size_t(A) * sizeof(float));
-    const size_t oAQ = off; off = align(off + size_t(A) * sizeof(float));
-    const size_t oIgn = off; off = align(off + size_t(A) * sizeof(int32_t));
-    const size_t oElec = off; off = align(off + (elec ? size_t(n) * sizeof(float) : 0));
-    const size_t oEmap = off; off = align(off + (emap ? size_t(n) * sizeof(float) : 0));
-    const size_t oET = off; off = align(off + size_t(npose) * sizeof(double));
-    const size_t oMT = off; off = align(off + size_t(npose) * sizeof(double));
-    const size_t oTot = off; off = align(off + size_t(npose) * sizeof(float));
-    char *b = nullptr;
-    FG_HIP(hipMalloc(&b, off));
     hipStream_t s = ctx->stream;
     auto up = [&](size_t o, const void *src, size_t bytes) { return hipMemcpyAsync(b + o, src, bytes, hipMemcpyHostToDevice, s); };
-    hipError_t e = up(oXyz, xyz, size_t(n) * 3 * sizeof(float));
-    if (e == hipSuccess) e = up(oType, lig->type, size_t(A) * sizeof(int32_t));
-    if (e == hipSuccess) e = up(oQ, lig->charge, size_t(A) * sizeof(float));
-    if (e == hipSuccess) e = up(oAQ, lig->abs_charge, size_t(A) * sizeof(float));
-    if (e == hipSuccess && lig->ignore) e = up(oIgn, lig->ignore, size_t(A) * sizeof(int32_t));
+    hipError_t e = up(L.xyz, xyz, size_t(n) * 3 * sizeof(float));
+    if (e == hipSuccess) e = up(L.type, lig->type, size_t(A) * sizeof(int32_t));
+    if (e == hipSuccess) e = up(L.q, lig->charge, size_t(A) * sizeof(float));
+    if (e == hipSuccess) e = up(L.aq, lig->abs_charge, size_t(A) * sizeof(float));
+    if (e == hipSuccess && lig->ignore) e = up(L.ign, lig->ignore, size_t(A) * sizeof(int32_t));
     fgk::InterpArgs a{};
     for (int k = 0; k < 3; ++k) {
         a.lo[k] = ms->g.lo[k];
@@ -2284,16 +2347,16 @@ int fg_trilinterp(fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int np
     a.elecSlot = lig->elec_slot;
     a.desolvSlot = lig->desolv_slot;
     a.cells = ms->cells;
-    a.xyz = reinterpret_cast<const float *>(b + oXyz);
-    a.type = reinterpret_cast<const int32_t *>(b + oType);
-    a.charge = reinterpret_cast<const float *>(b + oQ);
-    a.absq = reinterpret_cast<const float *>(b + oAQ);
-    a.ignore = lig->ignore ? reinterpret_cast<const int32_t *>(b + oIgn) : nullptr;
-    a.elec = elec ? reinterpret_cast<float *>(b + oElec) : nullptr;
-    a.emap = emap ? reinterpret_cast<float *>(b + oEmap) : nullptr;
-    a.elecTotal = reinterpret_cast<double *>(b + oET);
-    a.emapTotal = reinterpret_cast<double *>(b + oMT);
-    a.total = reinterpret_cast<float *>(b + oTot);
+    a.xyz = reinterpret_cast<const float *>(b + L.xyz);
+    a.type = reinterpret_cast<const int32_t *>(b + L.type);
+    a.charge = reinterpret_cast<const float *>(b + L.q);
+    a.absq = reinterpret_cast<const float *>(b + L.aq);
+    a.ignore = lig->ignore ? reinterpret_cast<const int32_t *>(b + L.ign) : nullptr;
+    a.elec = elec ? reinterpret_cast<float *>(b + L.elec) : nullptr;
+    a.emap = emap ? reinterpret_cast<float *>(b + L.emap) : nullptr;
+    a.elecTotal = reinterpret_cast<double *>(b + L.et);
+    a.emapTotal = reinterpret_cast<double *>(b + L.mt);
+    a.total = reinterpret_cast<float *>(b + L.tot);
     if (e == hipSuccess) e = hipEventRecord(ctx->ev[0], s);
     if (e == hipSuccess) {
         if (A <= fgk::kInterpThreads) {
@@ -2309,17 +2372,49 @@ int fg_trilinterp(fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int np
     auto down = [&](void *dst, size_t o, size_t bytes) {
         return dst ? hipMemcpyAsync(dst, b + o, bytes, hipMemcpyDeviceToHost, s) : hipSuccess;
     };
-    if (e == hipSuccess) e = down(elec, oElec, size_t(n) * sizeof(float));
-    if (e == hipSuccess) e = down(emap, oEmap, size_t(n) * sizeof(float));
-    if (e == hipSuccess) e = down(elecTotal, oET, size_t(npose) * sizeof(double));
-    if (e == hipSuccess) e = down(emapTotal, oMT, size_t(npose) * sizeof(double));
-    if (e == hipSuccess) e = down(total, oTot, size_t(npose) * sizeof(float));
+    if (e == hipSuccess) e = down(elec, L.elec, size_t(n) * sizeof(float));
+    if (e == hipSuccess) e = down(emap, L.emap, size_t(n) * sizeof(float));
+    if (e == hipSuccess) e = down(elecTotal, L.et, size_t(npose) * sizeof(double));
+    if (e == hipSuccess) e = down(emapTotal, L.mt, size_t(npose) * sizeof(double));
+    if (e == hipSuccess) e = down(total, L.tot, size_t(npose) * sizeof(float));
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    (void)hipFree(b);
-    if (e != hipSuccess) return fgc::fail(FG_E_DEVICE, std::string("fg_trilinterp: ") + hipGetErrorString(e));
+    if (e != hipSuccess) return fgc::fail(FG_E_DEVICE, std::string(who) + ": " + hipGetErrorString(e));
     ctx->haveTiming = true;
     ctx->splitTiming = false;
     return FG_OK;
+}
+
+}  // namespace
+
+int fg_trilinterp(fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int npose, const float *xyz, float *elec,
+                  float *emap, double *elecTotal, double *emapTotal, float *total)
+{
+    if (int rc = interpCheck("fg_trilinterp", ctx, ms, lig, npose, xyz)) return rc;
+    if (npose == 0) return FG_OK;
+    FG_HIP(hipSetDevice(ctx->device));
+    const InterpLayout L = interpLayout(int64_t(npose) * lig->natoms, lig->natoms, npose, elec, emap);
+    char *b = nullptr;
+    FG_HIP(hipMalloc(&b, L.bytes));
+    const int rc = interpRun("fg_trilinterp", ctx, ms, lig, npose, xyz, elec, emap, elecTotal, emapTotal, total, b, L);
+    (void)hipFree(b);
+    return rc;
+}
+
+int fg_trilinterp_noalloc(fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int npose, const float *xyz,
+                          float *elec, float *emap, double *elecTotal, double *emapTotal, float *total)
+{
+    if (int rc = interpCheck("fg_trilinterp_noalloc", ctx, ms, lig, npose, xyz)) return rc;
+    if (npose == 0) return FG_OK;
+    const InterpLayout L = interpLayout(int64_t(npose) * lig->natoms, lig->natoms, npose, elec, emap);
+    if (L.bytes > size_t(FG_NOALLOC_SCRATCH_BYTES))
+        return fgc::fail(FG_E_ARG, "fg_trilinterp_noalloc: " + std::to_string(npose) + " poses of " +
+                                       std::to_string(lig->natoms) + " atoms need " + std::to_string(L.bytes) +
+                                       " bytes of staging, more than its fixed " +
+                                       std::to_string(FG_NOALLOC_SCRATCH_BYTES) + "; split the call");
+    char *b = static_cast<char *>(scratchOf(ctx->device));
+    if (!b) return fgc::fail(FG_E_DEVICE, "fg_trilinterp_noalloc: no staging buffer on this device");
+    FG_HIP(hipSetDevice(ctx->device));
+    return interpRun("fg_trilinterp_noalloc", ctx, ms, lig, npose, xyz, elec, emap, elecTotal, emapTotal, total, b, L);
 }
 
 }  // extern "C"

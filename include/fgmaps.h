@@ -420,6 +420,22 @@ typedef struct {
 int fg_trilinterp(fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int npose, const float *xyz, float *elec,
                   float *emap, double *elec_total, double *emap_total, float *total);
 
+/* fg_trilinterp that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight scoring loops of a docking search, harnesses that forbid
+ * hipMalloc).  Same arguments, results and errors as fg_trilinterp, bit for
+ * bit; the device staging of the poses and outputs lives in a fixed array
+ * of FG_NOALLOC_SCRATCH_BYTES per GPU that the library holds until the
+ * process ends (its address is taken when a context is created on the GPU).
+ * A call whose staging (per pose 20 bytes + 12 bytes per atom, plus 8 per
+ * atom for each per-atom output requested, and the ligand's arrays) would not
+ * fit fails with FG_E_ARG: split the poses.  All calls on one GPU share the
+ * array: calls that overlap on one GPU (from several contexts or threads)
+ * race and must be serialised by the caller; calls on one context are. */
+#define FG_NOALLOC_SCRATCH_BYTES (32u << 20)
+int fg_trilinterp_noalloc(fg_ctx *ctx, const fg_mapset *ms, const fg_ligand *lig, int npose, const float *xyz,
+                          float *elec, float *emap, double *elec_total, double *emap_total, float *total);
+
 /* ---------------------------------------------------------------- */
 /* Synthetic receptors (seeded; deterministic across machines)      */
 /* ---------------------------------------------------------------- */
