@@ -39,7 +39,7 @@ OBJS := $(addprefix $(OBJ)/,$(SRCS:.cpp=.o))
 
 AD       := $(REF)/autogrid_original_4_2_1_ref/autodock
 
-all: $(OUT)/fastgrid4_ref $(OUT)/testdiff $(OUT)/ref_trilinterp
+all: $(OUT)/fastgrid4_ref $(OUT)/testdiff $(OUT)/ref_trilinterp $(OUT)/fastgrid4_amd
 
 $(OBJ)/%.o: $(FG)/%.cpp
 	@mkdir -p $(dir $@)
@@ -59,3 +59,19 @@ $(OUT)/ref_trilinterp: $(AD)/trilinterp.cc oracle/ref_trilinterp_main.cpp
 	$(CXX) -o $@ $(OBJ)/ad_trilinterp.o $(OBJ)/ref_trilinterp_main.o -lm
 
 .PHONY: all
+
+# The reference's own front end with libfgmaps in place of its two map
+# backends (electrostatics/Cpu.cpp and CalculateGridmaps.cpp): the binding in
+# fastgrid_amd/integration/amd_backend.cpp compiled against the reference's
+# headers and linked with its other objects.  Checks that the drop-in
+# boundary compiles and links as a maintainer would use it, and serves the
+# GPU test of that binding.  Needs fastgrid_amd/libfgmaps.so (built first).
+AMD_OBJS := $(addprefix $(OBJ)/,$(filter-out electrostatics/Cpu.o CalculateGridmaps.o,$(SRCS:.cpp=.o)))
+
+$(OBJ)/amd_backend.o: fastgrid_amd/integration/amd_backend.cpp include/fgmaps.h
+	@mkdir -p $(OBJ)
+	$(CXX) $(CXXFLAGS) -Wall -Iinclude -c $< -o $@
+
+$(OUT)/fastgrid4_amd: $(AMD_OBJS) $(OBJ)/amd_backend.o fastgrid_amd/libfgmaps.so
+	$(CXX) -fopenmp -o $@ $(AMD_OBJS) $(OBJ)/amd_backend.o -Lfastgrid_amd -lfgmaps \
+	    -Wl,-rpath,'$$ORIGIN/../../fastgrid_amd' -lpthread -lm
