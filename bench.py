@@ -25,9 +25,15 @@ with r <= 8 A.  Reported beside it:
   * AutoDock's consumer of the maps: trilinear interpolation of 2^19
     synthetic 32-atom ligand poses in the L maps handed off in HBM
     (trilinterp), checked bit for bit against the oracle on a sample;
+  * the small (1,500 atoms, 41^3) and medium (8,000 atoms, 81^3)
+    configurations: every form's per-map error against the CPU oracle on the
+    same inputs, and the oracle's time for the whole map set;
+  * the stress size (250,000 atoms, 161^3): every form's time and error,
+    roofline fraction and HBM bytes of the electrostatic kernel;
   * the FP64 CPU oracle timed on a bounded sample of L (cpu_baseline).
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-batch] [--no-interp] [--stress]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-batch] [--no-interp] [--no-stress]
+                    [--no-small] [--no-real] [--no-e2e]
 """
 from __future__ import annotations
 
@@ -46,6 +52,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3      # MI355X FP32 vector peak
+PLAIN_VARIANTS = {0: "table (global memory)", 2: "in place (analytic)", 3: "texture (point-sampled table)"}
 HBM_PEAK_GBS = 8000.0
 FLOP_PER_ELEC_PAIR = 15       # algorithmic FLOPs of one (point, atom) Coulomb term
 WORKLOAD_L = dict(tag="L", seed=3, n_atoms=50000, shape="box", dims=(70.0, 70.0, 50.0), npts=(127, 127, 127),
@@ -289,6 +296,14 @@ def run_batch(fg, ctx, steps, warmup, n_pockets):
 
     t_plain = timed(seq(fg.PREC_PLAIN))
     log(f"sequential plain: {t_plain * 1e3:.2f} ms")
+    # the reference's own dielectric choice per pocket (all in place here:
+    # no pocket grid pads past 160 points)
+    for j in jobs:
+        j.plain_dielectric = fg.PLAIN_DIEL_AUTO
+    auto_kinds = sorted({PLAIN_VARIANTS[j.plain_dielectric] for j in jobs})
+    t_plain_auto = timed(seq(fg.PREC_PLAIN))
+    log(f"sequential plain, reference's dielectric choice: {t_plain_auto * 1e3:.2f} ms")
+    t_plain_best = min(t_plain, t_plain_auto)
     # one launch per pocket is latency-bound: tune each size class's median
     # job alone and give its shape to the class, then restore the batch's
     batch_shapes = [j.shape for j in jobs]
@@ -315,7 +330,7 @@ def run_batch(fg, ctx, steps, warmup, n_pockets):
         comp = j.fetch()
         j.run(fg.RUN_ALL, fg.PREC_FP64)
         ref = j.fetch()
-        j.run(fg.RUN_ALL, fg.PREC_PLAIN)
+        j.run(fg.RUN_ALL, fg.PREC_PLAIN)  # the reference's dielectric choice
         plain = j.fetch()
         names = BATCH["ligand_types"].split() + ["e", "d"]
         for form, out in (("compensated", comp), ("plain", plain)):
@@ -336,12 +351,18 @@ def run_batch(fg, ctx, steps, warmup, n_pockets):
                     "gpair_evals_per_s": round(total / t_batch / 1e9, 2),
                     "form": "compensated, one hipGraph of persistent kernels, autotuned shapes"},
         "sequential_plain": {"ms_per_batch": round(t_plain * 1e3, 3), "pockets_per_s": round(n_pockets / t_plain, 1),
-                             "gpair_evals_per_s": round(total / t_plain / 1e9, 2)},
+                             "gpair_evals_per_s": round(total / t_plain / 1e9, 2),
+                             "dielectric": "table (global memory)"},
+        "sequential_plain_auto": {"ms_per_batch": round(t_plain_auto * 1e3, 3),
+                                  "pockets_per_s": round(n_pockets / t_plain_auto, 1),
+                                  "gpair_evals_per_s": round(total / t_plain_auto / 1e9, 2),
+                                  "dielectric": auto_kinds},
         "sequential_compensated": {"ms_per_batch": round(t_seq * 1e3, 3),
                                    "pockets_per_s": round(n_pockets / t_seq, 1),
                                    "gpair_evals_per_s": round(total / t_seq / 1e9, 2),
                                    "form": "one launch per pocket, shapes tuned for one launch per size class"},
-        "speedup_vs_sequential_plain": round(t_plain / t_batch, 2),
+        "speedup_vs_sequential_plain": round(t_plain_best / t_batch, 2),
+        "speedup_vs_sequential_plain_basis": "the faster of sequential_plain and sequential_plain_auto",
         "speedup_vs_sequential_compensated": round(t_seq / t_batch, 2),
         "autotune_s": round(t_tune, 2),
         "end_to_end": e2e,
@@ -427,6 +448,60 @@ REAL = dict(pdbqt=os.path.join(ROOT, "tests", "golden", "colagen2", "macromol_1t
             ligand_types="A C HD N NA OA SA S")
 
 
+# The small and medium configurations: the same generator, 1,500 atoms in a
+# 24 A sphere on 41^3 (5 maps) and 8,000 atoms in a 40 A sphere on 81^3
+# (8 maps)
+WORKLOAD_S = dict(tag="S", seed=1, n_atoms=1500, shape="sphere", dims=24.0, npts=(41, 41, 41),
+                  ligand_types="A C HD N OA")
+WORKLOAD_M = dict(tag="M", seed=2, n_atoms=8000, shape="sphere", dims=40.0, npts=(81, 81, 81),
+                  ligand_types="A C HD N NA OA SA S")
+
+
+def run_small_medium(fg, ctx, steps, warmup, cpu=True):
+    """S and M: every GPU form's per-map max |error| against the CPU oracle's
+    FP64 maps of the same input files (the reference's CPU algorithm), the
+    GPU times per map set, and the oracle's own time for the whole map set on
+    the host cores."""
+    from oracle import oracle as orc
+    out = {}
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    for w in (WORKLOAD_S, WORKLOAD_M):
+        with tempfile.TemporaryDirectory() as wd:
+            prep, a = make_inputs(fg, wd, w)
+        names = prep.map_types()[:a.n_maps] + ["e", "d"]
+        t0 = time.perf_counter()
+        oe = orc.elec_map(a, threads=threads)
+        om, od = orc.affinity_maps(a, threads=threads)
+        t_cpu = time.perf_counter() - t0
+        ref = list(om) + [oe, od]
+        job = fg.Job(ctx, a)
+        pcut = job.cutoff_pairs()
+        res = {"workload": f"{w['tag']}: {a.n_atoms} atoms in a {w['dims']:g} A sphere, {w['npts'][0]}^3 grid @ "
+                           f"0.375 A, {a.n_maps} affinity maps + e + d, seed {w['seed']}",
+               "elec_pairs": a.n_atoms * a.P, "cutoff_pairs": pcut}
+        if cpu:
+            res["cpu_oracle"] = {"seconds_per_map_set": round(t_cpu, 3), "cores": threads, "kind": "port",
+                                 "gpair_evals_per_s": round((a.n_atoms * a.P + pcut) / t_cpu / 1e9, 3)}
+        job.plain_dielectric = fg.PLAIN_DIEL_AUTO
+        auto = PLAIN_VARIANTS[job.plain_dielectric]
+        for key, prec, diel in (("compensated", fg.PREC_COMPENSATED, None), ("plain", fg.PREC_PLAIN, fg.PLAIN_DIEL_TABLE),
+                                ("plain_auto", fg.PREC_PLAIN, fg.PLAIN_DIEL_AUTO), ("fp64", fg.PREC_FP64, None)):
+            if diel is not None:
+                job.plain_dielectric = diel
+            t, _, _ = time_steps(fg, ctx, job, steps, warmup, prec)
+            err = map_errors(names, job.fetch(), ref)
+            res[key] = {"ms_per_step": round(t * 1e3, 3),
+                        "max_abs_error_vs_cpu_oracle": {n: e["max_abs"] for n, e in err.items()},
+                        "max_abs_rounded_error_vs_cpu_oracle": {n: e["max_abs_rounded"] for n, e in err.items()}}
+            if key == "plain_auto":
+                res[key]["dielectric"] = auto
+        job.close()
+        log(f"{w['tag']}: " + ", ".join(f"{k} {res[k]['ms_per_step']} ms" for k in ("compensated", "plain", "plain_auto",
+                                                                                      "fp64")))
+        out[w["tag"]] = res
+    return out
+
+
 def run_real(fg, ctx, steps, warmup):
     """The real receptor: FP64 form as the error reference, autotuned
     compensated and plain forms timed."""
@@ -481,7 +556,9 @@ def run_workload(fg, ctx, w, steps, warmup, tune=True, interp=False, cpu=True):
         log("generating", w)
         prep, a = make_inputs(fg, wd, w)
     N, P, M = a.n_atoms, a.P, a.n_maps
+    t0 = time.perf_counter()
     job = fg.Job(ctx, a)
+    t_create = time.perf_counter() - t0
     pcut = job.cutoff_pairs()
     elec_pairs = N * P
     log(f"N={N} P={P} maps={M}+2 N*P={elec_pairs:.4e} P_cut={pcut:.4e}")
@@ -497,8 +574,16 @@ def run_workload(fg, ctx, w, steps, warmup, tune=True, interp=False, cpu=True):
         log("  " + line)
     names = prep.map_types()[:M] + ["e", "d"]
     res = {"N": N, "P": P, "M": M, "elec_pairs": elec_pairs, "cutoff_pairs": pcut, "fp64_elec_ms": t64.elec_ms,
-           "shape": shape, "tune_report": report}
-    for key, prec in (("plain", fg.PREC_PLAIN), ("fp32", fg.PREC_FP32), ("compensated", fg.PREC_COMPENSATED)):
+           "shape": shape, "tune_report": report, "job_create_s": t_create}
+    # the plain form twice: with the dielectric table in global memory, and
+    # with the dielectric variant the reference's CUDA path picks for this
+    # grid (in place up to 160 padded points per axis, texture beyond)
+    job.plain_dielectric = fg.PLAIN_DIEL_AUTO
+    res["plain_auto_variant"] = PLAIN_VARIANTS[job.plain_dielectric]
+    for key, prec, diel in (("plain", fg.PREC_PLAIN, fg.PLAIN_DIEL_TABLE), ("plain_auto", fg.PREC_PLAIN, fg.PLAIN_DIEL_AUTO),
+                            ("fp32", fg.PREC_FP32, None), ("compensated", fg.PREC_COMPENSATED, None)):
+        if diel is not None:
+            job.plain_dielectric = diel
         t, el, af = time_steps(fg, ctx, job, steps, warmup, prec)
         res[key] = {"t": t, "elec_ms": el, "cutoff_ms": af, "errors": map_errors(names, job.fetch(), ref)}
         log(f"{key}: {t * 1e3:.2f} ms/step (elec {el:.2f}, cutoff+lists {af:.2f})")
@@ -519,7 +604,8 @@ def main():
     ap.add_argument("--no-interp", action="store_true", help="skip the trilinear-interpolation line")
     ap.add_argument("--no-real", action="store_true", help="skip the real-receptor line")
     ap.add_argument("--pockets", type=int, default=BATCH["pockets"])
-    ap.add_argument("--stress", action="store_true", help="also run the 250k-atom / 161^3 stress size")
+    ap.add_argument("--no-stress", action="store_true", help="skip the 250k-atom / 161^3 stress size")
+    ap.add_argument("--no-small", action="store_true", help="skip the small and medium configurations")
     ap.add_argument("--elec-shapes", default="",
                     help="only launch the electrostatic kernel on L at these shapes, e.g. '4,32,16,1;4,32,8,2' "
                          "(pz,tile,bx,px), or 'all' for the autotuner's 42 shapes (for rocprofv3 counter runs)")
@@ -550,7 +636,8 @@ def main():
         print(json.dumps({"batch": run_batch(fg, ctx, max(1, args.steps // 2), 1, args.pockets)}), flush=True)
         return
     L, a = run_workload(fg, ctx, WORKLOAD_L, args.steps, args.warmup, interp=not args.no_interp, cpu=not args.no_cpu)
-    comp, plain, fp32 = L["compensated"], L["plain"], L["fp32"]
+    comp, plain, plain_auto, fp32 = L["compensated"], L["plain"], L["plain_auto"], L["fp32"]
+    fastest_plain = min(plain["t"], plain_auto["t"])
     tot_pairs = L["elec_pairs"] + L["cutoff_pairs"]
     value = tot_pairs / comp["t"] / 1e9
     far_pairs = L["elec_pairs"] - L["cutoff_pairs"]
@@ -584,13 +671,22 @@ def main():
                       "ms_per_step": round(plain["t"] * 1e3, 3), "elec_ms": round(plain["elec_ms"], 3),
                       "cutoff_ms": round(plain["cutoff_ms"], 3),
                       "form": "the reference's GPU method in FP32 (8 z-points per lane, 4000 atoms per launch, "
-                              "global dielectric table) and a per-point FP32 cutoff kernel"},
-        "speedup_vs_plain_gpu": round(plain["t"] / comp["t"], 2),
+                              "dielectric table in global memory, --cuda-ddd=g) and a per-point FP32 cutoff kernel"},
+        "plain_gpu_auto": {"value": round(tot_pairs / plain_auto["t"] / 1e9, 2), "unit": "Gpair-evals/s",
+                           "ms_per_step": round(plain_auto["t"] * 1e3, 3), "elec_ms": round(plain_auto["elec_ms"], 3),
+                           "cutoff_ms": round(plain_auto["cutoff_ms"], 3),
+                           "dielectric": L["plain_auto_variant"],
+                           "form": "the same with the dielectric variant the reference's CUDA path picks for this "
+                                   "grid (FG/electrostatics/Cuda.cpp:114-124)"},
+        "speedup_vs_plain_gpu": round(fastest_plain / comp["t"], 2),
+        "speedup_vs_plain_gpu_basis": "the faster of plain_gpu and plain_gpu_auto",
         "fp32_tiled": {"value": round(tot_pairs / fp32["t"] / 1e9, 2), "ms_per_step": round(fp32["t"] * 1e3, 3),
                        "elec_ms": round(fp32["elec_ms"], 3)},
         "kernel_ms": {"elec_far": round(comp["elec_ms"], 3), "cutoff_and_lists": round(comp["cutoff_ms"], 3),
                       "elec_fp64_form": round(L["fp64_elec_ms"], 3)},
-        "error_vs_fp64": {"compensated": comp["errors"], "fp32_tiled": fp32["errors"], "plain": plain["errors"]},
+        "error_vs_fp64": {"compensated": comp["errors"], "fp32_tiled": fp32["errors"], "plain": plain["errors"],
+                          "plain_auto": plain_auto["errors"]},
+        "job_create_s": round(L["job_create_s"], 4),
         "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                      "traffic": tr["bytes_per_launch"] if tr else None,
@@ -611,7 +707,9 @@ def main():
         line["real_receptor"] = run_real(fg, ctx, args.steps, args.warmup)
     if not args.no_batch:
         line["batch"] = run_batch(fg, ctx, max(1, args.steps // 2), 1, args.pockets)
-    if args.stress:
+    if not args.no_small:
+        line["small_medium"] = run_small_medium(fg, ctx, args.steps, args.warmup, cpu=not args.no_cpu)
+    if not args.no_stress:
         X, _ = run_workload(fg, ctx, WORKLOAD_X, max(1, args.steps // 5), 1)
         xt = X["elec_pairs"] + X["cutoff_pairs"]
         line["stress"] = {
@@ -622,8 +720,12 @@ def main():
                             "roofline_frac": round(FLOP_PER_ELEC_PAIR * (X["elec_pairs"] - X["cutoff_pairs"]) /
                                                    (X["compensated"]["elec_ms"] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},
             "plain": {"value": round(xt / X["plain"]["t"] / 1e9, 2), "ms_per_step": round(X["plain"]["t"] * 1e3, 2)},
+            "plain_auto": {"value": round(xt / X["plain_auto"]["t"] / 1e9, 2),
+                           "ms_per_step": round(X["plain_auto"]["t"] * 1e3, 2), "dielectric": X["plain_auto_variant"]},
+            "speedup_vs_plain_gpu": round(min(X["plain"]["t"], X["plain_auto"]["t"]) / X["compensated"]["t"], 2),
+            "job_create_s": round(X["job_create_s"], 3),
             "fp32_tiled": {"value": round(xt / X["fp32"]["t"] / 1e9, 2)},
-            "error_vs_fp64": {k: X[k]["errors"] for k in ("compensated", "fp32", "plain")},
+            "error_vs_fp64": {k: X[k]["errors"] for k in ("compensated", "fp32", "plain", "plain_auto")},
             "shape": X["shape"],
         }
         xt_tr = traffic_entry(X["shape"], "X")

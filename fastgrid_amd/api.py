@@ -20,6 +20,8 @@ PREC_PLAIN, PREC_COMPENSATED, PREC_FP64, PREC_FP32 = 0, 1, 2, 3
 RUN_ELEC, RUN_AFFINITY, RUN_FLOATING, RUN_ALL = 1, 2, 4, 7
 RUN_PHASE_TIMING = 8  # fg_timing's elec_ms/affinity_ms for compensated and FP32 runs
 BATCH_NO_GRAPH = 1
+# dielectric variants of the plain form's e-kernel (FG_PLAIN_DIEL_*)
+PLAIN_DIEL_TABLE, PLAIN_DIEL_AUTO, PLAIN_DIEL_INPLACE, PLAIN_DIEL_TEXTURE = 0, 1, 2, 3
 TABLE_LEN = 8192
 NOALLOC_SCRATCH_BYTES = 32 << 20  # FG_NOALLOC_SCRATCH_BYTES
 
@@ -120,6 +122,8 @@ SIGNATURES = {
     "fg_job_cutoff_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]),
     "fg_job_get_shape": (C.c_int, [C.c_void_p, C.POINTER(FgShape)]),
     "fg_job_set_shape": (C.c_int, [C.c_void_p, C.POINTER(FgShape)]),
+    "fg_job_set_plain_dielectric": (C.c_int, [C.c_void_p, C.c_int]),
+    "fg_job_plain_dielectric": (C.c_int, [C.c_void_p]),
     "fg_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, C.c_int]),
     "fg_tune_report": (C.c_char_p, [C.c_void_p]),
     "fg_batch_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, C.c_uint, C.c_int, C.c_uint,
@@ -586,6 +590,18 @@ class Job:
         v = C.c_int64()
         _check(LIB.fg_job_cutoff_pairs(self.ctx.handle, self._h, C.byref(v)))
         return int(v.value)
+
+    @property
+    def plain_dielectric(self) -> int:
+        """The plain form's dielectric variant that runs (PLAIN_DIEL_*)."""
+        v = LIB.fg_job_plain_dielectric(self._h)
+        if v < 0:
+            _check(v)
+        return v
+
+    @plain_dielectric.setter
+    def plain_dielectric(self, kind: int) -> None:
+        _check(LIB.fg_job_set_plain_dielectric(self._h, int(kind)))
 
     @property
     def shape(self) -> tuple[int, int, int, int]:

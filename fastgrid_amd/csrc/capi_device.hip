@@ -161,6 +161,7 @@ struct fg_job {
     // k_elec shape: points per lane along x and z, atoms per tile, lanes
     // along x (defaults; the autotuner may change them)
     int px = 1, pz = 4, tile = 64, bxShift = 4;
+    int plainDiel = FG_PLAIN_DIEL_TABLE;  // dielectric variant of the plain e-kernel
     float farCut2 = 0, lastCut2 = 0, tableCut2 = 0, r2Lo = 0, r2Hi = 0, spacing100 = 0;
     // plain form (FP32 throughout)
     float4 *dPlainQ = nullptr;          // (x, y, z, w) per atom, index order
@@ -858,21 +859,44 @@ int runPlainAffinity(fg_job *j)
     return FG_OK;
 }
 
+// The dielectric variant the reference's CUDA path would run for this job:
+// the one asked for, or its automatic choice (FG/electrostatics/Cuda.cpp:
+// 114-124): in place when no axis of the grid padded to its 16 x 8 x (8 z)
+// blocks (:43-55) exceeds 160 points, else the texture.
+int plainDielKind(const fg_job *j)
+{
+    if (!j->distDepDiel) return fgk::kPlainConst;
+    switch (j->plainDiel) {
+    case FG_PLAIN_DIEL_INPLACE: return fgk::kPlainInPlace;
+    case FG_PLAIN_DIEL_TEXTURE: return fgk::kPlainTexture;
+    case FG_PLAIN_DIEL_AUTO: {
+        auto align = [](int v, int m) { return (v + m - 1) / m * m; };
+        const int mx = std::max({align(j->g.nx, fgk::kPlainBX), align(j->g.ny, fgk::kPlainBY), align(j->g.nz, fgk::kPlainPZ)});
+        return mx > 160 ? fgk::kPlainTexture : fgk::kPlainInPlace;
+    }
+    default: return fgk::kPlainTable;
+    }
+}
+
 int runPlainElec(fg_job *j, hipStream_t s)
 {
     fgk::PlainElecArgs a{};
     a.g = j->g;
-    a.ddd = j->distDepDiel;
     a.atoms = j->dPlainQ;
     a.eps = j->dEps32;
     a.acc = j->dAcc32;
     const dim3 grid(unsigned((j->g.nx + fgk::kPlainBX - 1) / fgk::kPlainBX),
                     unsigned((j->g.ny + fgk::kPlainBY - 1) / fgk::kPlainBY),
                     unsigned((j->g.nz + fgk::kPlainPZ - 1) / fgk::kPlainPZ));
+    const int kind = plainDielKind(j);
+    auto k = kind == fgk::kPlainConst     ? fgk::k_plain_elec<fgk::kPlainConst>
+             : kind == fgk::kPlainInPlace ? fgk::k_plain_elec<fgk::kPlainInPlace>
+             : kind == fgk::kPlainTexture ? fgk::k_plain_elec<fgk::kPlainTexture>
+                                          : fgk::k_plain_elec<fgk::kPlainTable>;
     for (int a0 = 0; a0 < j->N; a0 += fgk::kPlainAtomsPerLaunch) {
         a.a0 = a0;
         a.a1 = std::min(j->N, a0 + fgk::kPlainAtomsPerLaunch);
-        hipLaunchKernelGGL(fgk::k_plain_elec, grid, dim3(fgk::kPlainThreads), 0, s, a);
+        hipLaunchKernelGGL(k, grid, dim3(fgk::kPlainThreads), 0, s, a);
     }
     hipLaunchKernelGGL(fgk::k_plain_elec_finish, dim3(unsigned((j->P + 255) / 256)), dim3(256), 0, s, j->P, j->dAcc32,
                        j->dMaps + int64_t(j->M) * j->P);
@@ -1880,6 +1904,25 @@ int fg_job_set_shape(fg_job *job, const fg_shape *sh)
     job->tile = sh->tile;
     job->bxShift = shift;
     return FG_OK;
+}
+
+int fg_job_set_plain_dielectric(fg_job *job, int kind)
+{
+    if (!job) return fgc::fail(FG_E_ARG, "fg_job_set_plain_dielectric: NULL job");
+    if (kind < FG_PLAIN_DIEL_TABLE || kind > FG_PLAIN_DIEL_TEXTURE)
+        return fgc::fail(FG_E_ARG, "fg_job_set_plain_dielectric: unknown variant");
+    job->plainDiel = kind;
+    return FG_OK;
+}
+
+int fg_job_plain_dielectric(const fg_job *job)
+{
+    if (!job) return fgc::fail(FG_E_ARG, "fg_job_plain_dielectric: NULL job");
+    switch (plainDielKind(job)) {
+    case fgk::kPlainInPlace: return FG_PLAIN_DIEL_INPLACE;
+    case fgk::kPlainTexture: return FG_PLAIN_DIEL_TEXTURE;
+    default: return FG_PLAIN_DIEL_TABLE;
+    }
 }
 
 int fg_autotune(fg_ctx *ctx, fg_job *const *jobs, int n, int precision)

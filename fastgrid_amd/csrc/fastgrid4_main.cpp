@@ -4,10 +4,13 @@
 // outputs (.map/.fld/.xyz and a .glg log ending in "Successful Completion"),
 // same orchestration (FG/main.cpp:115-254): parse -> precompute -> maps ->
 // write.  The maps are computed by libfgmaps on the GPU; there is no CPU map
-// path, so --no-cuda is refused.  Switches that tune the reference's CUDA
-// kernels (--cuda-ddd, --cuda-unroll, --cuda-thread, --cuda-slices) and its
-// CPU search structures (--no-cogrid, --no-nns, --cogrid-mem) are accepted
-// and have no effect on the maps.
+// path, so --no-cuda is refused.  --cuda-ddd=g|c|i|t picks the dielectric
+// variant of the plain form (--precision=plain), as it picks the reference's
+// CUDA kernel variant; without it the plain form makes the reference's own
+// choice.  The other switches that tune the reference's CUDA kernels
+// (--cuda-unroll, --cuda-thread, --cuda-slices) and its CPU search structures
+// (--no-cogrid, --no-nns, --cogrid-mem) are accepted and have no effect on
+// the maps.
 //
 // Additional switches:
 //   --precision=plain|compensated|fp64|fp32   computation form (default compensated)
@@ -31,6 +34,7 @@ struct Options {
     std::string program, gpf, log;
     bool benchmark = false, v4 = false, agOrder = false, noCuda = false;
     int device = 0, threads = 0, debug = 0, precision = FG_PREC_COMPENSATED;
+    int plainDiel = FG_PLAIN_DIEL_AUTO;  // --cuda-ddd=: the plain form's dielectric variant
 };
 
 [[noreturn]] void usage(const char *prog)
@@ -55,7 +59,8 @@ struct Options {
                  "Advanced:\n"
                  "      --precision=plain|compensated|fp64|fp32  computation form, default: compensated\n"
                  "      --ag421-param-order  read parameter_file where the GPF names it (AutoGrid 4.2.1)\n"
-                 "      --cuda-ddd=g|c|t|i, --cuda-slices=y|n, --cuda-thread=y|n, --cuda-unroll=y|n,\n"
+                 "      --cuda-ddd=g|c|t|i  dielectric variant of --precision=plain (default: the reference's choice)\n"
+                 "      --cuda-slices=y|n, --cuda-thread=y|n, --cuda-unroll=y|n,\n"
                  "      --no-cogrid, --no-nns   accepted for compatibility (no effect on the maps)\n\n",
                  prog);
     std::exit(0);
@@ -118,6 +123,9 @@ Options parse(int argc, char **argv)
         else if (a == "--precision=compensated") o.precision = FG_PREC_COMPENSATED;
         else if (a == "--precision=fp64") o.precision = FG_PREC_FP64;
         else if (a == "--precision=fp32") o.precision = FG_PREC_FP32;
+        else if (a == "--cuda-ddd=g" || a == "--cuda-ddd=c") o.plainDiel = FG_PLAIN_DIEL_TABLE;
+        else if (a == "--cuda-ddd=i") o.plainDiel = FG_PLAIN_DIEL_INPLACE;
+        else if (a == "--cuda-ddd=t") o.plainDiel = FG_PLAIN_DIEL_TEXTURE;
         else if (a.rfind("--cuda-ddd=", 0) == 0 || a.rfind("--cuda-unroll=", 0) == 0 ||
                  a.rfind("--cuda-thread=", 0) == 0 || a.rfind("--cuda-slices=", 0) == 0 || a == "--no-cogrid" ||
                  a == "--no-nns") {
@@ -181,7 +189,8 @@ int main(int argc, char **argv)
     std::fprintf(log, "Beginning grid calculations.\n\nCalculating %d grids over %lld elements, around %d receptor atoms.\n\n",
                  nOut, (long long)P, desc.atoms.n);
     fg_job *job = nullptr;
-    if (fg_job_create(ctx, &desc, &job) != FG_OK) return fatal(0xbad, fg_last_error());
+    if (fg_job_create(ctx, &desc, &job) != FG_OK || fg_job_set_plain_dielectric(job, o.plainDiel) != FG_OK)
+        return fatal(0xbad, fg_last_error());
     const auto t2 = std::chrono::steady_clock::now();
     if (fg_job_run(ctx, job, FG_RUN_ALL | (o.benchmark ? FG_RUN_PHASE_TIMING : 0u), o.precision) != FG_OK ||
         fg_sync(ctx) != FG_OK)

@@ -27,15 +27,43 @@ constexpr int kPlainBX = 16, kPlainBY = 8, kPlainPZ = 8;  // 128 lanes, 8 z-poin
 constexpr int kPlainThreads = kPlainBX * kPlainBY;
 constexpr int kPlainCutThreads = 256;
 
+// Dielectric of the plain e-kernel, the reference's kernel variants
+// (Kernels.inl:73-119): the binned table in global memory (:73-76, also its
+// constant-memory variant), the analytic value in place (:100-104,
+// calculateDistDepDielInv, FG/autogrid.h:106-112) and the table through a
+// normalized, point-sampled, clamped texture (:93-97, :224-228), here a
+// global load at the texel that sampling picks; kPlainConst: constant
+// dielectric (no factor).
+enum { kPlainConst = 0, kPlainTable = 1, kPlainInPlace = 2, kPlainTexture = 3 };
+
 struct PlainElecArgs {
     GridGeom g;
     int32_t a0, a1;        // atom range of this launch
-    int32_t ddd, pad;
+    int32_t pad0, pad1;
     const float4 *atoms;   // (x, y, z, w) in A, FP32
     const float *eps;      // [8192]
     float *acc;            // [P] FP32 sum over launches
 };
 
+template <int kDiel>
+__device__ __forceinline__ float plainDiel(const float *eps, float r2, float ri)
+{
+    const float r = fmaxf(r2 * ri, 0.f);  // r = 0: 0 * inf = NaN -> 0
+    if (kDiel == kPlainTable) return eps[int(fminf(r * 100.f, float(kTableLen - 1)))];
+    if (kDiel == kPlainTexture) {
+        // normalized coordinate u = 100 r / 8191; point sampling reads texel
+        // floor(u * 8192), clamped to the table
+        const float u = (100.f / float(kTableLen - 1)) * r;
+        return eps[int(fminf(u * float(kTableLen), float(kTableLen - 1)))];
+    }
+    if (kDiel == kPlainInPlace) {
+        const float E = expf(-0.3153767175f * r);
+        return (332.f + float(332 * 7.7839) * E) / (78.4f + -66.57180475f * E);
+    }
+    return 1.f;
+}
+
+template <int kDiel>
 __global__ __launch_bounds__(kPlainThreads) void k_plain_elec(PlainElecArgs a)
 {
     const int ix = blockIdx.x * kPlainBX + (threadIdx.x % kPlainBX);
@@ -56,8 +84,7 @@ __global__ __launch_bounds__(kPlainThreads) void k_plain_elec(PlainElecArgs a)
             const float r2 = __builtin_fmaf(dz, dz, dxy);
             const float ri = __builtin_amdgcn_rsqf(r2);
             float d = fminf(ri, 2.f);
-            // bin in [0, 8191]; r = 0 gives 0 * inf = NaN, which maps to bin 0
-            if (a.ddd) d *= a.eps[int(fminf(fmaxf(100.f * r2 * ri, 0.f), float(kTableLen - 1)))];
+            if (kDiel != kPlainConst) d *= plainDiel<kDiel>(a.eps, r2, ri);
             en[k] = __builtin_fmaf(at.w, d, en[k]);
             dz += s;
         }

@@ -59,6 +59,19 @@ enum {
                                 with a plain FP32 sum (no compensation, no FP64 near part) */
 };
 
+/* Dielectric variants of the plain form's e-kernel: the reference's CUDA
+ * kernel variants (--cuda-ddd=, FG/ProgramParameters.cpp:63-112;
+ * FG/electrostatics/cuda_internal/Kernels.inl:73-119). */
+enum {
+    FG_PLAIN_DIEL_TABLE = 0,   /* binned table in global memory (--cuda-ddd=g; also =c) */
+    FG_PLAIN_DIEL_AUTO = 1,    /* the reference's own choice (FG/electrostatics/Cuda.cpp:114-124):
+                                  INPLACE unless an axis of the padded grid exceeds 160 points,
+                                  then TEXTURE */
+    FG_PLAIN_DIEL_INPLACE = 2, /* analytic calculateDistDepDielInv in FP32 (--cuda-ddd=i) */
+    FG_PLAIN_DIEL_TEXTURE = 3  /* table read at the texel of normalized point sampling
+                                  (--cuda-ddd=t) */
+};
+
 /* What a job run computes. */
 #define FG_RUN_ELEC 1u
 #define FG_RUN_AFFINITY 2u /* affinity maps + desolvation map */
@@ -254,6 +267,12 @@ typedef struct {
 } fg_shape;
 int fg_job_get_shape(const fg_job *job, fg_shape *out);
 int fg_job_set_shape(fg_job *job, const fg_shape *shape);
+
+/* Dielectric variant of the job's plain-form e-kernel (default
+ * FG_PLAIN_DIEL_TABLE); fg_job_plain_dielectric returns the variant that
+ * runs (AUTO resolved; TABLE for a constant dielectric). */
+int fg_job_set_plain_dielectric(fg_job *job, int kind);
+int fg_job_plain_dielectric(const fg_job *job);
 
 /* Groups the jobs into size classes (atom count and point count octaves,
  * best-padding brick width), times every shape on the median job of each

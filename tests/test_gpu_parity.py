@@ -284,3 +284,67 @@ def test_sign_symmetry_at_full_size(ctx, tmp_path):
             assert np.array_equal(oa[m], ob[m]), (prec, m)
     ja.close()
     jb.close()
+
+
+def _plain_e_reference(a, kind):
+    """FP64 restatement of the plain e-kernel's sum for one dielectric
+    variant (FG/electrostatics/cuda_internal/Kernels.inl:73-119): the binned
+    table, the texel of normalized point sampling, or the analytic
+    calculateDistDepDielInv (FG/autogrid.h:106-112)."""
+    import fastgrid_amd as fg
+    nx, ny, nz = a.n
+    s = a.spacing
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    gx = ((ix - a.div2[0]) * s).transpose(2, 1, 0).ravel()
+    gy = ((iy - a.div2[1]) * s).transpose(2, 1, 0).ravel()
+    gz = ((iz - a.div2[2]) * s).transpose(2, 1, 0).ravel()
+    e = np.zeros(a.P)
+    eps = np.asarray(a.eps)
+    for i in range(a.n_atoms):
+        r = np.sqrt((gx - a.x[i]) ** 2 + (gy - a.y[i]) ** 2 + (gz - a.z[i]) ** 2)
+        with np.errstate(divide="ignore"):
+            d = np.minimum(1.0 / r, 2.0)
+        if kind == fg.PLAIN_DIEL_TABLE:
+            d = d * eps[np.minimum((100 * r).astype(np.int64), 8191)]
+        elif kind == fg.PLAIN_DIEL_TEXTURE:
+            d = d * eps[np.minimum((100 * r / 8191 * 8192).astype(np.int64), 8191)]
+        else:
+            E = np.exp(-0.3153767175 * r)
+            d = d * (332 + 332 * 7.7839 * E) / (78.4 - 66.57180475 * E)
+        e += a.w[i] * d
+    return e
+
+
+@pytest.mark.parametrize("kind", ["table", "inplace", "texture"])
+def test_plain_dielectric_variants(ctx, tmp_path, kind):
+    """The plain form's three dielectric variants (the reference's
+    --cuda-ddd=g/i/t kernels) against an FP64 restatement of each: FP32
+    arithmetic, so isolated points differ where an FP32 distance lands in the
+    neighbouring table bin; the analytic variant has no bins.  And the
+    reference's automatic choice: in place up to 160 padded points per axis,
+    texture beyond."""
+    import fastgrid_amd as fg
+    run, gpf, _ = fx.synth(str(tmp_path), "micro")
+    a = fx.load_arrays(run, gpf)
+    k = {"table": fg.PLAIN_DIEL_TABLE, "inplace": fg.PLAIN_DIEL_INPLACE, "texture": fg.PLAIN_DIEL_TEXTURE}[kind]
+    job = fg.Job(ctx, a)
+    assert job.plain_dielectric == fg.PLAIN_DIEL_TABLE
+    job.plain_dielectric = fg.PLAIN_DIEL_AUTO
+    assert job.plain_dielectric == fg.PLAIN_DIEL_INPLACE  # 21^3 pads to 32 x 24 x 24
+    job.plain_dielectric = k
+    assert job.plain_dielectric == k
+    job.run(fg.RUN_ELEC, fg.PREC_PLAIN)
+    e = job.fetch()[a.n_maps]
+    ref = _plain_e_reference(a, k)
+    err = np.abs(e - ref) / np.maximum(1.0, np.abs(ref))
+    print(f"plain {kind}: max rel {err.max():.2e}, fraction > 1e-4: {(err > 1e-4).mean():.2e}")
+    assert (err > 1e-4).mean() < (1e-3 if kind != "inplace" else 1e-9)
+    assert err.max() < (0.05 if kind != "inplace" else 1e-4)
+    job.close()
+    import copy
+    wide = copy.copy(a)
+    wide.n, wide.div2 = (161, 5, 3), (80, 2, 1)
+    wj = fg.Job(ctx, wide)
+    wj.plain_dielectric = fg.PLAIN_DIEL_AUTO
+    assert wj.plain_dielectric == fg.PLAIN_DIEL_TEXTURE  # 161 pads to 176
+    wj.close()
