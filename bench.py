@@ -223,6 +223,37 @@ def pocket_specs(n):
     return specs
 
 
+def batch_graph_only(fg, ctx, n_pockets, launches=3):
+    """The 256-pocket batch made, tuned and replayed `launches` times, and
+    nothing else (for rocprofv3 counter runs of the batch kernels)."""
+    with tempfile.TemporaryDirectory() as wd:
+        gpfs = []
+        for i, (seed, atoms, npts) in enumerate(pocket_specs(n_pockets)):
+            radius = (3.0 * atoms / (4.0 * math.pi * BATCH["density"])) ** (1.0 / 3.0)
+            fg.synth_write(wd, f"p{i}", seed, atoms, "sphere", radius, npts, BATCH["ligand_types"])
+            gpfs.append(f"p{i}.gpf")
+        cwd = os.getcwd()
+        os.chdir(wd)
+        try:
+            preps = fg.prepare_many(gpfs, threads=min(16, os.cpu_count() or 1))
+            jobs = fg.create_jobs(ctx, preps, threads=min(16, os.cpu_count() or 1))
+        finally:
+            os.chdir(cwd)
+    fg.autotune(ctx, jobs)
+    batch = fg.Batch(ctx, jobs, fg.RUN_ALL, fg.PREC_COMPENSATED)
+    ms = []
+    for _ in range(launches):
+        ctx.sync()
+        t0 = time.perf_counter()
+        batch.launch()
+        ctx.sync()
+        ms.append(round((time.perf_counter() - t0) * 1e3, 3))
+    print(json.dumps({"batch_ms": ms}), flush=True)
+    batch.close()
+    for j in jobs:
+        j.close()
+
+
 def run_batch(fg, ctx, steps, warmup, n_pockets):
     """The 256-pocket batch: one hipGraph of persistent kernels against one
     plain launch per pocket (and one compensated launch per pocket), plus the
@@ -549,7 +580,7 @@ def run_real(fg, ctx, steps, warmup):
     return res
 
 
-def run_workload(fg, ctx, w, steps, warmup, tune=True, interp=False, cpu=True):
+def run_workload(fg, ctx, w, steps, warmup, tune=True, interp=False, cpu=True, shape=None, forms=None):
     """One large job: FP64 form (error reference), plain, FP32 and
     compensated forms.  Returns a dict of results and the job arrays."""
     with tempfile.TemporaryDirectory() as wd:
@@ -567,7 +598,9 @@ def run_workload(fg, ctx, w, steps, warmup, tune=True, interp=False, cpu=True):
     ref = job.fetch()
     t64 = ctx.timing()
     log(f"fp64 form: elec {t64.elec_ms:.1f} ms")
-    report = fg.autotune(ctx, [job]) if tune else ""
+    if shape:
+        job.shape = shape
+    report = fg.autotune(ctx, [job]) if tune and not shape else ""
     shape = job.shape
     log("tuned shape (pz, tile, bx, px):", shape)
     for line in report.splitlines()[1:]:
@@ -582,6 +615,8 @@ def run_workload(fg, ctx, w, steps, warmup, tune=True, interp=False, cpu=True):
     res["plain_auto_variant"] = PLAIN_VARIANTS[job.plain_dielectric]
     for key, prec, diel in (("plain", fg.PREC_PLAIN, fg.PLAIN_DIEL_TABLE), ("plain_auto", fg.PREC_PLAIN, fg.PLAIN_DIEL_AUTO),
                             ("fp32", fg.PREC_FP32, None), ("compensated", fg.PREC_COMPENSATED, None)):
+        if forms and key not in forms:
+            continue
         if diel is not None:
             job.plain_dielectric = diel
         t, el, af = time_steps(fg, ctx, job, steps, warmup, prec)
@@ -611,8 +646,13 @@ def main():
                          "(pz,tile,bx,px), or 'all' for the autotuner's 42 shapes (for rocprofv3 counter runs)")
     ap.add_argument("--elec-workload", default="L", choices=["L", "X"],
                     help="workload of --elec-shapes: L (50k atoms, 127^3) or X (the 250k-atom / 161^3 stress size)")
+    ap.add_argument("--batch-graph-only", action="store_true",
+                    help="only make, tune and replay the 256-pocket batch three times (for rocprofv3 counter runs)")
     ap.add_argument("--batch-only", action="store_true",
                     help="only run the 256-pocket batch and print its results (for rocprofv3 counter runs)")
+    ap.add_argument("--shape", default="",
+                    help="electrostatic kernel shape 'pz,tile,bx,px' for L instead of the autotuner's pick, with only "
+                         "the compensated form timed and nothing else run (for rocprofv3 counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (reference dispatch for HBM byte counters)")
     args = ap.parse_args()
@@ -632,8 +672,18 @@ def main():
 
     import fastgrid_amd as fg
     ctx = fg.Context(0)
+    if args.batch_graph_only:
+        batch_graph_only(fg, ctx, args.pockets)
+        return
     if args.batch_only:
         print(json.dumps({"batch": run_batch(fg, ctx, max(1, args.steps // 2), 1, args.pockets)}), flush=True)
+        return
+    if args.shape:
+        sh = tuple(int(v) for v in args.shape.split(","))
+        L, _ = run_workload(fg, ctx, WORKLOAD_L, args.steps, args.warmup, shape=sh, forms=("compensated",))
+        c = L["compensated"]
+        print(json.dumps({"workload": "L", "shape": sh, "ms_per_step": round(c["t"] * 1e3, 3),
+                          "elec_ms": round(c["elec_ms"], 3), "cutoff_ms": round(c["cutoff_ms"], 3)}), flush=True)
         return
     L, a = run_workload(fg, ctx, WORKLOAD_L, args.steps, args.warmup, interp=not args.no_interp, cpu=not args.no_cpu)
     comp, plain, plain_auto, fp32 = L["compensated"], L["plain"], L["plain_auto"], L["fp32"]

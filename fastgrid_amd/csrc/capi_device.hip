@@ -309,7 +309,6 @@ fgk::CutoffArgs cutoffArgs(const fg_job *j, int doElecNear)
     a.vs = j->dVS;
     a.brickStart = j->dStart;
     a.brickList = j->typed ? j->dSorted : j->dList;
-    a.typed = j->typed ? 1 : 0;
     a.hStart = j->dHStart;
     a.hList = j->dHList;
     a.typeStart = j->dTypeStart;
@@ -1165,7 +1164,7 @@ struct fg_batch {
     std::vector<fg_job *> jobs;  // largest first
     bool aff = false, near = false, elec = false, comp = true, fmap = false;
     int MM = 4, KH = 0, px = 1, pz = 4;
-    bool pointAng = false, clamp = false, typed = false;  // typed: some job has type-sorted lists
+    bool pointAng = false, clamp = false, typed = true;  // typed: every job has type-sorted lists
     int sharedTableLen = 0, maxTableLen = 0;
     int cutItems = 0, elecItems = 0, elecWGs = 0, farItems = 0;
     fgk::ListArgs *dList = nullptr;
@@ -1349,7 +1348,10 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
         b->pointAng = b->pointAng || j->pointAng;
         b->clamp = b->clamp || j->clamp;
         b->maxTableLen = std::max(b->maxTableLen, j->tableLen);
-        b->typed = b->typed || j->typed;
+        // a job without type-sorted lists (more than kMaxTypes receptor
+        // types) makes the batch visit every job's candidates in list order:
+        // the same maps up to the FP64 summation order of the job's own run
+        b->typed = b->typed && j->typed;
     }
     {
         int v = 0;
@@ -1378,12 +1380,7 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
     for (fg_job *j : b->jobs) {
         la.push_back(listArgs(j));
         ca.push_back(cutoffArgs(j, b->near ? 1 : 0));
-        // the typed kernel takes a job without type-sorted lists in list
-        // order, as that job's own (untyped) kernel does: same bits
-        if (!(b->typed && b->aff && j->typed)) {
-            ca.back().brickList = j->dList;
-            ca.back().typed = 0;
-        }
+        if (!(b->typed && b->aff)) ca.back().brickList = j->dList;
         // two points per lane along x need a 2D brick (the shape does not
         // change the results)
         const int shift = b->px == 2 && j->bxShift == 0 ? 4 : j->bxShift;
@@ -1508,6 +1505,11 @@ int jobBuildHost(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
     j->N = d->atoms.n;
     j->M = d->n_maps;
     j->T = d->tables.n_types;
+    // the kernels index the pair table with 32-bit offsets
+    if (j->T < 0 || int64_t(j->T) * fgk::kTableLen * std::max(j->M, 1) >= (int64_t(1) << 31)) {
+        delete j;
+        return fgc::fail(FG_E_ARG, "fg_job_create: pair table too large (n_types x 8192 x n_maps >= 2^31)");
+    }
     j->distDepDiel = d->dist_dep_diel;
     j->hydrogen = d->hydrogen;
     j->covOn = d->covalent ? 0 : d->cov_on;

@@ -478,14 +478,11 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
     double dmap = 0, enear = 0, sv = 0, sq = 0;
 
     // candidates: one run over the brick's list, or one run per receptor
-    // type over the type-sorted list with that type's rows in LDS (a batch's
-    // typed kernel also serves jobs without sorted lists: one run, rows read
-    // from the table, the same values)
-    const bool typedRun = kTyped && a.typed;
-    const int nRuns = typedRun ? a.T : 1;
+    // type over the type-sorted list with that type's rows in LDS
+    const int nRuns = kTyped ? a.T : 1;
     for (int run = 0; run < nRuns; ++run) {
         int c0, c1;
-        if (typedRun) {
+        if (kTyped) {
             c0 = constView(a.typeStart)[b * (a.T + 1) + run];
             c1 = constView(a.typeStart)[b * (a.T + 1) + run + 1];
             if (c0 == c1) continue;
@@ -555,14 +552,15 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                     Hramp = t.Hramp;
                 }
                 const double S = a.desolv[idx];
-                const double *grow = a.pair + (int64_t(type) * kTableLen + idx) * M;
-                const double *lrow = typedRun ? &L.row[idx * ldsMaps<MM>()] : grow;
+                // 32-bit offsets (job creation bounds the table below 2^31 entries)
+                const double *grow = a.pair + unsigned((type * kTableLen + idx) * M);
+                const double *lrow = kTyped ? &L.row[idx * ldsMaps<MM>()] : grow;
                 // map m's entry: LDS for the first MM/2 maps of the typed kernel
 #define FG_ROW(m) ((kTyped && (m) < ldsMaps<MM>()) ? lrow[(m)] : grow[(m)])
                 const double *row = grow;
                 const double *rowH =
                     (dis && a.hydrogen >= 0)
-                        ? a.pair + (int64_t(a.hydrogen) * kTableLen + (idx - 110 > 0 ? idx - 110 : 0)) * M
+                        ? a.pair + unsigned((a.hydrogen * kTableLen + (idx - 110 > 0 ? idx - 110 : 0)) * M)
                         : row;
                 const double vol = sVS[2 * c], solQ = sVS[2 * c + 1];
                 const int shift = 2 * ahb;

@@ -91,8 +91,7 @@ struct CellGrid {
 struct CutoffArgs {
     GridGeom g;
     int32_t nbx, nby, nbz;  // bricks per axis
-    int32_t M, hydrogen, doElecNear;
-    int32_t typed;                            // brickList is type-sorted with runs in typeStart
+    int32_t M, hydrogen, doElecNear, pad;
     int64_t P;
     const CutAtom *atoms;
     const double *vs;                         // [N][2] vol and solpar + solparQ |q| per atom
