@@ -142,7 +142,7 @@ struct fg_job {
     hipGraphExec_t runExec = nullptr;
     unsigned runWhat = 0;
     int runKey[5] = {-1, 0, 0, 0, 0};  // precision, px, pz, tile, bxShift
-    double *dPair = nullptr, *dDesolv = nullptr, *dEps = nullptr, *dCovalent = nullptr;
+    double *dPair = nullptr, *dRsph = nullptr, *dDesolv = nullptr, *dEps = nullptr, *dCovalent = nullptr;
     double *dXYZ = nullptr;
     int32_t *dHbond = nullptr, *dCellStart = nullptr, *dCellAtoms = nullptr;
     fgk::CellGrid cg{};
@@ -315,6 +315,7 @@ fgk::CutoffArgs cutoffArgs(const fg_job *j, int doElecNear)
     a.T = j->T;
     a.addOnlyClasses = j->addOnlyClasses;
     a.pair = j->dPair;
+    a.rsph = j->dRsph;
     a.desolv = j->dDesolv;
     a.eps = j->dEps;
     a.maps = j->dMaps;
@@ -733,19 +734,28 @@ int buildCutoff(fg_job *j, const fg_job_desc &d)
         if (int rc = devUpload(j, j->dType, ty.data(), ty.size())) return rc;
     }
     // tables
-    const size_t pairN = size_t(std::max(d.tables.n_types, 1)) * fgk::kTableLen * size_t(std::max(j->M, 1));
-    std::vector<double> zeros;
+    // bins 0..800 of every row, maps in kernel order, and per entry the
+    // saturated E / 100 of the H-bond terms (FG/CalculateGridmaps.cpp:210),
+    // formed with the device's operations
+    const int Mx = std::max(j->M, 1);
+    const size_t pairN = size_t(std::max(d.tables.n_types, 1)) * fgk::kPairBins * size_t(Mx);
+    std::vector<double> perm(pairN, 0.0), rsph(pairN, 0.0);
     if (j->hasGeom && d.tables.pair) {
-        std::vector<double> perm(pairN);
-        const size_t rows = pairN / size_t(std::max(j->M, 1));
-        for (size_t r = 0; r < rows; ++r)
-            for (int k = 0; k < j->M; ++k)
-                perm[r * size_t(j->M) + size_t(k)] = d.tables.pair[r * size_t(j->M) + size_t(j->order[size_t(k)])];
-        if (int rc = devUpload(j, j->dPair, perm.data(), perm.size())) return rc;
-    } else {
-        zeros.assign(pairN, 0.0);
-        if (int rc = devUpload(j, j->dPair, zeros.data(), zeros.size())) return rc;
+        for (int t = 0; t < d.tables.n_types; ++t)
+            for (int b = 0; b < fgk::kPairBins; ++b)
+                for (int k = 0; k < j->M; ++k) {
+                    const double v = d.tables.pair[(size_t(t) * fgk::kTableLen + size_t(b)) * size_t(j->M) +
+                                                   size_t(j->order[size_t(k)])];
+                    double q = v / 100;
+                    q = q > 0 ? q : 0;
+                    q = q < 1 ? q : 1;
+                    const size_t o = (size_t(t) * fgk::kPairBins + size_t(b)) * size_t(j->M) + size_t(k);
+                    perm[o] = v;
+                    rsph[o] = q;
+                }
     }
+    if (int rc = devUpload(j, j->dPair, perm.data(), perm.size())) return rc;
+    if (int rc = devUpload(j, j->dRsph, rsph.data(), rsph.size())) return rc;
     std::vector<double> zt(fgk::kTableLen, 0.0);
     if (int rc = devUpload(j, j->dDesolv, (j->hasGeom && d.tables.desolv) ? d.tables.desolv : zt.data(), zt.size()))
         return rc;
@@ -806,10 +816,13 @@ int buildPlain(fg_job *j, const fg_job_desc &d)
     }
     if (int rc = devUpload(j, j->dPlainQ, q.data(), q.size())) return rc;
     if (int rc = devUpload(j, j->dPlainAtoms, pa.data(), pa.size())) return rc;
-    const size_t pairN = size_t(std::max(d.tables.n_types, 1)) * fgk::kTableLen * size_t(std::max(j->M, 1));
+    const size_t pairN = size_t(std::max(d.tables.n_types, 1)) * fgk::kPairBins * size_t(std::max(j->M, 1));
     std::vector<float> pair(pairN, 0.f), desolv(fgk::kTableLen, 0.f);
     if (j->hasGeom) {
-        for (size_t k = 0; k < pairN && d.tables.pair; ++k) pair[k] = float(d.tables.pair[k]);
+        for (int t = 0; t < d.tables.n_types && d.tables.pair; ++t)
+            for (size_t k = 0; k < size_t(fgk::kPairBins) * size_t(j->M); ++k)
+                pair[size_t(t) * fgk::kPairBins * size_t(j->M) + k] =
+                    float(d.tables.pair[size_t(t) * fgk::kTableLen * size_t(j->M) + k]);
         for (int k = 0; k < fgk::kTableLen; ++k) desolv[size_t(k)] = float(d.tables.desolv[k]);
         for (int m = 0; m < j->M; ++m) {
             const fg_map_params &p = d.maps[m];
@@ -1506,9 +1519,9 @@ int jobBuildHost(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
     j->M = d->n_maps;
     j->T = d->tables.n_types;
     // the kernels index the pair table with 32-bit offsets
-    if (j->T < 0 || int64_t(j->T) * fgk::kTableLen * std::max(j->M, 1) >= (int64_t(1) << 31)) {
+    if (j->T < 0 || int64_t(j->T) * fgk::kPairBins * std::max(j->M, 1) >= (int64_t(1) << 31)) {
         delete j;
-        return fgc::fail(FG_E_ARG, "fg_job_create: pair table too large (n_types x 8192 x n_maps >= 2^31)");
+        return fgc::fail(FG_E_ARG, "fg_job_create: pair table too large (n_types x 801 x n_maps >= 2^31)");
     }
     j->distDepDiel = d->dist_dep_diel;
     j->hydrogen = d->hydrogen;

@@ -336,29 +336,29 @@ __device__ __noinline__ AngTerms pointAngular(const CutAtom &A, bool isClosest, 
     return t;
 }
 
-// FP64 1/sqrt(x) and x*that from the hardware estimate plus two Newton steps:
-// a few ulp from the correctly rounded values, which is all the continuous
-// uses need; table-bin decisions made from it are guarded (see binGuarded).
+// FP64 1/sqrt(x) for the cutoff kernel's pairs (1e-24 <= x <= 64): the
+// hardware estimate plus one Newton step.  v_rsq_f64 measured at most
+// 2^-24.2 relative on 4 M inputs in [1e-6, 64] on MI355X; from any estimate
+// within 2^-22 one step leaves at most 1.5 e^2 plus rounding, below 4e-13,
+// which is all the continuous uses need; table-bin decisions made from it
+// are guarded (see binGuarded).
 __device__ __forceinline__ double rsqrtFast(double x)
 {
-    double y = __builtin_amdgcn_rsq(x);
-    const double h = 0.5 * x;
-    y = y * __builtin_fma(-h * y, y, 1.5);
-    y = y * __builtin_fma(-h * y, y, 1.5);
-    return y;
+    const double y = __builtin_amdgcn_rsq(x);
+    return y * __builtin_fma(-0.5 * x * y, y, 1.5);
 }
 
-// floor(100 r) from an approximation r_a within ~1e-15 relative of the exact
-// r: equal to the reference's angstromToIndex unless 100 r_a lies within 1e-7
-// of an integer, in which case `edge` is set and the caller recomputes r in
-// the reference's exact operation order.
+// floor(100 r) for 0 <= r <= 8 (bins 0..800: no clamp) from an approximation
+// r_a within 1e-12 relative of the exact r: equal to the reference's
+// angstromToIndex unless 100 r_a lies within 1e-7 of an integer, in which
+// case `edge` is set and the caller recomputes r in the reference's exact
+// operation order.
 __device__ __forceinline__ int binGuarded(double r, bool &edge)
 {
     const double t = r * 100.0;
     const int k = int(t);
-    const double f = t - double(k);
-    edge = f < 1e-7 || f > 1.0 - 1e-7;
-    return k < kTableLen - 1 ? k : kTableLen - 1;
+    edge = __builtin_fabs((t - double(k)) - 0.5) > 0.5 - 1e-7;
+    return k;
 }
 
 // Initial value of a covalent map at (gx, gy, gz) (FG/main.cpp:39-46).
@@ -371,7 +371,7 @@ __device__ __forceinline__ double covalentValue(const CutoffArgs &a, double gx, 
 }
 
 constexpr int kChunk = 32;     // candidate atoms staged in LDS per round
-constexpr int kRowBins = 801;  // table bins of pairs with r <= 8 A: floor(100 r) <= 800
+constexpr int kRowBins = kPairBins;  // table bins of pairs with r <= 8 A: floor(100 r) <= 800
 constexpr int kMaxTypes = 32;  // receptor types the type-sorted lists support
 
 // LDS of one cutoff workgroup.  kTyped: the brick's candidates are sorted by
@@ -487,7 +487,7 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
             c1 = constView(a.typeStart)[b * (a.T + 1) + run + 1];
             if (c0 == c1) continue;
             __syncthreads();  // every lane is done with the previous type's rows
-            const double *src = a.pair + int64_t(run) * kTableLen * M;
+            const double *src = a.pair + unsigned(run * kPairBins * M);
             constexpr int ML = ldsMaps<MM>();
             for (int q = tid; q < kRowBins * ML; q += kCutThreads) {
                 const int bin = q / ML, m = q - bin * ML;
@@ -516,7 +516,7 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                 const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
                 double rSq = dx2 + dy2 + dz2;
                 if (!valid || rSq > 64.0) continue;
-                if (rSq < 1.0E-24) rSq = 1.0E-24;
+                rSq = __builtin_fmax(rSq, 1.0E-24);  // rSq >= 0, never NaN
                 // guarded fast geometry (see binGuarded)
                 const double yinv = rsqrtFast(rSq);
                 bool edge;
@@ -531,7 +531,7 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                     idxE = binOf(sqrt(dx2 + (dy2 + dz2)));
                 }
                 // r <= 8 A: both bins are <= 800
-                if (a.doElecNear) enear += A.w * dmin_(yinv, 2) * (kTyped ? L.eps[idxE] : a.eps[idxE]);
+                if (a.doElecNear) enear += A.w * __builtin_fmin(yinv, 2.0) * (kTyped ? L.eps[idxE] : a.eps[idxE]);
                 if (!kAffinity) continue;
                 // atom fields are the same in every lane: keep them in scalar registers
                 const int type = uniform(A.type), ahb = uniform(A.hbond), fl = uniform(A.flags);
@@ -553,14 +553,15 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                 }
                 const double S = a.desolv[idx];
                 // 32-bit offsets (job creation bounds the table below 2^31 entries)
-                const double *grow = a.pair + unsigned((type * kTableLen + idx) * M);
+                const unsigned rowOff = unsigned((type * kPairBins + idx) * M);
+                const double *grow = a.pair + rowOff;
                 const double *lrow = kTyped ? &L.row[idx * ldsMaps<MM>()] : grow;
                 // map m's entry: LDS for the first MM/2 maps of the typed kernel
 #define FG_ROW(m) ((kTyped && (m) < ldsMaps<MM>()) ? lrow[(m)] : grow[(m)])
                 const double *row = grow;
                 const double *rowH =
                     (dis && a.hydrogen >= 0)
-                        ? a.pair + unsigned((a.hydrogen * kTableLen + (idx - 110 > 0 ? idx - 110 : 0)) * M)
+                        ? a.pair + unsigned((a.hydrogen * kPairBins + (idx - 110 > 0 ? idx - 110 : 0)) * M)
                         : row;
                 const double vol = sVS[2 * c], solQ = sVS[2 * c + 1];
                 const int shift = 2 * ahb;
@@ -586,7 +587,8 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                     // keeps a min/max adds nothing here: e + 0 == e)
                     double add = E;
                     if (act != kActAdd) {
-                        const double rsph = dmin_(dmax_(E / 100, 0), 1);
+                        // min(max(E / 100, 0), 1), formed on the host per table entry
+                        const double rsph = a.rsph[rowOff + m];
                         if (act == kActDonor) {
                             add = rowH[m] * Hramp * (racc + (1 - racc) * rsph);
                         } else {

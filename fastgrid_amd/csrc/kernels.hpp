@@ -31,6 +31,9 @@
 namespace fgk {
 
 constexpr int kTableLen = 8192;
+// The device copies of the pair-energy table (and of its derived tables) hold
+// bins 0..800 only: every pair the cutoff kernels look up has r <= 8 A.
+constexpr int kPairBins = 801;
 constexpr int kSubTile = 32;       // atoms per culling box of the electrostatic tiles
 constexpr int kCutBX = 8, kCutBY = 8, kCutBZ = 4;  // k_cutoff brick (points)
 constexpr int kCutThreads = kCutBX * kCutBY * kCutBZ;
@@ -101,7 +104,8 @@ struct CutoffArgs {
     int32_t T;                                // receptor types (rows of the pair table)
     int32_t addOnlyClasses;                   // bit h: atoms of H-bond class h add E to every map
                                               // and no map is covalent
-    const double *pair;    // [T][8192][M], maps in kernel order
+    const double *pair;    // [T][kPairBins][M], maps in kernel order
+    const double *rsph;    // [T][kPairBins][M]: min(max(pair / 100, 0), 1) (FG/CalculateGridmaps.cpp:210)
     const double *desolv;  // [8192]
     const double *eps;     // [8192]
     double *maps;          // [M+2][P]: affinity maps, e (near part), d

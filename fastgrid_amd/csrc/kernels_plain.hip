@@ -131,7 +131,7 @@ struct PlainCutArgs {
     const PlainAtom *atoms;
     const int32_t *brickStart, *brickList;  // the candidate lists of k_cutoff serve as cells
     const int32_t *hStart, *hList;
-    const float *pair;     // [T][8192][M], output map order
+    const float *pair;     // [T][kPairBins][M], output map order
     const float *desolv;   // [8192]
     double *maps;          // [M+2][P]
     const double *covalent;
@@ -193,13 +193,13 @@ __global__ __launch_bounds__(kPlainCutThreads) void k_plain_cutoff(PlainCutArgs 
 #pragma unroll
         for (int m = 0; m < MM; ++m) {
             if (m >= M || a.mp[m].isCovalent) continue;
-            const float E = a.pair[(int64_t(A.type) * kTableLen + idx) * M + m];
+            const float E = a.pair[(int64_t(A.type) * kPairBins + min(idx, kPairBins - 1)) * M + m];
             const int mh = a.mp[m].hbond;
             if (a.mp[m].isHBonder) {
                 const float rsph = fminf(fmaxf(E / 100.f, 0.f), 1.f);
                 if ((mh == HB_AS || mh == HB_A2) && aDon) {
                     const float Eh = (A.disorder && a.hydrogen >= 0)
-                                         ? a.pair[(int64_t(a.hydrogen) * kTableLen + max(0, idx - 110)) * M + m]
+                                         ? a.pair[(int64_t(a.hydrogen) * kPairBins + max(0, min(idx, kPairBins - 1) - 110)) * M + m]
                                          : E;
                     e[m] += Eh * Hramp * (racc + (1.f - racc) * rsph);
                 } else if (mh == HB_A1 && aDon) {
