@@ -65,6 +65,12 @@ class FgTiming(C.Structure):
     _fields_ = [("elec_ms", C.c_double), ("affinity_ms", C.c_double), ("total_ms", C.c_double)]
 
 
+class FgSummary(C.Structure):
+    _fields_ = [("n_maps", C.c_int32), ("min", C.c_double * 18), ("max", C.c_double * 18),
+                ("has_charge", C.c_int32), ("charge_integral", C.c_int32), ("total_charge", C.c_double),
+                ("charge_residual", C.c_double)]
+
+
 class FgShape(C.Structure):
     _fields_ = [("pz", C.c_int32), ("tile", C.c_int32), ("bx", C.c_int32), ("px", C.c_int32)]
 
@@ -120,6 +126,7 @@ SIGNATURES = {
     "fg_sync": (C.c_int, [C.c_void_p]),
     "fg_job_fetch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_dp)]),
     "fg_job_cutoff_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]),
+    "fg_job_summary": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FgSummary)]),
     "fg_job_get_shape": (C.c_int, [C.c_void_p, C.POINTER(FgShape)]),
     "fg_job_set_shape": (C.c_int, [C.c_void_p, C.POINTER(FgShape)]),
     "fg_job_set_plain_dielectric": (C.c_int, [C.c_void_p, C.c_int]),
@@ -585,6 +592,17 @@ class Job:
         ptrs = (_dp * n)(*[_ptr(o, C.c_double) for o in outs])
         _check(LIB.fg_job_fetch(self.ctx.handle, self._h, ptrs))
         return outs
+
+    def summary(self) -> dict:
+        """GPU checks of the last run (fg_job_summary): per map the min/max of
+        the rounded values, the receptor's total charge and its distance to
+        the nearest integer."""
+        sm = FgSummary()
+        _check(LIB.fg_job_summary(self.ctx.handle, self._h, C.byref(sm)))
+        n = sm.n_maps
+        return {"min": [sm.min[i] for i in range(n)], "max": [sm.max[i] for i in range(n)],
+                "has_charge": bool(sm.has_charge), "total_charge": sm.total_charge,
+                "charge_residual": sm.charge_residual, "charge_integral": bool(sm.charge_integral)}
 
     def cutoff_pairs(self) -> int:
         v = C.c_int64()

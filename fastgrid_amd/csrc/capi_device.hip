@@ -135,6 +135,8 @@ struct fg_job {
     CutAtom *dAtoms = nullptr;
     double *dVS = nullptr;
     double *dFar = nullptr;  // [P] far-field e of the compensated form
+    double *dCharge = nullptr;   // [N] partial charges (when given), for the summary
+    double *dSummary = nullptr;  // [M + 2][2] min/max of the rounded maps, then the total charge
     int32_t *dCounter = nullptr;  // work counter of the persistent e-kernel
     int elecPerCU[2][2][2] = {};  // e-kernel workgroups per CU by [px - 1][pz == 8][compensated]
     // the graph of the last compensated or FP32 run and what it was made for
@@ -1086,6 +1088,15 @@ int tuneJobs(fg_ctx *ctx, fg_job *const *jobs, int n, bool comp)
                   kVar[vi][0], kVar[vi][1], total[0], total[1], total[2], total[3],
                   n > 1 ? "; times x device fill (throughput of a batch)" : "");
     report += line;
+    {
+        const fg_job *j0 = jobs[0];
+        std::snprintf(line, sizeof line,
+                      "e-kernel workgroups per CU (compensated; first job's table of %d entries): px1 pz4 %d, px1 pz8 %d, "
+                      "px2 pz4 %d, px2 pz8 %d\n",
+                      j0->tableLen, j0->elecPerCU[0][0][1], j0->elecPerCU[0][1][1], j0->elecPerCU[1][0][1],
+                      j0->elecPerCU[1][1][1]);
+        report += line;
+    }
     for (Cls &c : cls) {
         int bb = 0, bd = 0;
         for (int b = 0; b < 3; ++b)
@@ -1606,6 +1617,8 @@ int jobBuildHost(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
     if (!rc) rc = devAlloc(j, j->dMaps, size_t(j->M + 2) * size_t(j->P));
     if (!rc) rc = devAlloc(j, j->dFar, size_t(j->P));
     if (!rc) rc = devAlloc(j, j->dCounter, size_t(1));
+    if (!rc) rc = devAlloc(j, j->dSummary, size_t(2) * size_t(j->M + 2) + 1);
+    if (!rc && A.charge && j->N > 0) rc = devUploadDirect(j, j->dCharge, A.charge, size_t(j->N));
     if (!rc && j->floating) rc = devAlloc(j, j->dFmap, size_t(j->P));
     if (rc) {
         destroyJob(j);
@@ -1892,6 +1905,32 @@ int fg_job_cutoff_pairs(fg_ctx *ctx, fg_job *j, int64_t *pairs)
     (void)hipFree(d);
     if (e != hipSuccess) return fgc::fail(FG_E_DEVICE, std::string("fg_job_cutoff_pairs: ") + hipGetErrorString(e));
     *pairs = int64_t(h);
+    return FG_OK;
+}
+
+int fg_job_summary(fg_ctx *ctx, fg_job *j, fg_summary *out)
+{
+    if (!ctx || !j || !out) return fgc::fail(FG_E_ARG, "fg_job_summary: NULL argument");
+    if (j->ctx != ctx) return fgc::fail(FG_E_ARG, "fg_job_summary: job does not belong to this context");
+    FG_HIP(hipSetDevice(ctx->device));
+    const int nm = j->M + 2;
+    hipLaunchKernelGGL(fgk::k_job_summary, dim3(unsigned(nm + 1)), dim3(256), 0, ctx->stream, j->dMaps, j->P, nm,
+                       j->dCharge, j->dCharge ? j->N : 0, j->dSummary);
+    FG_HIP(hipGetLastError());
+    std::vector<double> h(size_t(2 * nm + 1));
+    FG_HIP(hipMemcpyAsync(h.data(), j->dSummary, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FG_HIP(hipStreamSynchronize(ctx->stream));
+    *out = fg_summary{};
+    out->n_maps = nm;
+    for (int m = 0; m < nm; ++m) {
+        out->min[m] = h[size_t(2 * m)];
+        out->max[m] = h[size_t(2 * m + 1)];
+    }
+    out->has_charge = j->dCharge ? 1 : 0;
+    const double q = j->dCharge ? h[size_t(2 * nm)] : 0.0;
+    out->total_charge = q;
+    out->charge_residual = std::fabs(q - std::nearbyint(q));
+    out->charge_integral = j->dCharge && out->charge_residual < 0.005 ? 1 : 0;
     return FG_OK;
 }
 

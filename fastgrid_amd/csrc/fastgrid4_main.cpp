@@ -197,6 +197,9 @@ int main(int argc, char **argv)
         return fatal(0xbad, fg_last_error());
     fg_timing tm{};
     fg_last_timing(ctx, &tm);
+    // the log's map extremes and total charge, computed on the GPU
+    fg_summary sm{};
+    if (fg_job_summary(ctx, job, &sm) != FG_OK) return fatal(0xbad, fg_last_error());
     std::vector<std::vector<double>> host(static_cast<size_t>(nOut), std::vector<double>(static_cast<size_t>(P)));
     std::vector<double *> ptrs(static_cast<size_t>(nOut));
     for (size_t m = 0; m < host.size(); ++m) ptrs[m] = host[m].data();
@@ -210,14 +213,15 @@ int main(int argc, char **argv)
     // summary (FG/GridMap.cpp:71-101)
     std::fprintf(log, "\nGrid\tAtom\tMinimum   \tMaximum\nMap \tType\tEnergy    \tEnergy \n\t\t(kcal/mol)\t(kcal/mol)\n"
                       "____\t____\t_____________\t_____________\n");
-    double mn[18], mx[18];
-    for (int m = 0; m < M + 2; ++m) {
-        fg_prep_summary(prep, m, &mn[m], &mx[m]);
+    const double *mn = sm.min, *mx = sm.max;
+    for (int m = 0; m < M + 2; ++m)
         std::fprintf(log, " %d\t %s\t  %6.2lf\t%9.2le\n", m + 1, fg_prep_map_type(prep, m), mn[m], mx[m]);
-    }
     std::fprintf(log, " %d\t %c\t  %6.2lf\t%9.2le\tElectrostatic Potential\n %d\t %c\t  %6.2lf\t%9.2le\tDesolvation Potential\n"
                       "\n\n * Note:  Every pairwise-atomic interaction was clamped at %.2f\n\n\n",
                  M + 1, 'e', mn[M], mx[M], M + 2, 'd', mn[M + 1], mx[M + 1], 100000.0);
+    if (sm.has_charge)
+        std::fprintf(log, "Total charge :\t\t\t%.2lf e%s\n\n", sm.total_charge,
+                     sm.charge_integral ? "" : "  (not an integral net charge)");
     std::fprintf(stderr, "\n%s: Successful Completion.\n", prog);
     std::fprintf(log, "%s: Successful Completion.\n", prog);
     std::fprintf(log, "Real= %.2fs\n", seconds(t0, t4));

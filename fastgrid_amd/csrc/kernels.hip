@@ -1066,6 +1066,57 @@ __global__ __launch_bounds__(256) void k_add_far(int64_t P, const double *far, d
     if (p < P) e[p] = e[p] + far[p];
 }
 
+// roundOutput (FG/autogrid.h:128-131) in FP64, as the writers apply it.
+__device__ __forceinline__ double round3(double a)
+{
+    return __builtin_fabs(a) < 0.0005 ? 0.0 : floor(a * 1000 + 0.5) * 0.001;
+}
+
+// The job's summary: workgroup m < nMaps the minimum and maximum of map m's
+// rounded values (GridMapList::calculateEnergyMinMax over the stored, rounded
+// energies, FG/GridMap.cpp:214-224, starting from +-BIG = 1e12), the last
+// workgroup the receptor's total charge summed in atom order by one lane, the
+// reference's own loop (FG/InputDataLoader.cpp:325).  Min and max are exact,
+// so the reduction order does not matter.  out: [nMaps][2], then the charge.
+__global__ __launch_bounds__(256) void k_job_summary(const double *maps, int64_t P, int nMaps, const double *charge,
+                                                      int N, double *out)
+{
+    __shared__ double sLo[4], sHi[4];
+    const int m = int(blockIdx.x);
+    if (m == nMaps) {
+        if (threadIdx.x == 0) {
+            double q = 0;
+            for (int i = 0; i < N; ++i) q += charge[i];
+            out[2 * nMaps] = q;
+        }
+        return;
+    }
+    double lo = 1.0e12, hi = -1.0e12;
+    const double *v = maps + int64_t(m) * P;
+    for (int64_t p = threadIdx.x; p < P; p += 256) {
+        const double r = round3(v[p]);
+        lo = __builtin_fmin(lo, r);
+        hi = __builtin_fmax(hi, r);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = __builtin_fmin(lo, __shfl_xor(lo, o));
+        hi = __builtin_fmax(hi, __shfl_xor(hi, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sLo[threadIdx.x >> 6] = lo;
+        sHi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            lo = __builtin_fmin(lo, sLo[w]);
+            hi = __builtin_fmax(hi, sHi[w]);
+        }
+        out[2 * m] = lo;
+        out[2 * m + 1] = hi;
+    }
+}
+
 // ------------------------------------------------------------------------
 // Batches: many jobs in one launch.  Work items are (job, brick) pairs,
 // numbered job after job; base[j] is the first item of job j and

@@ -251,6 +251,26 @@ int fg_job_fetch(fg_ctx *ctx, fg_job *job, double *const *out);
 /* Number of (point, atom) pairs with r <= 8 A (computed on the device). */
 int fg_job_cutoff_pairs(fg_ctx *ctx, fg_job *job, int64_t *pairs);
 
+/* Checks of a job's last run, computed on the GPU in FP64 from the maps in
+ * HBM and the receptor's charges.  min/max: per map (affinity maps in the
+ * job's order, then e, then d) the extremes of the rounded values
+ * (roundOutput, FG/autogrid.h:128-131) that the writers print and that the
+ * reference's log summary reports (GridMapList::logSummary /
+ * calculateEnergyMinMax, FG/GridMap.cpp:71-101, 214-224).  total_charge: the
+ * receptor's partial charges summed in atom order, the reference's
+ * "Total charge" (FG/InputDataLoader.cpp:325, :380); charge_residual its
+ * distance to the nearest integer and charge_integral whether that is below
+ * 0.005 e (a receptor prepared with whole-residue charges has an integral
+ * net charge; the synthetic receptors are neutral).  has_charge is 0 for
+ * jobs made without charges. */
+typedef struct {
+    int32_t n_maps;                  /* affinity maps + 2 */
+    double min[FG_MAX_MAPS + 2], max[FG_MAX_MAPS + 2];
+    int32_t has_charge, charge_integral;
+    double total_charge, charge_residual;
+} fg_summary;
+int fg_job_summary(fg_ctx *ctx, fg_job *job, fg_summary *out);
+
 /* ---------------------------------------------------------------- */
 /* Electrostatic kernel shapes and the autotuner (no reference      */
 /* counterpart: the reference picks its CUDA launch shape by fixed  */
