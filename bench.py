@@ -284,18 +284,22 @@ def run_batch(fg, ctx, steps, warmup, n_pockets):
             first.launch()
             ctx.sync()
             t4 = time.perf_counter()
-            for j in jobs:
-                j.fetch()
+            # the maps as the files print them: thousandths rounded on the GPU
+            fg.fetch_many(ctx, jobs, milli=True)
             t5 = time.perf_counter()
+            fg.fetch_many(ctx, jobs)
+            t_fp64 = time.perf_counter() - t5
             first.close()
         finally:
             os.chdir(cwd)
     e2e = {"seconds": round(t5 - t0, 3), "prepare_s": round(t1 - t0, 3), "prepare_one_thread_s": round(t_prep1, 3),
            "host_threads": threads, "create_jobs_s": round(t2 - t1, 3), "build_graph_s": round(t3 - t2, 3),
            "first_launch_s": round(t4 - t3, 3), "fetch_maps_s": round(t5 - t4, 3),
+           "fetch_maps_fp64_s": round(t_fp64, 3),
            "form": "GPF files -> fg_prep_load_many (host threads) -> fg_job_create_many (host threads) -> "
-                   "fg_batch_create (graph) "
-                   "-> one launch -> fg_job_fetch of every map (FP64), default kernel shapes"}
+                   "fg_batch_create (graph) -> one launch -> fg_job_fetch_milli_many (every map as the int32 "
+                   "thousandths the files print, rounded on the GPU), batch default kernel shapes (no tuning); "
+                   "fetch_maps_fp64_s: the same maps as unrounded FP64 (fg_job_fetch_many), not in seconds"}
     log(f"batch end to end: {e2e}")
     pairs = sum(p.n_atoms * p.P for p in preps)
     pcut = sum(j.cutoff_pairs() for j in jobs)

@@ -125,6 +125,9 @@ SIGNATURES = {
     "fg_job_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_int]),
     "fg_sync": (C.c_int, [C.c_void_p]),
     "fg_job_fetch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_dp)]),
+    "fg_job_fetch_many": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.POINTER(_dp))]),
+    "fg_job_fetch_milli": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_ip)]),
+    "fg_job_fetch_milli_many": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.POINTER(_ip))]),
     "fg_job_cutoff_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]),
     "fg_job_summary": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FgSummary)]),
     "fg_job_get_shape": (C.c_int, [C.c_void_p, C.POINTER(FgShape)]),
@@ -152,6 +155,7 @@ SIGNATURES = {
     "fg_prep_gpf_echo": (C.c_char_p, [C.c_void_p]),
     "fg_prep_summary": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "fg_write_outputs": (C.c_int, [C.c_void_p, C.POINTER(_dp), C.c_char_p, C.c_int]),
+    "fg_write_outputs_milli": (C.c_int, [C.c_void_p, C.POINTER(_ip), _dp, C.c_char_p, C.c_int]),
     "fg_synth_write": (C.c_int, [C.POINTER(FgSynthSpec), C.c_char_p]),
     "fg_read_fld": (C.c_int, [C.c_char_p, C.POINTER(FgMapGrid)]),
     "fg_read_map": (C.c_int, [C.c_char_p, C.POINTER(FgMapGrid), C.c_int, C.c_int, C.POINTER(C.c_float)]),
@@ -387,6 +391,15 @@ class Prepared:
         ptrs = (_dp * len(keep))(*[_ptr(m, C.c_double) for m in keep])
         _check(LIB.fg_write_outputs(self._h, ptrs, program_name.encode(), threads))
 
+    def write_outputs_milli(self, maps: list[np.ndarray], fmap=None, program_name: str = "fastgrid4",
+                            threads: int = 0) -> None:
+        """The files from int32 thousandths (fg_write_outputs_milli)."""
+        keep = [np.ascontiguousarray(m, dtype=np.int32) for m in maps]
+        ptrs = (_ip * len(keep))(*[_ptr(m, C.c_int32) for m in keep])
+        fm = None if fmap is None else np.ascontiguousarray(fmap, dtype=np.float64)
+        _check(LIB.fg_write_outputs_milli(self._h, ptrs, _ptr(fm, C.c_double) if fm is not None else C.cast(None, _dp),
+                                          program_name.encode(), threads))
+
 
 def prepare_many(gpfs: list[str], threads: int = 0, v4: bool = False,
                  ag421_param_order: bool = False) -> list[Prepared]:
@@ -593,6 +606,15 @@ class Job:
         _check(LIB.fg_job_fetch(self.ctx.handle, self._h, ptrs))
         return outs
 
+    def fetch_milli(self) -> list[np.ndarray]:
+        """[affinity maps..., e, d] rounded on the device to int32 thousandths
+        (fg_job_fetch_milli): the values the map files print, times 1000."""
+        P, M = self.arrays.P, self.arrays.n_maps
+        outs = [np.empty(P, np.int32) for _ in range(M + 2)]
+        ptrs = (_ip * (M + 2))(*[_ptr(o, C.c_int32) for o in outs])
+        _check(LIB.fg_job_fetch_milli(self.ctx.handle, self._h, ptrs))
+        return outs
+
     def summary(self) -> dict:
         """GPU checks of the last run (fg_job_summary): per map the min/max of
         the rounded values, the receptor's total charge and its distance to
@@ -639,6 +661,32 @@ class Job:
 
 def _handles(jobs: list[Job]):
     return (C.c_void_p * len(jobs))(*[j.handle for j in jobs])
+
+
+def fetch_many(ctx: Context, jobs: list[Job], milli: bool = False) -> list[list[np.ndarray]]:
+    """Every job's maps in one pipelined copy (fg_job_fetch_many, or
+    fg_job_fetch_milli_many with milli: int32 thousandths, no floating
+    grid).  The maps of all jobs are views of one host array: one allocation
+    is large enough for numpy to ask for transparent huge pages, so the first
+    writes fault once per 2 MiB instead of once per 4 KiB page.  A view keeps
+    the whole array alive."""
+    dt = np.dtype(np.int32 if milli else np.float64)
+    counts = [j.arrays.n_maps + 2 + (1 if (j.arrays.floating_grid and not milli) else 0) for j in jobs]
+    sizes = [int(j.arrays.P) for j in jobs]
+    starts = np.zeros(len(jobs) + 1, np.int64)
+    np.cumsum([c * p for c, p in zip(counts, sizes)], out=starts[1:])
+    buf = np.empty(max(int(starts[-1]), 1), dt)
+    base = buf.ctypes.data
+    # the address of every map, then the address of every job's row of them
+    offs = np.concatenate([starts[i] + sizes[i] * np.arange(counts[i], dtype=np.int64) for i in range(len(jobs))]
+                          or [np.zeros(0, np.int64)])
+    addrs = (base + offs * dt.itemsize).astype(np.uintp)
+    row0 = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int64) if jobs else np.zeros(1, np.int64)
+    rows = (addrs.ctypes.data + row0 * addrs.itemsize).astype(np.uintp)
+    fn = LIB.fg_job_fetch_milli_many if milli else LIB.fg_job_fetch_many
+    _check(fn(ctx.handle, _handles(jobs), len(jobs), C.cast(rows.ctypes.data, C.POINTER(C.POINTER(_ip if milli else _dp)))))
+    return [[buf[int(o):int(o) + sizes[i]] for o in offs[int(row0[i]):int(row0[i]) + counts[i]]]
+            for i in range(len(jobs))]
 
 
 def autotune(ctx: Context, jobs: list[Job], precision: int = PREC_COMPENSATED) -> str:

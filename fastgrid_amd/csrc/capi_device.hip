@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -101,6 +102,17 @@ struct fg_ctx {
     std::vector<fg_job *> jobs;
     std::vector<fg_batch *> batches;
     std::vector<fg_mapset *> mapsets;
+    // pinned staging of device-to-host copies (fetches), made on first use
+    void *pin[2] = {nullptr, nullptr};
+    hipEvent_t pinEv[2] = {nullptr, nullptr};
+    // pinned staging of job creation's uploads, grown to the largest creation
+    unsigned char *upPin = nullptr;
+    size_t upPinBytes = 0;
+    // the distinct dielectric tables (first kTableLen FP32 entries) of the
+    // jobs made on this context, at most kMaxEpsTables: a job's epsId indexes
+    // them (-1: not kept), so a batch compares ids instead of tables
+    std::mutex epsMu;
+    std::vector<std::vector<float>> epsTables;
 };
 
 // After a failure inside a capture that may have forked the side stream:
@@ -137,6 +149,7 @@ struct fg_job {
     double *dFar = nullptr;  // [P] far-field e of the compensated form
     double *dCharge = nullptr;   // [N] partial charges (when given), for the summary
     double *dSummary = nullptr;  // [M + 2][2] min/max of the rounded maps, then the total charge
+    int32_t *dMilli = nullptr;   // [M + 2][P] rounded thousandths (fg_job_fetch_milli), then an overflow flag
     int32_t *dCounter = nullptr;  // work counter of the persistent e-kernel
     int elecPerCU[2][2][2] = {};  // e-kernel workgroups per CU by [px - 1][pz == 8][compensated]
     // the graph of the last compensated or FP32 run and what it was made for
@@ -158,11 +171,13 @@ struct fg_job {
     float4 *dTiles = nullptr;
     double *dTiles64 = nullptr;
     float *dSubBox = nullptr, *dEps32 = nullptr;
+    int epsId = -1;  // the context's id of the job's dielectric table (-1: none)
     int nAtoms32 = 0, tableLen = fgk::kTableLen;
     bool clamp = true;
     // k_elec shape: points per lane along x and z, atoms per tile, lanes
     // along x (defaults; the autotuner may change them)
     int px = 1, pz = 4, tile = 64, bxShift = 4;
+    bool shaped = false;  // the shape came from fg_autotune or fg_job_set_shape
     int plainDiel = FG_PLAIN_DIEL_TABLE;  // dielectric variant of the plain e-kernel
     float farCut2 = 0, lastCut2 = 0, tableCut2 = 0, r2Lo = 0, r2Hi = 0, spacing100 = 0;
     // plain form (FP32 throughout)
@@ -178,11 +193,13 @@ struct fg_job {
     float *dFmap = nullptr;
     // outputs: [M affinity][e][d]
     double *dMaps = nullptr;
-    std::vector<void *> owned;
-    // Device buffers are carved out of a few allocations: devAlloc/devUpload
-    // record a slot (and stage the upload's bytes); flushArena makes one
-    // allocation for the recorded slots, one copy of the staged bytes, and
-    // sets the recorded pointers.
+    // the device allocations the job's buffers are carved from; jobs created
+    // together share them, and the last job to go frees them
+    std::vector<std::shared_ptr<void>> owned;
+    // Device buffers are carved out of a few arenas: devAlloc/devUpload
+    // record a slot (and stage the upload's bytes); jobsFinishDevice places
+    // every recorded slot, copies the staged bytes in and sets the recorded
+    // pointers.
     struct Slot {
         void **dst;          // the job's pointer to set
         size_t off, bytes;   // place in the arena
@@ -196,7 +213,7 @@ struct fg_job {
 namespace {
 
 // A device buffer of `count` elements in the job's next arena; `ptr` is set
-// by flushArena.
+// when the arena is placed.
 template <class T>
 int devAlloc(fg_job *j, T *&ptr, size_t count)
 {
@@ -222,32 +239,13 @@ int devUpload(fg_job *j, T *&ptr, const T *host, size_t count)
     return FG_OK;
 }
 
-// As devUpload for caller memory that stays valid until flushArena: copied
-// straight from `host` (large arrays are not staged twice).
+// As devUpload for caller memory that stays valid until the job's device
+// half has run: copied from `host` then (large arrays are not staged twice).
 template <class T>
 int devUploadDirect(fg_job *j, T *&ptr, const T *host, size_t count)
 {
     if (int rc = devAlloc(j, ptr, count)) return rc;
     if (count) j->slots.back().direct = host;
-    return FG_OK;
-}
-
-// One allocation for every slot recorded since the last flush, one copy of
-// the staged bytes, then the direct copies.
-int flushArena(fg_job *j)
-{
-    if (j->slots.empty()) return FG_OK;
-    void *base = nullptr;
-    FG_HIP(hipMalloc(&base, j->arenaBytes));
-    j->owned.push_back(base);
-    char *b = static_cast<char *>(base);
-    for (const fg_job::Slot &sl : j->slots) *sl.dst = b + sl.off;
-    if (!j->stage.empty()) FG_HIP(hipMemcpy(b, j->stage.data(), j->stage.size(), hipMemcpyHostToDevice));
-    for (const fg_job::Slot &sl : j->slots)
-        if (sl.direct) FG_HIP(hipMemcpy(b + sl.off, sl.direct, sl.bytes, hipMemcpyHostToDevice));
-    j->slots.clear();
-    std::vector<unsigned char>().swap(j->stage);
-    j->arenaBytes = 0;
     return FG_OK;
 }
 
@@ -261,12 +259,15 @@ uint32_t spread3(uint32_t v)  // 10 bits -> every third bit
     return v;
 }
 
-// Launches the candidate-list count pass and the two scans.
-int runListCount(fg_job *j, const fgk::ListArgs &la)
+// Launches the candidate-list count pass and the two scans; the list totals
+// also go to tot[0] (all candidates) and tot[1] (H-bond candidates).
+int runListCount(fg_job *j, const fgk::ListArgs &la, int32_t *tot)
 {
     hipLaunchKernelGGL(fgk::k_brick_lists<false>, dim3(j->nb), dim3(fgk::kListThreads), 0, j->ctx->stream, la);
-    hipLaunchKernelGGL(fgk::k_scan_counts, dim3(1), dim3(1024), 0, j->ctx->stream, j->dCounts, j->nb, 0, j->dStart);
-    hipLaunchKernelGGL(fgk::k_scan_counts, dim3(1), dim3(1024), 0, j->ctx->stream, j->dCounts, j->nb, 1, j->dHStart);
+    hipLaunchKernelGGL(fgk::k_scan_counts, dim3(1), dim3(1024), 0, j->ctx->stream, j->dCounts, j->nb, 0, j->dStart,
+                       tot);
+    hipLaunchKernelGGL(fgk::k_scan_counts, dim3(1), dim3(1024), 0, j->ctx->stream, j->dCounts, j->nb, 1, j->dHStart,
+                       tot + 1);
     FG_HIP(hipGetLastError());
     return FG_OK;
 }
@@ -497,7 +498,22 @@ int launchElec(fg_job *j, int precision, int reserve, hipStream_t s)
 // ElecArgs).
 constexpr int kTableMaxNoClamp = 12288;
 
-int buildElecTiles(fg_job *j, const fg_job_desc &d, double gridMaxAbs)
+constexpr size_t kMaxEpsTables = 64;
+
+// The id of dielectric table t[0..kTableLen) among ctx's tables (bit-equal
+// tables share an id); -1 once kMaxEpsTables distinct ones are kept.
+int internEps(fg_ctx *ctx, const float *t)
+{
+    std::lock_guard<std::mutex> lock(ctx->epsMu);
+    const size_t bytes = size_t(fgk::kTableLen) * sizeof(float);
+    for (size_t i = 0; i < ctx->epsTables.size(); ++i)
+        if (std::memcmp(ctx->epsTables[i].data(), t, bytes) == 0) return int(i);
+    if (ctx->epsTables.size() >= kMaxEpsTables) return -1;
+    ctx->epsTables.emplace_back(t, t + fgk::kTableLen);
+    return int(ctx->epsTables.size() - 1);
+}
+
+int buildElecTiles(fg_ctx *ctx, fg_job *j, const fg_job_desc &d, double gridMaxAbs)
 {
     const int N = j->N, TT = fgk::kSubTile;
     const fg_atoms &A = d.atoms;
@@ -613,6 +629,7 @@ int buildElecTiles(fg_job *j, const fg_job_desc &d, double gridMaxAbs)
         double v = d.dist_dep_diel ? eps[std::min(k, fgk::kTableLen - 1)] : 1.0;
         eps32[size_t(k)] = float(v);
     }
+    j->epsId = internEps(ctx, eps32.data());
     if (int rc = devUpload(j, j->dTiles, tiles.data(), tiles.size())) return rc;
     if (int rc = devUpload(j, j->dTiles64, t64.data(), t64.size())) return rc;
     if (int rc = devUpload(j, j->dSubBox, box.data(), box.size())) return rc;
@@ -772,23 +789,6 @@ int buildCutoff(fg_job *j, const fg_job_desc &d)
 
 // Candidate lists: one count pass sizes them (the inputs are on the device),
 // then they and the type-sorted copies get an arena of their own.
-int buildLists(fg_job *j)
-{
-    fgk::ListArgs la = listArgs(j);
-    if (int rc = runListCount(j, la)) return rc;
-    int32_t tot = 0, totH = 0;
-    FG_HIP(hipMemcpyAsync(&tot, j->dStart + j->nb, sizeof tot, hipMemcpyDeviceToHost, j->ctx->stream));
-    FG_HIP(hipMemcpyAsync(&totH, j->dHStart + j->nb, sizeof totH, hipMemcpyDeviceToHost, j->ctx->stream));
-    FG_HIP(hipStreamSynchronize(j->ctx->stream));
-    if (int rc = devAlloc(j, j->dList, size_t(tot))) return rc;
-    if (int rc = devAlloc(j, j->dHList, size_t(totH))) return rc;
-    if (j->typed) {
-        if (int rc = devAlloc(j, j->dSorted, size_t(tot))) return rc;
-        if (int rc = devAlloc(j, j->dTypeStart, size_t(j->nb) * size_t(j->T + 1))) return rc;
-    }
-    return flushArena(j);
-}
-
 // FP32 inputs of the plain form: atoms in index order, tables in output map order.
 int buildPlain(fg_job *j, const fg_job_desc &d)
 {
@@ -930,7 +930,7 @@ void destroyJob(fg_job *j)
     if (j->ctx) forget(j->ctx->jobs, j);
     if (j->runExec) (void)hipGraphExecDestroy(j->runExec);
     if (j->runGraph) (void)hipGraphDestroy(j->runGraph);
-    for (void *p : j->owned) (void)hipFree(p);
+    j->owned.clear();  // frees the allocations no other job shares
     delete j;
 }
 
@@ -1363,10 +1363,29 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
     for (fg_job *j : b->jobs) b->fmap = b->fmap || ((what & FG_RUN_FLOATING) && j->floating);
     // one kernel variant (points per lane along x and z) for the batch: the
     // one carrying the most work
+    // jobs never given a shape run at the one the autotuner picks for most
+    // pockets of a batch: 8 points per lane along z, 256-column bricks,
+    // 64-atom tiles (shapes change no bits)
+    const bool batchDefault = n > 1;
+    auto shapeOf = [&](const fg_job *j, int &px, int &pz, int &tile, int &shift) {
+        if (batchDefault && !j->shaped) {
+            px = 1;
+            pz = 8;
+            tile = 64;
+            shift = 0;
+        } else {
+            px = j->px;
+            pz = j->pz;
+            tile = j->tile;
+            shift = j->bxShift;
+        }
+    };
     double work[4] = {0, 0, 0, 0};
     int maxM = 0;
     for (fg_job *j : b->jobs) {
-        work[(j->px == 2 ? 2 : 0) + (j->pz == 8 ? 1 : 0)] += double(j->N) * double(j->P);
+        int jpx, jpz, jtile, jshift;
+        shapeOf(j, jpx, jpz, jtile, jshift);
+        work[(jpx == 2 ? 2 : 0) + (jpz == 8 ? 1 : 0)] += double(j->N) * double(j->P);
         maxM = std::max(maxM, j->M);
         b->KH = std::max(b->KH, j->KH);
         b->pointAng = b->pointAng || j->pointAng;
@@ -1386,15 +1405,10 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
     }
     b->MM = maxM <= 4 ? 4 : b->KH > 4 ? 16 : maxM <= 6 ? 6 : maxM <= 8 ? 8 : 16;
     if (b->aff && maxM == 0) b->aff = false;
-    // one LDS table for all jobs when their dielectric tables agree
+    // one LDS table for all jobs when their dielectric tables are bit-equal
     {
         bool same = true;
-        std::vector<float> t0(static_cast<size_t>(fgk::kTableLen)), t1(static_cast<size_t>(fgk::kTableLen));
-        FG_HIP(hipMemcpy(t0.data(), b->jobs[0]->dEps32, t0.size() * sizeof(float), hipMemcpyDeviceToHost));
-        for (size_t i = 1; i < b->jobs.size() && same; ++i) {
-            FG_HIP(hipMemcpy(t1.data(), b->jobs[i]->dEps32, t1.size() * sizeof(float), hipMemcpyDeviceToHost));
-            same = t0 == t1;
-        }
+        for (const fg_job *j : b->jobs) same = same && j->epsId >= 0 && j->epsId == b->jobs[0]->epsId;
         b->sharedTableLen = same ? b->maxTableLen : 0;
     }
     std::vector<fgk::ListArgs> la;
@@ -1407,8 +1421,10 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
         if (!(b->typed && b->aff)) ca.back().brickList = j->dList;
         // two points per lane along x need a 2D brick (the shape does not
         // change the results)
-        const int shift = b->px == 2 && j->bxShift == 0 ? 4 : j->bxShift;
-        fgk::ElecArgs e = elecArgs(j, b->px, b->pz, j->tile, shift);
+        int jpx, jpz, jtile, jshift;
+        shapeOf(j, jpx, jpz, jtile, jshift);
+        const int shift = b->px == 2 && jshift == 0 ? 4 : jshift;
+        fgk::ElecArgs e = elecArgs(j, b->px, b->pz, jtile, shift);
         if (b->sharedTableLen > 0) e.clampMax = b->sharedTableLen - 1;
         ea.push_back(e);
         const int64_t c = int64_t(cutBase.back()) + j->nb, el = int64_t(elecBase.back()) + int64_t(e.nbx) * e.nby * e.nbz;
@@ -1499,8 +1515,11 @@ void fg_destroy(fg_ctx *ctx)
     for (fg_job *j : ctx->jobs) j->ctx = nullptr;
     for (fg_batch *b : ctx->batches) b->ctx = nullptr;
     for (fg_mapset *m : ctx->mapsets) m->ctx = nullptr;
-    for (hipEvent_t e : {ctx->ev[0], ctx->ev[1], ctx->ev[2], ctx->ev[3], ctx->fork, ctx->e0, ctx->e1, ctx->join})
+    for (hipEvent_t e : {ctx->ev[0], ctx->ev[1], ctx->ev[2], ctx->ev[3], ctx->fork, ctx->e0, ctx->e1, ctx->join,
+                         ctx->pinEv[0], ctx->pinEv[1]})
         if (e) (void)hipEventDestroy(e);
+    for (void *p : ctx->pin)
+        if (p) (void)hipHostFree(p);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     delete ctx;
@@ -1612,12 +1631,13 @@ int jobBuildHost(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
     }
     // inputs and outputs staged for one arena
     int rc = buildCutoff(j, *d);
-    if (!rc) rc = buildElecTiles(j, *d, gridMaxAbs);
+    if (!rc) rc = buildElecTiles(ctx, j, *d, gridMaxAbs);
     if (!rc) rc = buildPlain(j, *d);
     if (!rc) rc = devAlloc(j, j->dMaps, size_t(j->M + 2) * size_t(j->P));
     if (!rc) rc = devAlloc(j, j->dFar, size_t(j->P));
     if (!rc) rc = devAlloc(j, j->dCounter, size_t(1));
     if (!rc) rc = devAlloc(j, j->dSummary, size_t(2) * size_t(j->M + 2) + 1);
+    if (!rc) rc = devAlloc(j, j->dMilli, size_t(j->M + 2) * size_t(j->P) + 1);
     if (!rc && A.charge && j->N > 0) rc = devUploadDirect(j, j->dCharge, A.charge, size_t(j->N));
     if (!rc && j->floating) rc = devAlloc(j, j->dFmap, size_t(j->P));
     if (rc) {
@@ -1629,26 +1649,158 @@ int jobBuildHost(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
     return FG_OK;
 }
 
-// Device half: the first arena (inputs, outputs), the count pass and the
-// candidate lists' arena, cleared outputs; registers the job with ctx.  The
-// caller-owned arrays of `d` are read again here (direct uploads).  On
-// failure the job is destroyed.
-int jobFinishDevice(fg_ctx *ctx, fg_job *j)
+// Places the slots a job recorded at `base` (its arena) and clears them.
+void placeArena(fg_job *j, char *base)
 {
-    int rc = flushArena(j);
-    if (!rc) rc = elecOccupancy(j);
-    if (!rc) rc = buildLists(j);
-    if (!rc) {
-        hipError_t e = hipMemsetAsync(j->dMaps, 0, size_t(j->M + 2) * size_t(j->P) * sizeof(double), ctx->stream);
-        if (e == hipSuccess) e = hipMemsetAsync(j->dAcc32, 0, size_t(j->P) * sizeof(float), ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) rc = fgc::fail(FG_E_DEVICE, std::string("fg_job_create: ") + hipGetErrorString(e));
+    for (const fg_job::Slot &sl : j->slots) *sl.dst = base + sl.off;
+    j->slots.clear();
+    j->arenaBytes = 0;
+}
+
+// One device allocation of `bytes`, freed when the last owner lets go.
+int sharedBlock(size_t bytes, std::shared_ptr<void> &out)
+{
+    void *p = nullptr;
+    FG_HIP(hipMalloc(&p, std::max<size_t>(bytes, 1)));
+    out.reset(p, [](void *q) { (void)hipFree(q); });
+    return FG_OK;
+}
+
+// Host bytes `src` that go to byte `dst` of the shared block, staged at
+// byte `pin` of the context's pinned upload buffer.
+struct Upload {
+    size_t dst;
+    const void *src;
+    size_t bytes, pin;
+};
+
+// Copies the uploads' bytes into ctx->upPin on up to `threads` threads, in
+// pieces of at most 2 MiB.
+void stageUploads(fg_ctx *ctx, const std::vector<Upload> &ups, int threads)
+{
+    struct Piece {
+        const unsigned char *src;
+        unsigned char *dst;
+        size_t bytes;
+    };
+    std::vector<Piece> pieces;
+    size_t total = 0;
+    for (const Upload &u : ups)
+        for (size_t o = 0; o < u.bytes; o += size_t(2) << 20) {
+            const size_t b = std::min(u.bytes - o, size_t(2) << 20);
+            pieces.push_back(Piece{static_cast<const unsigned char *>(u.src) + o, ctx->upPin + u.pin + o, b});
+            total += b;
+        }
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        for (size_t i; (i = next.fetch_add(1)) < pieces.size();)
+            std::memcpy(pieces[i].dst, pieces[i].src, pieces[i].bytes);
+    };
+    threads = int(std::min<size_t>(size_t(std::max(threads, 1)), std::max<size_t>(total >> 21, 1)));
+    std::vector<std::thread> pool;
+    try {
+        for (int t = 1; t < threads; ++t) pool.emplace_back(work);
+    } catch (const std::system_error &) {
+        // fewer threads: the running ones share the pieces
     }
-    if (rc) {
-        destroyJob(j);
+    work();
+    for (std::thread &t : pool) t.join();
+}
+
+// Device half of creating jobs[0..n) on ctx's stream, with two device
+// allocations for all of them: the first arenas (inputs and outputs, cleared,
+// then the staged inputs and the caller arrays copied in from pinned memory),
+// then, after the count passes and one read-back of every job's list totals,
+// the candidate lists' arenas.  Registers the jobs with ctx.  The caller-owned
+// arrays of the descriptions are read here (direct uploads).  On failure
+// every job is destroyed.
+int jobsFinishDevice(fg_ctx *ctx, fg_job *const *jobs, int n, int threads)
+{
+    auto fail = [&](int rc) {
+        (void)hipStreamSynchronize(ctx->stream);
+        for (int i = 0; i < n; ++i) destroyJob(jobs[i]);
         return rc;
+    };
+    auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
+    // the first arenas, then two list totals per job
+    std::vector<size_t> off(static_cast<size_t>(n));
+    std::vector<Upload> ups;
+    size_t total = 0, pinBytes = 0;
+    for (int i = 0; i < n; ++i) {
+        fg_job *j = jobs[i];
+        off[size_t(i)] = total;
+        auto add = [&](size_t dst, const void *src, size_t bytes) {
+            if (!bytes) return;
+            ups.push_back(Upload{dst, src, bytes, pinBytes});
+            pinBytes = up(pinBytes + bytes);
+        };
+        add(total, j->stage.data(), j->stage.size());
+        for (const fg_job::Slot &sl : j->slots)
+            if (sl.direct) add(total + sl.off, sl.direct, sl.bytes);
+        total = up(total + j->arenaBytes);
     }
-    ctx->jobs.push_back(j);
+    const size_t totOff = total;
+    total += size_t(2) * size_t(n) * sizeof(int32_t);
+    // pinned staging: the host copies run on many threads and the DMAs do not
+    // block the host; pageable copies when pinned memory cannot be had
+    if (pinBytes > ctx->upPinBytes) {
+        if (ctx->upPin) (void)hipHostFree(ctx->upPin);
+        ctx->upPin = nullptr;
+        ctx->upPinBytes = 0;
+        void *p = nullptr;
+        if (hipHostMalloc(&p, pinBytes, hipHostMallocDefault) == hipSuccess) {
+            ctx->upPin = static_cast<unsigned char *>(p);
+            ctx->upPinBytes = pinBytes;
+        } else {
+            (void)hipGetLastError();
+        }
+    }
+    const bool pinned = ctx->upPin != nullptr && pinBytes <= ctx->upPinBytes;
+    if (pinned) stageUploads(ctx, ups, threads);
+    std::shared_ptr<void> block;
+    if (int rc = sharedBlock(total, block)) return fail(rc);
+    char *base = static_cast<char *>(block.get());
+    for (int i = 0; i < n; ++i) {
+        placeArena(jobs[i], base + off[size_t(i)]);
+        jobs[i]->owned.push_back(block);
+    }
+    int32_t *dTot = reinterpret_cast<int32_t *>(base + totOff);
+    std::vector<int32_t> tot(size_t(2) * size_t(n));
+    hipError_t e = hipMemsetAsync(base, 0, total, ctx->stream);
+    for (size_t u = 0; u < ups.size() && e == hipSuccess; ++u)
+        e = hipMemcpyAsync(base + ups[u].dst, pinned ? static_cast<const void *>(ctx->upPin + ups[u].pin) : ups[u].src,
+                           ups[u].bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return fail(fgc::fail(FG_E_DEVICE, std::string("fg_job_create: ") + hipGetErrorString(e)));
+    for (int i = 0; i < n; ++i) {
+        if (int rc = elecOccupancy(jobs[i])) return fail(rc);
+        if (int rc = runListCount(jobs[i], listArgs(jobs[i]), dTot + 2 * i)) return fail(rc);
+    }
+    e = hipMemcpyAsync(tot.data(), dTot, tot.size() * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(fgc::fail(FG_E_DEVICE, std::string("fg_job_create: ") + hipGetErrorString(e)));
+    // the candidate lists
+    size_t total2 = 0;
+    for (int i = 0; i < n; ++i) {
+        fg_job *j = jobs[i];
+        std::vector<unsigned char>().swap(j->stage);
+        off[size_t(i)] = total2;
+        const int32_t t0 = tot[size_t(2 * i)], t1 = tot[size_t(2 * i + 1)];
+        if (t0 < 0 || t1 < 0) return fail(fgc::fail(FG_E_DEVICE, "fg_job_create: candidate count overflow"));
+        devAlloc(j, j->dList, size_t(t0));
+        devAlloc(j, j->dHList, size_t(t1));
+        if (j->typed) {
+            devAlloc(j, j->dSorted, size_t(t0));
+            devAlloc(j, j->dTypeStart, size_t(j->nb) * size_t(j->T + 1));
+        }
+        total2 = up(total2 + j->arenaBytes);
+    }
+    std::shared_ptr<void> block2;
+    if (int rc = sharedBlock(total2, block2)) return fail(rc);
+    for (int i = 0; i < n; ++i) {
+        placeArena(jobs[i], static_cast<char *>(block2.get()) + off[size_t(i)]);
+        jobs[i]->owned.push_back(block2);
+    }
+    for (int i = 0; i < n; ++i) ctx->jobs.push_back(jobs[i]);
     return FG_OK;
 }
 
@@ -1661,7 +1813,7 @@ int fg_job_create(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
     FG_HIP(hipSetDevice(ctx->device));
     fg_job *j = nullptr;
     if (int rc = jobBuildHost(ctx, d, &j)) return rc;
-    if (int rc = jobFinishDevice(ctx, j)) return rc;
+    if (int rc = jobsFinishDevice(ctx, &j, 1, 8)) return rc;
     *out = j;
     return FG_OK;
 }
@@ -1672,6 +1824,7 @@ int fg_job_create_many(fg_ctx *ctx, const fg_job_desc *descs, int n, int threads
     for (int i = 0; i < n; ++i) out[i] = nullptr;
     FG_HIP(hipSetDevice(ctx->device));
     if (threads <= 0) threads = int(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+    const int copyThreads = std::min(threads, 16);
     threads = std::min(threads, std::max(n, 1));
     std::vector<int> rcs(size_t(n), FG_OK);
     std::vector<std::string> errs(static_cast<size_t>(n));
@@ -1698,25 +1851,19 @@ int fg_job_create_many(fg_ctx *ctx, const fg_job_desc *descs, int n, int threads
             rc = rcs[size_t(i)];
             msg = errs[size_t(i)];
         }
-    // the device halves, in list order, on the context's stream
-    for (int i = 0; i < n && bad < 0; ++i) {
-        fg_job *j = out[i];
-        out[i] = nullptr;
-        if ((rc = jobFinishDevice(ctx, j)) != FG_OK) {
-            bad = i;
-            msg = fg_last_error();
-        } else {
-            out[i] = j;
+    if (bad >= 0) {
+        for (int i = 0; i < n; ++i) {
+            destroyJob(out[i]);  // host halves only: nothing on the device yet
+            out[i] = nullptr;
         }
+        return fgc::fail(rc, "job " + std::to_string(bad) + ": " + msg);
     }
-    if (bad < 0) return FG_OK;
-    for (int i = 0; i < n; ++i) {
-        if (!out[i]) continue;
-        if (out[i]->ctx) (void)hipStreamSynchronize(ctx->stream);
-        destroyJob(out[i]);
-        out[i] = nullptr;
+    // the device halves of all jobs together, on the context's stream
+    if ((rc = jobsFinishDevice(ctx, out, n, copyThreads)) != FG_OK) {
+        for (int i = 0; i < n; ++i) out[i] = nullptr;  // destroyed by jobsFinishDevice
+        return rc;
     }
-    return fgc::fail(rc, "job " + std::to_string(bad) + ": " + msg);
+    return FG_OK;
 }
 
 void fg_job_destroy(fg_job *job)
@@ -1866,22 +2013,198 @@ int fg_last_timing(fg_ctx *ctx, fg_timing *t)
     return FG_OK;
 }
 
+namespace {
+
+// Device-to-host copies go through the context's two pinned staging buffers:
+// the DMA of one buffer overlaps the host's copy of the other out to the
+// caller's (pageable) memory, on several host threads for large pieces.
+// Copying straight to pageable memory runs at a fraction of the link's rate.
+constexpr size_t kStageBytes = size_t(32) << 20;
+
+struct HostSeg {
+    void *dst;
+    const void *src;
+    size_t bytes;
+};
+
+struct StagePart {
+    char *dst;
+    size_t off, bytes;  // place in the staging buffer
+};
+
+// Copies the parts of one staging buffer out to their destinations: the
+// buffer's bytes are split into equal ranges, one per host thread (up to 8,
+// at least 2 MiB each), whatever the parts' sizes.
+void copyOut(const char *pin, const std::vector<StagePart> &parts)
+{
+    size_t total = 0;
+    for (const StagePart &pt : parts) total = std::max(total, pt.off + pt.bytes);
+    constexpr size_t kPerThread = size_t(2) << 20;
+    const int n = int(std::max<size_t>(1, std::min<size_t>(8, total / kPerThread)));
+    auto range = [&](size_t lo, size_t hi) {
+        for (const StagePart &pt : parts) {
+            const size_t a = std::max(lo, pt.off), b = std::min(hi, pt.off + pt.bytes);
+            if (a < b) std::memcpy(pt.dst + (a - pt.off), pin + a, b - a);
+        }
+    };
+    if (n == 1) {
+        range(0, total);
+        return;
+    }
+    const size_t step = (total + size_t(n) - 1) / size_t(n);
+    std::vector<std::thread> pool;
+    for (int t = 1; t < n; ++t) pool.emplace_back(range, step * size_t(t), std::min(total, step * size_t(t + 1)));
+    range(0, std::min(total, step));
+    for (auto &th : pool) th.join();
+}
+
+// Copies every segment (device src -> host dst) in order and returns when all
+// are on the host.  Segments whose sources follow each other share DMAs.
+int fetchSegments(fg_ctx *ctx, const std::vector<HostSeg> &segs)
+{
+    if (!ctx->pin[0]) {
+        for (int b = 0; b < 2; ++b) {
+            FG_HIP(hipHostMalloc(&ctx->pin[b], kStageBytes, hipHostMallocDefault));
+            FG_HIP(hipEventCreateWithFlags(&ctx->pinEv[b], hipEventDisableTiming));
+        }
+    }
+    std::vector<StagePart> parts[2];
+    int cur = 0;
+    size_t fill = 0;
+    const char *runSrc = nullptr;  // the device run being gathered into buffer cur
+    size_t runOff = 0, runLen = 0;
+    auto flushRun = [&]() -> hipError_t {
+        hipError_t e = hipSuccess;
+        if (runLen) e = hipMemcpyAsync(static_cast<char *>(ctx->pin[cur]) + runOff, runSrc, runLen, hipMemcpyDeviceToHost,
+                                       ctx->stream);
+        runLen = 0;
+        return e;
+    };
+    bool pending = false;  // buffer cur ^ 1 holds a DMA whose host copy is due
+    auto finish = [&](int b) -> hipError_t {
+        const hipError_t e = hipEventSynchronize(ctx->pinEv[b]);
+        if (e != hipSuccess) return e;
+        copyOut(static_cast<const char *>(ctx->pin[b]), parts[b]);
+        parts[b].clear();
+        return hipSuccess;
+    };
+    // closes buffer cur (its DMAs and event), then copies the other one out
+    auto rotate = [&]() -> int {
+        FG_HIP(flushRun());
+        FG_HIP(hipEventRecord(ctx->pinEv[cur], ctx->stream));
+        if (pending) FG_HIP(finish(cur ^ 1));
+        pending = true;
+        cur ^= 1;
+        fill = 0;
+        return FG_OK;
+    };
+    for (const HostSeg &sg : segs) {
+        size_t done = 0;
+        while (done < sg.bytes) {
+            if (fill == kStageBytes)
+                if (int rc = rotate()) return rc;
+            const size_t len = std::min(sg.bytes - done, kStageBytes - fill);
+            const char *src = static_cast<const char *>(sg.src) + done;
+            if (runLen && runSrc + runLen == src) {
+                runLen += len;
+            } else {
+                FG_HIP(flushRun());
+                runSrc = src;
+                runOff = fill;
+                runLen = len;
+            }
+            parts[cur].push_back(StagePart{static_cast<char *>(sg.dst) + done, fill, len});
+            fill += len;
+            done += len;
+        }
+    }
+    if (fill)
+        if (int rc = rotate()) return rc;
+    if (pending) FG_HIP(finish(cur ^ 1));
+    return FG_OK;
+}
+
+// The fetch segments of job j: its maps into out[m] (a null entry is
+// skipped), the floating grid (floats) into fm.
+void jobSegments(const fg_job *j, double *const *out, std::vector<float> &fm, std::vector<HostSeg> &segs)
+{
+    const size_t bytes = size_t(j->P) * sizeof(double);
+    for (int m = 0; m < j->M + 2; ++m)
+        if (out[m]) segs.push_back(HostSeg{out[m], j->dMaps + int64_t(m) * j->P, bytes});
+    if (j->floating && out[j->M + 2]) {
+        fm.resize(size_t(j->P));
+        segs.push_back(HostSeg{fm.data(), j->dFmap, fm.size() * sizeof(float)});
+    }
+}
+
+}  // namespace
+
+int fg_job_fetch_many(fg_ctx *ctx, fg_job *const *jobs, int n, double *const *const *outs)
+{
+    if (!ctx || (n > 0 && (!jobs || !outs))) return fgc::fail(FG_E_ARG, "fg_job_fetch_many: NULL argument");
+    for (int i = 0; i < n; ++i)
+        if (!jobs[i] || jobs[i]->ctx != ctx || !outs[i])
+            return fgc::fail(FG_E_ARG, "fg_job_fetch_many: job " + std::to_string(i) + " is NULL or not of this context");
+    FG_HIP(hipSetDevice(ctx->device));
+    std::vector<HostSeg> segs;
+    std::vector<std::vector<float>> fms(static_cast<size_t>(std::max(n, 0)));
+    for (int i = 0; i < n; ++i) jobSegments(jobs[i], outs[i], fms[size_t(i)], segs);
+    if (int rc = fetchSegments(ctx, segs)) return rc;
+    for (int i = 0; i < n; ++i)
+        for (size_t p = 0; p < fms[size_t(i)].size(); ++p) outs[i][jobs[i]->M + 2][p] = double(fms[size_t(i)][p]);
+    return FG_OK;
+}
+
 int fg_job_fetch(fg_ctx *ctx, fg_job *j, double *const *out)
 {
     if (!ctx || !j || !out) return fgc::fail(FG_E_ARG, "fg_job_fetch: NULL argument");
+    fg_job *js[1] = {j};
+    double *const *os[1] = {out};
+    return fg_job_fetch_many(ctx, js, 1, os);
+}
+
+int fg_job_fetch_milli_many(fg_ctx *ctx, fg_job *const *jobs, int n, int32_t *const *const *outs)
+{
+    if (!ctx || (n > 0 && (!jobs || !outs))) return fgc::fail(FG_E_ARG, "fg_job_fetch_milli_many: NULL argument");
+    for (int i = 0; i < n; ++i)
+        if (!jobs[i] || jobs[i]->ctx != ctx || !outs[i])
+            return fgc::fail(FG_E_ARG,
+                             "fg_job_fetch_milli_many: job " + std::to_string(i) + " is NULL or not of this context");
     FG_HIP(hipSetDevice(ctx->device));
-    for (int m = 0; m < j->M + 2; ++m)
-        if (out[m])
-            FG_HIP(hipMemcpyAsync(out[m], j->dMaps + int64_t(m) * j->P, size_t(j->P) * sizeof(double),
-                                  hipMemcpyDeviceToHost, ctx->stream));
-    std::vector<float> fm;
-    if (j->floating && out[j->M + 2]) {
-        fm.resize(size_t(j->P));
-        FG_HIP(hipMemcpyAsync(fm.data(), j->dFmap, fm.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    hipStream_t s = ctx->stream;
+    // round every job's maps on the device (each job's buffer made on first use)
+    for (int i = 0; i < n; ++i) {
+        fg_job *j = jobs[i];
+        int32_t *overflow = j->dMilli + size_t(j->M + 2) * size_t(j->P);
+        const int64_t cnt = int64_t(j->M + 2) * j->P;
+        hipLaunchKernelGGL(fgk::k_fill_i32, dim3(1), dim3(256), 0, s, overflow, 1, 0);
+        hipLaunchKernelGGL(fgk::k_round_milli, dim3(unsigned((cnt + 255) / 256)), dim3(256), 0, s, j->dMaps, cnt,
+                           j->dMilli, overflow);
     }
-    FG_HIP(hipStreamSynchronize(ctx->stream));
-    for (size_t p = 0; p < fm.size(); ++p) out[j->M + 2][p] = double(fm[p]);
+    FG_HIP(hipGetLastError());
+    std::vector<int32_t> over(static_cast<size_t>(std::max(n, 0)), 0);
+    std::vector<HostSeg> segs;
+    for (int i = 0; i < n; ++i) {
+        const fg_job *j = jobs[i];
+        segs.push_back(HostSeg{&over[size_t(i)], j->dMilli + size_t(j->M + 2) * size_t(j->P), sizeof(int32_t)});
+        for (int m = 0; m < j->M + 2; ++m)
+            if (outs[i][m])
+                segs.push_back(HostSeg{outs[i][m], j->dMilli + int64_t(m) * j->P, size_t(j->P) * sizeof(int32_t)});
+    }
+    if (int rc = fetchSegments(ctx, segs)) return rc;
+    for (int i = 0; i < n; ++i)
+        if (over[size_t(i)])
+            return fgc::fail(FG_E_ARG, "fg_job_fetch_milli_many: job " + std::to_string(i) +
+                                           " has map values whose thousandths exceed 32 bits; fetch it as FP64");
     return FG_OK;
+}
+
+int fg_job_fetch_milli(fg_ctx *ctx, fg_job *j, int32_t *const *out)
+{
+    if (!ctx || !j || !out) return fgc::fail(FG_E_ARG, "fg_job_fetch_milli: NULL argument");
+    fg_job *js[1] = {j};
+    int32_t *const *os[1] = {out};
+    return fg_job_fetch_milli_many(ctx, js, 1, os);
 }
 
 int fg_job_cutoff_pairs(fg_ctx *ctx, fg_job *j, int64_t *pairs)
@@ -1957,6 +2280,7 @@ int fg_job_set_shape(fg_job *job, const fg_shape *sh)
     job->pz = sh->pz;
     job->tile = sh->tile;
     job->bxShift = shift;
+    job->shaped = true;
     return FG_OK;
 }
 
@@ -1987,7 +2311,10 @@ int fg_autotune(fg_ctx *ctx, fg_job *const *jobs, int n, int precision)
     for (int i = 0; i < n; ++i)
         if (!jobs[i] || jobs[i]->ctx != ctx) return fgc::fail(FG_E_ARG, "fg_autotune: job does not belong to this context");
     FG_HIP(hipSetDevice(ctx->device));
-    return tuneJobs(ctx, jobs, n, precision == FG_PREC_COMPENSATED);
+    const int rc = tuneJobs(ctx, jobs, n, precision == FG_PREC_COMPENSATED);
+    if (!rc)
+        for (int i = 0; i < n; ++i) jobs[i]->shaped = true;
+    return rc;
 }
 
 const char *fg_tune_report(const fg_ctx *ctx) { return ctx ? ctx->tuneReport.c_str() : ""; }

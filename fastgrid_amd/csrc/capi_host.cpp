@@ -165,7 +165,17 @@ int fg_write_outputs(fg_prep *prep, double *const *maps, const char *program_nam
 {
     if (!prep || !maps) return fgc::fail(FG_E_ARG, "fg_write_outputs: NULL argument");
     std::string err;
-    int rc = fgh::writeOutputs(prep->p, maps, program_name ? program_name : "fastgrid4", threads, err);
+    int rc = fgh::writeOutputs(prep->p, maps, nullptr, nullptr, program_name ? program_name : "fastgrid4", threads,
+                               err);
+    return rc ? fgc::fail(FG_E_IO, err) : FG_OK;
+}
+
+int fg_write_outputs_milli(fg_prep *prep, const int32_t *const *maps, const double *fmap, const char *program_name,
+                           int threads)
+{
+    if (!prep || !maps) return fgc::fail(FG_E_ARG, "fg_write_outputs_milli: NULL argument");
+    std::string err;
+    int rc = fgh::writeOutputs(prep->p, nullptr, maps, fmap, program_name ? program_name : "fastgrid4", threads, err);
     return rc ? fgc::fail(FG_E_IO, err) : FG_OK;
 }
 

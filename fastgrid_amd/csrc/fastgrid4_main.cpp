@@ -200,14 +200,33 @@ int main(int argc, char **argv)
     // the log's map extremes and total charge, computed on the GPU
     fg_summary sm{};
     if (fg_job_summary(ctx, job, &sm) != FG_OK) return fatal(0xbad, fg_last_error());
-    std::vector<std::vector<double>> host(static_cast<size_t>(nOut), std::vector<double>(static_cast<size_t>(P)));
-    std::vector<double *> ptrs(static_cast<size_t>(nOut));
-    for (size_t m = 0; m < host.size(); ++m) ptrs[m] = host[m].data();
-    if (fg_job_fetch(ctx, job, ptrs.data()) != FG_OK) return fatal(0xbad, fg_last_error());
+    // the maps as the files print them: integer thousandths rounded on the
+    // device (half the bytes of FP64); a job with values past 32-bit
+    // thousandths comes back in FP64
+    std::vector<std::vector<int32_t>> milli(static_cast<size_t>(M + 2), std::vector<int32_t>(static_cast<size_t>(P)));
+    std::vector<int32_t *> mptrs(static_cast<size_t>(M + 2));
+    for (size_t m = 0; m < milli.size(); ++m) mptrs[m] = milli[m].data();
+    std::vector<std::vector<double>> host;
+    std::vector<double *> ptrs(static_cast<size_t>(nOut), nullptr);
+    const bool asMilli = fg_job_fetch_milli(ctx, job, mptrs.data()) == FG_OK;
+    if (asMilli) {
+        if (desc.floating_grid) {
+            host.assign(1, std::vector<double>(static_cast<size_t>(P)));
+            ptrs[size_t(M + 2)] = host[0].data();
+        }
+    } else {
+        host.assign(static_cast<size_t>(nOut), std::vector<double>(static_cast<size_t>(P)));
+        for (size_t m = 0; m < host.size(); ++m) ptrs[m] = host[m].data();
+    }
+    if ((!asMilli || desc.floating_grid) && fg_job_fetch(ctx, job, ptrs.data()) != FG_OK)
+        return fatal(0xbad, fg_last_error());
     fg_job_destroy(job);
     fg_destroy(ctx);
     const auto t3 = std::chrono::steady_clock::now();
-    if (fg_write_outputs(prep, ptrs.data(), prog, o.threads) != FG_OK) return fatal(-2, fg_last_error());
+    const int wrc = asMilli ? fg_write_outputs_milli(prep, mptrs.data(), desc.floating_grid ? ptrs[size_t(M + 2)] : nullptr,
+                                                     prog, o.threads)
+                            : fg_write_outputs(prep, ptrs.data(), prog, o.threads);
+    if (wrc != FG_OK) return fatal(-2, fg_last_error());
     const auto t4 = std::chrono::steady_clock::now();
 
     // summary (FG/GridMap.cpp:71-101)

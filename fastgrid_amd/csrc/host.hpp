@@ -104,7 +104,11 @@ int precompute(Prepared &p, ParamLibrary &lib, std::string &err);
 
 double round3(double a);
 // Writes the .fld and every .map file; fills p.mapMin/mapMax.
-int writeOutputs(Prepared &p, double *const *maps, const std::string &programName, int threads,
+// maps: unrounded FP64 maps [n_maps + 2 (+ floating grid)], or milli: the
+// rounded maps in integer thousandths [n_maps + 2] with the floating grid in
+// fmapValues.
+int writeOutputs(Prepared &p, double *const *maps, const int32_t *const *milli, const double *fmapValues,
+                 const std::string &programName, int threads,
                  std::string &err);
 // The .glg log: GPF echo, map summary, completion line, times.
 int writeLog(const Prepared &p, FILE *f, const std::string &programName, double realSeconds);

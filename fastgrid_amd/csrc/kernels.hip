@@ -244,10 +244,14 @@ __device__ __forceinline__ void scanCounts(const int32_t *counts, int n, int whi
     if (tid == 1023) out[n] = part[1023];
 }
 
-__global__ __launch_bounds__(1024) void k_scan_counts(const int32_t *counts, int n, int which, int32_t *out)
+// out[0..n] as scanCounts; the total out[n] also goes to *total (when given),
+// so that the totals of many jobs can be read back with one copy.
+__global__ __launch_bounds__(1024) void k_scan_counts(const int32_t *counts, int n, int which, int32_t *out,
+                                                      int32_t *total)
 {
     __shared__ int32_t part[1024];
     scanCounts(counts, n, which, out, part);
+    if (total && threadIdx.x == 1023) *total = out[n];
 }
 
 // ------------------------------------------------------------------------
@@ -1070,6 +1074,23 @@ __global__ __launch_bounds__(256) void k_add_far(int64_t P, const double *far, d
 __device__ __forceinline__ double round3(double a)
 {
     return __builtin_fabs(a) < 0.0005 ? 0.0 : floor(a * 1000 + 0.5) * 0.001;
+}
+
+// The maps as the writers print them, in integer thousandths: roundOutput's
+// floor(1000 v + 0.5) (0 below 0.0005 in magnitude).  A value whose
+// thousandths do not fit 32 bits sets *overflow (and stores 0).
+__global__ __launch_bounds__(256) void k_round_milli(const double *maps, int64_t n, int32_t *out, int32_t *overflow)
+{
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double v = maps[i];
+    const double k = __builtin_fabs(v) < 0.0005 ? 0.0 : floor(v * 1000 + 0.5);
+    if (!(__builtin_fabs(k) <= 2147483647.0)) {
+        *overflow = 1;
+        out[i] = 0;
+        return;
+    }
+    out[i] = int32_t(k);
 }
 
 // The job's summary: workgroup m < nMaps the minimum and maximum of map m's

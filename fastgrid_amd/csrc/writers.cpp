@@ -46,15 +46,24 @@ inline size_t formatValue(double v, char *out)
     return size_t(p - out);
 }
 
+// A map as unrounded FP64 values (rounded here) or as the rounded values'
+// integer thousandths.
+struct MapValues {
+    const double *v;
+    const int32_t *milli;
+    double rounded(int64_t i) const { return milli ? double(milli[i]) * 0.001 : round3(v[i]); }
+};
+
 // Formats values [lo, hi) after rounding; the buffer holds at most 24 bytes
-// per value.
-void formatRange(const double *v, int64_t lo, int64_t hi, std::string &buf, double &mn, double &mx)
+// per value.  A rounded value is k * 0.001 for the integer k, so the
+// thousandths give the same text and the same extremes.
+void formatRange(MapValues v, int64_t lo, int64_t hi, std::string &buf, double &mn, double &mx)
 {
     buf.resize(size_t(hi - lo) * 24);
     char *p = &buf[0];
     double a = 1e12, b = -1e12;
     for (int64_t i = lo; i < hi; ++i) {
-        double r = round3(v[i]);
+        double r = v.rounded(i);
         a = std::min(a, r);
         b = std::max(b, r);
         // values beyond the fixed-point range fall back to printf
@@ -78,8 +87,8 @@ void formatFloatRange(const double *v, int64_t lo, int64_t hi, std::string &buf)
 
 }  // namespace
 
-int writeOutputs(Prepared &p, double *const *maps, const std::string &programName, int threads,
-                 std::string &err)
+int writeOutputs(Prepared &p, double *const *maps, const int32_t *const *milli, const double *fmapValues,
+                 const std::string &programName, int threads, std::string &err)
 {
     const int nm = int(p.maps.size());
     // the floating grid is one more variable of the field (FG/Utils.cpp:51-81)
@@ -132,11 +141,12 @@ int writeOutputs(Prepared &p, double *const *maps, const std::string &programNam
     std::vector<std::string> chunks(static_cast<size_t>(threads));
     std::vector<double> mn(static_cast<size_t>(threads)), mx(static_cast<size_t>(threads));
     for (int m = 0; m < nm; ++m) {
-        if (p.maps[size_t(m)].filename.empty() || !maps[m]) continue;
+        const MapValues mv{maps ? maps[m] : nullptr, milli ? milli[m] : nullptr};
+        if (p.maps[size_t(m)].filename.empty() || (!mv.v && !mv.milli)) continue;
         std::vector<std::thread> pool;
         for (int t = 0; t < threads; ++t) {
             int64_t lo = P * t / threads, hi = P * (t + 1) / threads;
-            pool.emplace_back(formatRange, maps[m], lo, hi, std::ref(chunks[size_t(t)]), std::ref(mn[size_t(t)]),
+            pool.emplace_back(formatRange, mv, lo, hi, std::ref(chunks[size_t(t)]), std::ref(mn[size_t(t)]),
                               std::ref(mx[size_t(t)]));
         }
         for (auto &th : pool) th.join();
@@ -161,11 +171,12 @@ int writeOutputs(Prepared &p, double *const *maps, const std::string &programNam
         p.mapMin[size_t(m)] = a;
         p.mapMax[size_t(m)] = b;
     }
-    if (fmap && maps[nm]) {
+    const double *fv = maps ? maps[nm] : fmapValues;
+    if (fmap && fv) {
         std::vector<std::thread> pool;
         for (int t = 0; t < threads; ++t) {
             int64_t lo = P * t / threads, hi = P * (t + 1) / threads;
-            pool.emplace_back(formatFloatRange, maps[nm], lo, hi, std::ref(chunks[size_t(t)]));
+            pool.emplace_back(formatFloatRange, fv, lo, hi, std::ref(chunks[size_t(t)]));
         }
         for (auto &th : pool) th.join();
         FILE *mf = std::fopen(p.fmapPath.c_str(), "w");

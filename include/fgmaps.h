@@ -248,6 +248,19 @@ int fg_sync(fg_ctx *ctx);
  * one (P doubles each, the floating grid's values are floats; NULL entries
  * are skipped). */
 int fg_job_fetch(fg_ctx *ctx, fg_job *job, double *const *out);
+/* fg_job_fetch of many jobs at once: outs[i] is job i's out array.  The
+ * copies of all jobs are pipelined through the context's pinned staging
+ * buffers (also for fg_job_fetch). */
+int fg_job_fetch_many(fg_ctx *ctx, fg_job *const *jobs, int n, double *const *const *outs);
+/* The maps as the writers print them, rounded on the device to integer
+ * thousandths (roundOutput's floor(1000 v + 0.5), 0 below 0.0005 in
+ * magnitude, FG/autogrid.h:128-131): out[m] for m = 0 .. n_maps + 1 (P
+ * int32 each, NULL entries skipped), half the bytes of the FP64 maps.  Fails
+ * with FG_E_ARG if a value's thousandths do not fit 32 bits (|v| > 2.1e6;
+ * fetch such a job with fg_job_fetch).  Every job holds a device buffer for
+ * the rounded maps (4 bytes per value).  No reference counterpart. */
+int fg_job_fetch_milli(fg_ctx *ctx, fg_job *job, int32_t *const *out);
+int fg_job_fetch_milli_many(fg_ctx *ctx, fg_job *const *jobs, int n, int32_t *const *const *outs);
 /* Number of (point, atom) pairs with r <= 8 A (computed on the device). */
 int fg_job_cutoff_pairs(fg_ctx *ctx, fg_job *job, int64_t *pairs);
 
@@ -391,6 +404,11 @@ int fg_prep_summary(const fg_prep *prep, int m, double *min, double *max);
  * with "%.3f" as the reference prints its float values).  Also
  * records each map's min/max in the prep for fg_prep_summary. */
 int fg_write_outputs(fg_prep *prep, double *const *maps, const char *program_name, int threads);
+/* The same files from the rounded maps in integer thousandths
+ * (fg_job_fetch_milli): maps[m] for m < n_maps + 2, and the floating grid's
+ * values in fmap (NULL when the GPF names none).  Byte-identical output. */
+int fg_write_outputs_milli(fg_prep *prep, const int32_t *const *maps, const double *fmap, const char *program_name,
+                           int threads);
 
 /* ---------------------------------------------------------------- */
 /* The maps' consumer: AutoDock 4.2.1's map set and its trilinear    */
