@@ -1520,6 +1520,7 @@ void fg_destroy(fg_ctx *ctx)
         if (e) (void)hipEventDestroy(e);
     for (void *p : ctx->pin)
         if (p) (void)hipHostFree(p);
+    if (ctx->upPin) (void)hipHostFree(ctx->upPin);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     delete ctx;

@@ -230,9 +230,12 @@ int fg_floating_grid(fg_ctx *ctx, const fg_grid *grid, const fg_atoms *atoms, fl
 int fg_job_create(fg_ctx *ctx, const fg_job_desc *desc, fg_job **out);
 /* n jobs at once: the host-side inputs (atom records, tiles, cells, tables)
  * are built on up to `threads` host threads (<= 0: one per core, at most 16),
- * then uploaded and sized on ctx's stream in list order.  On an error no job
- * is kept (every out[i] is NULL) and the message names the first failing
- * job.  No reference counterpart (one receptor per process there). */
+ * then uploaded and sized on ctx's stream.  The n jobs share two device
+ * allocations, freed when the last of them is destroyed.  Uploads are staged
+ * in a pinned host buffer that ctx keeps (as large as the largest creation's
+ * inputs) until fg_destroy.  On an error no job is kept (every out[i] is
+ * NULL) and the message names the first failing job.  No reference
+ * counterpart (one receptor per process there). */
 int fg_job_create_many(fg_ctx *ctx, const fg_job_desc *descs, int n, int threads, fg_job **out);
 void fg_job_destroy(fg_job *job);
 /* Enqueue the computation on the context's stream; returns immediately.  The
