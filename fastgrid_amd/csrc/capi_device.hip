@@ -169,7 +169,6 @@ struct fg_job {
     int32_t *dType = nullptr, *dSorted = nullptr, *dTypeStart = nullptr;
     // electrostatic tiles
     float4 *dTiles = nullptr;
-    double *dTiles64 = nullptr;
     float *dSubBox = nullptr, *dEps32 = nullptr;
     int epsId = -1;  // the context's id of the job's dielectric table (-1: none)
     int nAtoms32 = 0, tableLen = fgk::kTableLen;
@@ -179,7 +178,9 @@ struct fg_job {
     int px = 1, pz = 4, tile = 64, bxShift = 4;
     bool shaped = false;  // the shape came from fg_autotune or fg_job_set_shape
     int plainDiel = FG_PLAIN_DIEL_TABLE;  // dielectric variant of the plain e-kernel
-    float farCut2 = 0, lastCut2 = 0, tableCut2 = 0, r2Lo = 0, r2Hi = 0, spacing100 = 0;
+    float farCut2 = 0, lastCut2 = 0, tableCut2 = 0, spacing100 = 0;
+    // e-map band of the cutoff kernel (A^2) and the candidate lists' reach (A)
+    double e2Lo = 0, e2Hi = 0, listReach = 8.0 + 1e-4;
     // plain form (FP32 throughout)
     float4 *dPlainQ = nullptr;          // (x, y, z, w) per atom, index order
     fgk::PlainAtom *dPlainAtoms = nullptr;
@@ -281,6 +282,7 @@ fgk::ListArgs listArgs(const fg_job *j)
     la.nby = j->nby;
     la.nbz = j->nbz;
     la.N = j->N;
+    la.reach = j->listReach;
     la.xyz = j->dXYZ;
     la.hbond = j->dHbond;
     la.cellStart = j->dCellStart;
@@ -330,6 +332,9 @@ fgk::CutoffArgs cutoffArgs(const fg_job *j, int doElecNear)
     a.N = j->N;
     a.fmap = j->floating ? j->dFmap : nullptr;
     for (int m = 0; m < j->M && m < 16; ++m) a.mp[m] = j->mp[size_t(m)];
+    a.e2Lo = j->e2Lo;
+    a.e2Hi = j->e2Hi;
+    a.spacing100 = j->spacing100;
     return a;
 }
 
@@ -353,14 +358,15 @@ void launchCutoffM(fg_job *j, const fgk::CutoffArgs &a, dim3 grid, dim3 block)
     else launchCutoffKH<MM, false, TY>(j, a, grid, block);
 }
 
-// Candidate lists of the job: fill, and the type sort when the typed kernel
-// runs.  The per-brick offsets come from the count pass made when the job was
-// created (they depend only on the atoms and the grid).
-int runLists(fg_job *j, bool affinity)
+// Candidate lists of the job, made once when it is created (they depend only
+// on the atoms and the grid): the fill pass at the count pass's offsets, and
+// the type-sorted copy the typed cutoff kernel reads.  Runs and batches read
+// them; the unsorted lists serve the floating grid and the untyped kernels.
+int fillLists(fg_job *j)
 {
     fgk::ListArgs la = listArgs(j);
     hipLaunchKernelGGL(fgk::k_brick_lists<true>, dim3(j->nb), dim3(fgk::kListThreads), 0, j->ctx->stream, la);
-    if (affinity && j->typed)
+    if (j->typed)
         hipLaunchKernelGGL(fgk::k_type_sort, dim3(j->nb), dim3(fgk::kListThreads), 0, j->ctx->stream, la);
     FG_HIP(hipGetLastError());
     return FG_OK;
@@ -421,7 +427,6 @@ fgk::ElecArgs elecArgs(const fg_job *j, int px, int pz, int tile, int bxShift)
     a.floorBitsNeg4 = int32_t(0u - 4u * fgk::kFloorBits);
     a.P = j->P;
     a.tiles = j->dTiles;
-    a.tiles64 = j->dTiles64;
     a.subBox = j->dSubBox;
     a.eps32 = j->dEps32;
     a.e = j->dMaps + int64_t(j->M) * j->P;
@@ -429,8 +434,6 @@ fgk::ElecArgs elecArgs(const fg_job *j, int px, int pz, int tile, int bxShift)
     a.farCut2 = j->farCut2;
     a.lastCut2 = j->lastCut2;
     a.tableCut2 = j->tableCut2;
-    a.r2Lo = j->r2Lo;
-    a.r2Hi = j->r2Hi;
     a.spacing100 = j->spacing100;
     return a;
 }
@@ -537,7 +540,6 @@ int buildElecTiles(fg_ctx *ctx, fg_job *j, const fg_job_desc &d, double gridMaxA
     j->nAtoms32 = nSub * TT;
     const size_t NT = size_t(j->nAtoms32);
     std::vector<float4> tiles(std::max<size_t>(NT, 1));
-    std::vector<double> t64(std::max<size_t>(NT * 3, 1));
     std::vector<float> box(std::max<size_t>(size_t(nSub) * 6, 1));
     const double s = j->g.spacing;
     for (size_t q = 0; q < NT; ++q) {
@@ -557,7 +559,6 @@ int buildElecTiles(fg_ctx *ctx, fg_job *j, const fg_job_desc &d, double gridMaxA
             w = 0;
         }
         tiles[q] = make_float4(float(c[0] * 100.0), float(c[1] * 100.0), float(c[2] * 100.0), float(w * 100.0));
-        for (int k = 0; k < 3; ++k) t64[3 * q + k] = c[k];
     }
     for (int t = 0; t < nSub; ++t) {
         float bl[3] = {1e30f, 1e30f, 1e30f}, bh[3] = {-1e30f, -1e30f, -1e30f};
@@ -580,8 +581,16 @@ int buildElecTiles(fg_ctx *ctx, fg_job *j, const fg_job_desc &d, double gridMaxA
     const double dmax = 900.0;  // |d'| near the boundary (<= 9 A)
     const double Ed = u * (2 * C + dmax) + 1e-9 * C;
     const double B2 = 3 * u * 6.5e5 + 2 * dmax * std::sqrt(3.0) * Ed + 3 * Ed * Ed;
-    j->r2Lo = float(6.4e5 - 4 * B2 - 1.0);
-    j->r2Hi = float(6.4e5 + 4 * B2 + 1.0);
+    // The e-map's near pairs are those whose FP32 bin is below 800, i.e.
+    // whose r2' * rsq(r2') < 800, within 2 ulp of sqrt(r2'): below
+    // 6.4e5 - 4 B2 - 1 for certain (scaled r^2), above 6.4e5 + 4 B2 + 1 not.
+    // The cutoff kernel takes the first for certain and decides the band by
+    // the far field's own FP32 arithmetic; its lists reach past the band.
+    j->e2Lo = (6.4e5 - 4 * B2 - 1.0) * 1e-4;
+    j->e2Hi = (6.4e5 + 4 * B2 + 1.0) * 1e-4;
+    if (j->e2Hi >= 64.08)  // bins up to 800 only in the cutoff kernel's tables
+        return fgc::fail(FG_E_ARG, "fg_job_create: coordinates too far from the origin (FP32 near-field band)");
+    j->listReach = std::sqrt(j->e2Hi) + 1e-4;
     // Boxes farther apart than ~8.02 A hold only far pairs: the FP32
     // coordinates and outward-rounded boxes differ from the FP64 ones by at
     // most Ed per axis (< 1e-5 A for atoms within 100 A of the origin), far
@@ -631,7 +640,6 @@ int buildElecTiles(fg_ctx *ctx, fg_job *j, const fg_job_desc &d, double gridMaxA
     }
     j->epsId = internEps(ctx, eps32.data());
     if (int rc = devUpload(j, j->dTiles, tiles.data(), tiles.size())) return rc;
-    if (int rc = devUpload(j, j->dTiles64, t64.data(), t64.size())) return rc;
     if (int rc = devUpload(j, j->dSubBox, box.data(), box.size())) return rc;
     if (int rc = devUpload(j, j->dEps32, eps32.data(), eps32.size())) return rc;
     return FG_OK;
@@ -1191,7 +1199,6 @@ struct fg_batch {
     bool pointAng = false, clamp = false, typed = true;  // typed: every job has type-sorted lists
     int sharedTableLen = 0, maxTableLen = 0;
     int cutItems = 0, elecItems = 0, elecWGs = 0, farItems = 0;
-    fgk::ListArgs *dList = nullptr;
     fgk::CutoffArgs *dCut = nullptr;
     fgk::ElecArgs *dElec = nullptr;
     int32_t *dCutBase = nullptr, *dElecBase = nullptr, *dFarBase = nullptr, *dCounters = nullptr;
@@ -1210,7 +1217,7 @@ void destroyBatch(fg_batch *b)
     }
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    for (void *p : {static_cast<void *>(b->dList), static_cast<void *>(b->dCut), static_cast<void *>(b->dElec),
+    for (void *p : {static_cast<void *>(b->dCut), static_cast<void *>(b->dElec),
                     static_cast<void *>(b->dCutBase), static_cast<void *>(b->dElecBase),
                     static_cast<void *>(b->dFarBase), static_cast<void *>(b->dCounters)})
         if (p) (void)hipFree(p);
@@ -1320,14 +1327,7 @@ int enqueueBatch(fg_batch *b, hipEvent_t mid, bool launch = true)
         if (launch) FG_HIP(hipEventRecord(ctx->e1, side));
     }
     if (b->aff || b->near || b->fmap) {
-        if (launch) {
-            // list offsets are fixed at job creation: fill (and sort) only
-            hipLaunchKernelGGL(fgk::k_brick_lists_batch<true>, dim3(b->cutItems), dim3(fgk::kListThreads), 0, s,
-                               b->dList, b->dCutBase, n);
-            if (b->typed)
-                hipLaunchKernelGGL(fgk::k_type_sort_batch, dim3(b->cutItems), dim3(fgk::kListThreads), 0, s, b->dList,
-                                   b->dCutBase, n);
-        }
+        // every job's lists (and type-sorted lists) were made at its creation
         if (b->fmap && launch)
             hipLaunchKernelGGL(fgk::k_fmap_batch, dim3(b->cutItems), dim3(fgk::kCutThreads), 0, s, b->dCut, b->dCutBase,
                                n);
@@ -1411,12 +1411,10 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
         for (const fg_job *j : b->jobs) same = same && j->epsId >= 0 && j->epsId == b->jobs[0]->epsId;
         b->sharedTableLen = same ? b->maxTableLen : 0;
     }
-    std::vector<fgk::ListArgs> la;
     std::vector<fgk::CutoffArgs> ca;
     std::vector<fgk::ElecArgs> ea;
     std::vector<int32_t> cutBase(1, 0), elecBase(1, 0), farBase(1, 0);
     for (fg_job *j : b->jobs) {
-        la.push_back(listArgs(j));
         ca.push_back(cutoffArgs(j, b->near ? 1 : 0));
         if (!(b->typed && b->aff)) ca.back().brickList = j->dList;
         // two points per lane along x need a 2D brick (the shape does not
@@ -1438,7 +1436,6 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
     b->cutItems = cutBase.back();
     b->elecItems = elecBase.back();
     b->farItems = farBase.back();
-    if (int rc = batchUpload(b->dList, la)) return rc;
     if (int rc = batchUpload(b->dCut, ca)) return rc;
     if (int rc = batchUpload(b->dElec, ea)) return rc;
     if (int rc = batchUpload(b->dCutBase, cutBase)) return rc;
@@ -1801,6 +1798,11 @@ int jobsFinishDevice(fg_ctx *ctx, fg_job *const *jobs, int n, int threads)
         placeArena(jobs[i], static_cast<char *>(block2.get()) + off[size_t(i)]);
         jobs[i]->owned.push_back(block2);
     }
+    // the lists themselves, once
+    for (int i = 0; i < n; ++i)
+        if (int rc = fillLists(jobs[i])) return fail(rc);
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(fgc::fail(FG_E_DEVICE, std::string("fg_job_create: ") + hipGetErrorString(e)));
     for (int i = 0; i < n; ++i) ctx->jobs.push_back(jobs[i]);
     return FG_OK;
 }
@@ -1908,15 +1910,12 @@ static int enqueueRun(fg_ctx *ctx, fg_job *j, unsigned what, int precision, RunM
         FG_HIP(hipEventRecord(ctx->join, ctx->side));
     }
     if (precision == FG_PREC_PLAIN) {
-        if ((aff && j->hasGeom) || fmap)
-            if (int rc = runLists(j, false)) return rc;
         if (aff && j->hasGeom)
             if (int rc = runPlainAffinity(j)) return rc;
         if (fmap)
             if (int rc = launchFmap(j)) return rc;
     } else if (aff || near || fmap) {
         const bool affMaps = aff && j->hasGeom;
-        if (int rc = runLists(j, affMaps)) return rc;
         if (aff || near)
             if (int rc = launchCutoff(j, affMaps, near ? 1 : 0)) return rc;
         if (fmap)
@@ -2213,7 +2212,6 @@ int fg_job_cutoff_pairs(fg_ctx *ctx, fg_job *j, int64_t *pairs)
     if (!ctx || !j || !pairs) return fgc::fail(FG_E_ARG, "fg_job_cutoff_pairs: NULL argument");
     FG_HIP(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
-    if (int rc = runLists(j, false)) return rc;
     fgk::CutoffArgs ca = cutoffArgs(j, 0);
     ca.brickList = j->dList;  // the unsorted lists
     unsigned long long *d = nullptr;

@@ -41,6 +41,34 @@ __device__ __forceinline__ int binOf(double r)
 }
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// The FP32 table bin of the far field: r2 * rsq(r2) + (2^23 - 1/2) lands in
+// [2^23, 2^24), where the FP32 spacing is 1, so its bit pattern is 2^23's
+// plus floor(100 r) (coordinates in units of 0.01 A).
+constexpr float kFloorBias = 8388607.5f;       // 2^23 - 1/2
+constexpr unsigned kFloorBits = 0x4B000000u;   // bit pattern of 2^23
+// Split of the compensated e-map: a pair whose FP32 bin is below kNearBins
+// (100 r < 800) is the cutoff kernel's, in FP64; every other pair is the
+// far-field kernel's, in FP32.  The far-field kernel drops its near pairs by
+// reading zeros from table bins 0..kNearBins-1.
+constexpr int kNearBins = 800;
+
+// The far-field kernel's FP32 squared distance (scaled by 1e4) between atom
+// (x, y, z, scaled by 100) and a grid point: the operations in its order.
+__device__ __forceinline__ float farR2(float ax, float ay, float az, float gx, float gy, float gz)
+{
+    const float dy = ay - gy;
+    const float dy2 = dy * dy;
+    const float dx = ax - gx;
+    const float dxy2 = __builtin_fmaf(dx, dx, dy2);
+    const float dz = az - gz;
+    return __builtin_fmaf(dz, dz, dxy2);
+}
+// The biased bin bits of a far-field pair with FP32 squared distance r2 >= 1.
+__device__ __forceinline__ unsigned farBinBits(float r2)
+{
+    return __float_as_uint(__builtin_fmaf(r2, __builtin_amdgcn_rsqf(r2), kFloorBias));
+}
+
 // Read-only data of a launch seen through the constant address space:
 // wave-uniform loads of it become scalar loads, also in persistent loops
 // whose stores (elsewhere) would otherwise make the compiler reload it per
@@ -66,8 +94,10 @@ __device__ __forceinline__ T loadConst(const T *p)
 
 // ------------------------------------------------------------------------
 // Candidate lists: for each brick of the cutoff kernel, every atom whose
-// distance to the brick's box is <= 8 A (+1e-4 A).  Order: coarse cells in
-// z, y, x order, ascending atom index within a cell; the same on every run.
+// distance to the brick's box is <= reach (8 A plus the band in which the
+// cutoff kernel decides e-map pairs by the far-field kernel's FP32 bin), and
+// the DS/D1 atoms within 8 A + 1e-4 A.  Order: coarse cells in z, y, x
+// order, ascending atom index within a cell; the same on every run.
 // ------------------------------------------------------------------------
 constexpr int kListThreads = 256;
 constexpr int kMaxCellsPerBrick = 1024;
@@ -76,6 +106,7 @@ struct ListArgs {
     GridGeom g;
     CellGrid cg;
     int32_t nbx, nby, nbz, N;
+    double reach;             // list reach (A), >= 8 + 1e-4
     const double *xyz;        // [N][3]
     const int32_t *hbond;     // [N]
     const int32_t *cellStart; // [ncells+1]
@@ -111,7 +142,7 @@ __device__ __forceinline__ void brickList(const ListArgs &a, int b, ListLds &S)
     const int n[3] = {a.g.nx, a.g.ny, a.g.nz}, d2[3] = {a.g.dx2, a.g.dy2, a.g.dz2};
     double lo[3], hi[3];
     int c0[3], c1[3];
-    const double reach = 8.0 + 1e-4;
+    const double reach = a.reach, reachH = 8.0 + 1e-4;
     for (int k = 0; k < 3; ++k) {
         int i1 = ib[k] + ext[k] - 1;
         if (i1 > n[k] - 1) i1 = n[k] - 1;
@@ -143,7 +174,7 @@ __device__ __forceinline__ void brickList(const ListArgs &a, int b, ListLds &S)
     __syncthreads();
     const int m = ncell < kMaxCellsPerBrick ? ncell : kMaxCellsPerBrick;
     const int total = cellOff[m];
-    const double reach2 = reach * reach;
+    const double reach2 = reach * reach, reachH2 = reachH * reachH;
     int cnt = 0, cntH = 0;
     for (int v0 = 0; v0 < total; v0 += kListThreads) {
         const int v = v0 + tid;
@@ -165,7 +196,7 @@ __device__ __forceinline__ void brickList(const ListArgs &a, int b, ListLds &S)
                 dd += t * t;
             }
             keep = dd <= reach2;
-            keepH = keep && (a.hbond[ia] == HB_DS || a.hbond[ia] == HB_D1);
+            keepH = dd <= reachH2 && (a.hbond[ia] == HB_DS || a.hbond[ia] == HB_D1);
         }
         if (!kFill) {
             cnt += keep;
@@ -340,7 +371,8 @@ __device__ __noinline__ AngTerms pointAngular(const CutAtom &A, bool isClosest, 
     return t;
 }
 
-// FP64 1/sqrt(x) for the cutoff kernel's pairs (1e-24 <= x <= 64): the
+// FP64 1/sqrt(x) for the cutoff kernel's pairs (1e-24 <= x <= e2Hi, a hair
+// above 64; the same binade parity as the measured range): the
 // hardware estimate plus one Newton step.  v_rsq_f64 measured at most
 // 2^-24.2 relative on 4 M inputs in [1e-6, 64] on MI355X; from any estimate
 // within 2^-22 one step leaves at most 1.5 e^2 plus rounding, below 4e-13,
@@ -429,7 +461,8 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
     const int iy = by * kCutBY + (w >> 1) * 4 + ((l >> 2) & 3);
     const int iz = bz * kCutBZ + (l >> 4);
     const bool valid = ix < a.g.nx && iy < a.g.ny && iz < a.g.nz;
-    const double gx = (ix - a.g.dx2) * a.g.spacing;
+    // a lane past the grid sits 1e30 A away: every pair test rejects it
+    const double gx = valid ? (ix - a.g.dx2) * a.g.spacing : 1e30;
     const double gy = (iy - a.g.dy2) * a.g.spacing;
     const double gz = (iz - a.g.dz2) * a.g.spacing;
     const int64_t p = valid ? (int64_t(iz) * a.g.ny + iy) * a.g.nx + ix : 0;
@@ -480,6 +513,7 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
         hmax[k] = -999999;
     }
     double dmap = 0, enear = 0, sv = 0, sq = 0;
+    const double cut2 = a.doElecNear ? a.e2Hi : 64.0;  // e2Hi > 64
 
     // candidates: one run over the brick's list, or one run per receptor
     // type over the type-sorted list with that type's rows in LDS
@@ -519,7 +553,33 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                 const double dx = A.x - gx, dy = A.y - gy, dz = A.z - gz;
                 const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
                 double rSq = dx2 + dy2 + dz2;
-                if (!valid || rSq > 64.0) continue;
+                // affinity and d maps: r^2 <= 64; e-map: the pairs the
+                // far-field kernel leaves (FP32 bin < kNearBins), certain
+                // up to e2Lo (< 64) and decided by its own FP32 arithmetic up
+                // to e2Hi.  One test on the path most candidates take; the
+                // band's pairs (rare) branch off.
+                if (rSq > cut2) continue;
+                double eW = A.w;  // the pair's e-map weight (0: the far field's pair)
+                if (a.doElecNear && rSq > a.e2Lo) {
+                    const float s100 = a.spacing100;
+                    const float r2 = farR2(float(A.x * 100.0), float(A.y * 100.0), float(A.z * 100.0),
+                                           float(ix - a.g.dx2) * s100, float(iy - a.g.dy2) * s100,
+                                           float(iz - a.g.dz2) * s100);
+                    const bool farOwns = farBinBits(r2) >= kFloorBits + unsigned(kNearBins);
+                    if (rSq > 64.0) {
+                        // past the affinity cutoff: the e term alone, if the far field left it
+                        if (!farOwns) {
+                            const double y = rsqrtFast(rSq);
+                            bool edgeE;
+                            int iE = binGuarded(rSq * y, edgeE);
+                            if (edgeE) iE = binOf(sqrt(dx2 + (dy2 + dz2)));
+                            enear += A.w * __builtin_fmin(y, 2.0) * (kTyped ? L.eps[iE] : a.eps[iE]);
+                        }
+                        continue;
+                    }
+                    if (farOwns) eW = 0.0;
+                }
+                // from here r^2 <= 64
                 rSq = __builtin_fmax(rSq, 1.0E-24);  // rSq >= 0, never NaN
                 // guarded fast geometry (see binGuarded)
                 const double yinv = rsqrtFast(rSq);
@@ -535,7 +595,7 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                     idxE = binOf(sqrt(dx2 + (dy2 + dz2)));
                 }
                 // r <= 8 A: both bins are <= 800
-                if (a.doElecNear) enear += A.w * __builtin_fmin(yinv, 2.0) * (kTyped ? L.eps[idxE] : a.eps[idxE]);
+                if (a.doElecNear) enear += eW * __builtin_fmin(yinv, 2.0) * (kTyped ? L.eps[idxE] : a.eps[idxE]);
                 if (!kAffinity) continue;
                 // atom fields are the same in every lane: keep them in scalar registers
                 const int type = uniform(A.type), ahb = uniform(A.hbond), fl = uniform(A.flags);
@@ -756,14 +816,14 @@ __global__ __launch_bounds__(kListThreads) void k_type_sort(ListArgs a)
 // dielectric table directly and w is stored as 100*w.
 // ------------------------------------------------------------------------
 
-constexpr float kFloorBias = 8388607.5f;       // 2^23 - 1/2
-constexpr unsigned kFloorBits = 0x4B000000u;   // bit pattern of 2^23
 constexpr int kFold = 64;  // atoms per FP32 partial sum of the compensated form
 
-// Loads the dielectric table into LDS (all lanes of the workgroup).
-__device__ __forceinline__ void loadEps(float *eps, const float *src, int n)
+// Loads the dielectric table into LDS (all lanes of the workgroup), with
+// bins below `zeroBelow` read as 0: the compensated form's near pairs (bins
+// below kNearBins, which its far loops never reach) then add nothing.
+__device__ __forceinline__ void loadEps(float *eps, const float *src, int n, int zeroBelow)
 {
-    for (int i = threadIdx.x; i < n; i += kElecThreads) eps[i] = src[i];
+    for (int i = threadIdx.x; i < n; i += kElecThreads) eps[i] = i < zeroBelow ? 0.f : src[i];
 }
 
 // One brick of the electrostatic map: 256 lanes in x, y; each lane owns PX
@@ -852,10 +912,9 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
     const unsigned biasBytes = unsigned(a.floorBitsNeg4);
     const char *epsBytes = reinterpret_cast<const char *>(eps);
 
-    // atom tiles, their boxes and FP64 coordinates: read-only, uniform loads
+    // atom tiles and their boxes: read-only, uniform loads
     const FG_CONST float *tiles = constView(reinterpret_cast<const float *>(a.tiles));
     const FG_CONST float *subBox = constView(a.subBox);
-    const FG_CONST double *tiles64 = constView(a.tiles64);
 
     float hi[NP], lo[NP], part[NP];
 #pragma unroll
@@ -864,8 +923,14 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
     // far field: every lane's pair is beyond 8 A (box tests at 8.02 A).
     // Clamp: bins past the table's end are clamped to its last entry (the
     // clamped kernel variant, for tiles that may reach beyond the table).
-    auto farAtoms = [&](int c0, int c1, auto clampTag) {
+    // Near (sub-tiles within 8.02 A of the wave): some pairs have FP32 bins
+    // below kNearBins; the compensated form's table reads 0 there, so they
+    // add nothing (they are the cutoff kernel's).  r^2 is raised to 1
+    // (r = 0.01 A, bin 0) so that an atom on a grid point gives 1/r = 1, not
+    // inf; every pair the far field keeps has r^2 > 6.4e5, unchanged.
+    auto farAtoms = [&](int c0, int c1, auto clampTag, auto nearTag) {
         constexpr bool kClampHere = kClamp && decltype(clampTag)::value;
+        constexpr bool kNear = decltype(nearTag)::value;
 #pragma unroll 2
         for (int j = c0; j < c1; ++j) {
             const float4 at = make_float4(tiles[4 * j], tiles[4 * j + 1], tiles[4 * j + 2], tiles[4 * j + 3]);
@@ -883,7 +948,8 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
             unsigned bin[NP];
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
-                const float r2 = __builtin_fmaf(dz[q % PZ], dz[q % PZ], dxy2[q / PZ]);
+                float r2 = __builtin_fmaf(dz[q % PZ], dz[q % PZ], dxy2[q / PZ]);
+                if (kNear) r2 = __builtin_fmaxf(r2, 1.0f);
                 ri[q] = __builtin_amdgcn_rsqf(r2);
                 if (kComp) {
                     // floor(100 r) in one operation: r2 * ri + (2^23 - 1/2)
@@ -909,44 +975,6 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
                 // compensated form are all beyond 8 A
                 const float rc = kComp ? ri[q] : fminf(ri[q], 0.02f);
                 part[q] = __builtin_fmaf(at.w * rc, ev[q], part[q]);
-            }
-        }
-    };
-    // near field: pairs within 8 A (they belong to k_cutoff) are excluded
-    auto nearAtoms = [&](int c0, int c1) {
-        for (int j = c0; j < c1; ++j) {
-            const float4 at = make_float4(tiles[4 * j], tiles[4 * j + 1], tiles[4 * j + 2], tiles[4 * j + 3]);
-            const FG_CONST double *T64 = tiles64 + int64_t(j) * 3;
-            const float dy = at.y - gy;
-            const float dy2 = dy * dy;
-            float dxy2[PX], dz[PZ];
-#pragma unroll
-            for (int i = 0; i < PX; ++i) {
-                const float dx = at.x - gx[i];
-                dxy2[i] = __builtin_fmaf(dx, dx, dy2);
-            }
-#pragma unroll
-            for (int k = 0; k < PZ; ++k) dz[k] = at.z - gz[k];
-#pragma unroll
-            for (int q = 0; q < NP; ++q) {
-                const float r2 = __builtin_fmaf(dz[q % PZ], dz[q % PZ], dxy2[q / PZ]);
-                bool include = r2 > a.r2Hi;
-                const bool amb = !include && r2 >= a.r2Lo;
-                if (amb) {
-                    // exact FP64 predicate of the cutoff kernel: r^2 <= 64 is near
-                    const double ddx = T64[0] - (cx[q / PZ] - a.g.dx2) * a.g.spacing;
-                    const double ddy = T64[1] - (cy - a.g.dy2) * a.g.spacing;
-                    const double ddz = T64[2] - (cz[q % PZ] - a.g.dz2) * a.g.spacing;
-                    include = !(ddx * ddx + ddy * ddy + ddz * ddz <= 64.0);
-                }
-                // every lane does the arithmetic; an excluded pair has
-                // its r^2 replaced by +inf, so 1/r is 0 and its biased
-                // bin (a NaN pattern) is clamped into the table
-                const float ri = __builtin_amdgcn_rsqf(include ? r2 : __builtin_inff());
-                const unsigned bin =
-                    min(__float_as_uint(__builtin_fmaf(include ? r2 : __builtin_inff(), ri, kFloorBias)), clampBits);
-                const float ev = *reinterpret_cast<const float *>(epsBytes + ((bin << 2) + biasBytes));
-                part[q] = __builtin_fmaf(at.w * ri, ev, part[q]);
             }
         }
     };
@@ -1019,17 +1047,17 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
                 lastBinAtoms(c0, c1);
             } else if (kind == 1) {
                 // the FP32 form (kComp false) clamps as its variant says
-                if (kComp) farAtoms(c0, c1, kNo());
-                else farAtoms(c0, c1, kYes());
+                if (kComp) farAtoms(c0, c1, kNo(), kNo());
+                else farAtoms(c0, c1, kYes(), kNo());
             } else if (kind == 2) {
-                farAtoms(c0, c1, kYes());
+                farAtoms(c0, c1, kYes(), kNo());
             } else {
                 // near tile: decide again per wave and sub-tile (a fold chunk
                 // holds whole sub-tiles)
                 for (int q = c0; q < c1; q += kSubTile) {
                     const int q1 = min(q + kSubTile, c1);
-                    if (uniform(farFrom(q, wlo, whi))) farAtoms(q, q1, kYes());
-                    else nearAtoms(q, q1);
+                    if (uniform(farFrom(q, wlo, whi))) farAtoms(q, q1, kYes(), kNo());
+                    else farAtoms(q, q1, kYes(), kYes());
                 }
             }
             if (kComp && ((c1 & (kFold - 1)) == 0 || c1 == a.nAtoms)) {
@@ -1180,7 +1208,7 @@ __global__ __launch_bounds__(kElecThreads) void k_elec(ElecArgs a, int *counter)
 {
     extern __shared__ float eps[];
     __shared__ int slot;
-    loadEps(eps, a.eps32, a.tableLen);
+    loadEps(eps, a.eps32, a.tableLen, kComp ? kNearBins : 0);
     const int total = a.nbx * a.nby * a.nbz;
     for (;;) {
         const int item = nextItem(counter, &slot);  // its barrier also orders the table load
@@ -1200,31 +1228,19 @@ __global__ __launch_bounds__(kElecThreads) void k_elec_batch(const ElecArgs *job
     __shared__ int slot;
     const int total = base[nJobs];
     int loaded = -1;
-    if (sharedTableLen > 0) loadEps(eps, loadConst(jobs).eps32, sharedTableLen);
+    if (sharedTableLen > 0) loadEps(eps, loadConst(jobs).eps32, sharedTableLen, kComp ? kNearBins : 0);
     for (;;) {
         const int item = nextItem(counter, &slot);
         if (item >= total) break;
         const int job = uniform(jobOfItem(base, nJobs, item));
         const ElecArgs a = loadConst(jobs + job);
         if (sharedTableLen <= 0 && job != loaded) {
-            loadEps(eps, a.eps32, a.tableLen);
+            loadEps(eps, a.eps32, a.tableLen, kComp ? kNearBins : 0);
             loaded = job;
         }
         __syncthreads();  // table loaded
         elecBrick<PX, PZ, kComp, kClamp>(a, item - base[job], eps);
     }
-}
-
-// Candidate lists of every job (fill pass; the offsets were counted when the
-// jobs were created): one workgroup per (job, brick) item.
-template <bool kFill>
-__global__ __launch_bounds__(kListThreads) void k_brick_lists_batch(const ListArgs *jobs, const int32_t *base,
-                                                                   int nJobs)
-{
-    __shared__ ListLds S;
-    const int item = int(blockIdx.x);
-    const int job = uniform(jobOfItem(base, nJobs, item));
-    brickList<kFill>(loadConst(jobs + job), item - base[job], S);
 }
 
 // Cutoff maps of every job: one workgroup per (job, brick) item, items in
@@ -1250,16 +1266,6 @@ __global__ __launch_bounds__(kCutThreads) void k_fmap_batch(const CutoffArgs *jo
     if (a.fmap) fmapBrick(a, item - base[job]);
 }
 
-// Type sort of every job's lists: one workgroup per (job, brick) item.
-__global__ __launch_bounds__(kListThreads) void k_type_sort_batch(const ListArgs *jobs, const int32_t *base, int nJobs)
-{
-    __shared__ int32_t cnt[kMaxTypes], runStart[kMaxTypes], wcnt[4 * kMaxTypes];
-    const int item = int(blockIdx.x);
-    const int job = uniform(jobOfItem(base, nJobs, item));
-    const ListArgs a = loadConst(jobs + job);
-    if (!a.sorted) return;  // a job without type-sorted lists
-    typeSortBrick(a, item - base[job], cnt, runStart, wcnt);
-}
 
 // k_add_far for every job: items are (job, 256-point chunk) pairs.
 __global__ __launch_bounds__(256) void k_add_far_batch(const ElecArgs *jobs, const int32_t *base, int nJobs)

@@ -116,6 +116,11 @@ struct CutoffArgs {
     float *fmap;              // floating grid [P], or null
     double covPoint[3], covHwSqInv, covBarrier;
     MapParam mp[16];
+    // e-map pairs (compensated form): r^2 <= e2Lo are the cutoff kernel's
+    // for certain, r^2 > e2Hi the far-field kernel's; in between the far
+    // field's FP32 bin decides (kNearBins), recomputed here with spacing100
+    double e2Lo, e2Hi;
+    float spacing100;
 };
 
 struct ElecArgs {
@@ -131,7 +136,6 @@ struct ElecArgs {
     int32_t pad;
     int64_t P;
     const float4 *tiles;     // [nAtoms] (x, y, z, w) scaled by 100 (w: 100*w)
-    const double *tiles64;   // [nAtoms][3] FP64 coordinates (for the exact mask)
     const float *subBox;     // [nAtoms/kSubTile][6] lo xyz, hi xyz (scaled)
     const float *eps32;      // [tableLen]
     double *e;               // [P] electrostatic map (the FP32 form writes it)
@@ -140,7 +144,6 @@ struct ElecArgs {
     float farCut2;           // (802)^2: sub-tiles farther than 8.02 A hold only far pairs
     float lastCut2;          // sub-tiles farther than this (~81.94 A) read only the last table bin
     float tableCut2;         // sub-tiles closer than this at their farthest stay inside the table
-    float r2Lo, r2Hi;        // ambiguity band around 64e4 in scaled r^2
     float spacing100;        // 100 * spacing
 };
 
