@@ -85,6 +85,7 @@ void *scratchOf(int device)
 
 struct fg_ctx {
     int device = 0, numCUs = 0;
+    size_t ldsPerCU = 0;  // LDS bytes per CU (0: unknown)
     hipStream_t stream = nullptr;
     // The electrostatic kernels run on `side`, concurrently with the cutoff
     // kernels on `stream` (both at the default priority): fork is recorded
@@ -1199,6 +1200,7 @@ struct fg_batch {
     bool pointAng = false, clamp = false, typed = true;  // typed: every job has type-sorted lists
     int sharedTableLen = 0, maxTableLen = 0;
     int cutItems = 0, elecItems = 0, elecWGs = 0, farItems = 0;
+    size_t cutLds = 0;  // LDS of one workgroup of the batch's cutoff kernel
     fgk::CutoffArgs *dCut = nullptr;
     fgk::ElecArgs *dElec = nullptr;
     int32_t *dCutBase = nullptr, *dElecBase = nullptr, *dFarBase = nullptr, *dCounters = nullptr;
@@ -1279,11 +1281,17 @@ int launchElecBatch(fg_batch *b, bool launch, hipStream_t s)
     auto k = fgk::k_elec_batch<PX, PZ, kComp, kClamp>;
     const size_t lds = size_t(b->sharedTableLen > 0 ? b->sharedTableLen : b->maxTableLen) * sizeof(float);
     // a persistent grid would hold every slot until the queue is empty: keep
-    // one per CU for the concurrent lists and cutoff kernels
-    if (!b->elecWGs)
-        if (int rc = persistentWGs(b, k, fgk::kElecThreads, lds, b->elecItems, b->elecWGs,
-                                   (b->aff || b->near || b->fmap) ? 1 : 0))
+    // one per CU, and the LDS of one cutoff workgroup, for the concurrent
+    // cutoff kernel
+    if (!b->elecWGs) {
+        const bool shared = b->aff || b->near || b->fmap;
+        if (int rc = persistentWGs(b, k, fgk::kElecThreads, lds, b->elecItems, b->elecWGs, shared ? 1 : 0))
             return rc;
+        if (shared && b->cutLds > 0 && b->ctx->ldsPerCU > b->cutLds) {
+            const int fit = int((b->ctx->ldsPerCU - b->cutLds) / (lds + 256));
+            b->elecWGs = std::max(1, std::min(b->elecWGs, std::max(fit, 1) * b->ctx->numCUs));
+        }
+    }
     if (!launch) return FG_OK;
     hipLaunchKernelGGL(k, dim3(b->elecWGs), dim3(fgk::kElecThreads), lds, s, b->dElec, b->dElecBase,
                        int(b->jobs.size()), b->dCounters + 1, b->sharedTableLen);
@@ -1405,6 +1413,11 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
     }
     b->MM = maxM <= 4 ? 4 : b->KH > 4 ? 16 : maxM <= 6 ? 6 : maxM <= 8 ? 8 : 16;
     if (b->aff && maxM == 0) b->aff = false;
+    b->cutLds = !b->aff ? sizeof(fgk::CutoffLds<1, false, false>)
+                : b->MM == 4 ? (b->typed ? sizeof(fgk::CutoffLds<4, true, true>) : sizeof(fgk::CutoffLds<4, true, false>))
+                : b->MM == 6 ? (b->typed ? sizeof(fgk::CutoffLds<6, true, true>) : sizeof(fgk::CutoffLds<6, true, false>))
+                : b->MM == 8 ? (b->typed ? sizeof(fgk::CutoffLds<8, true, true>) : sizeof(fgk::CutoffLds<8, true, false>))
+                             : (b->typed ? sizeof(fgk::CutoffLds<16, true, true>) : sizeof(fgk::CutoffLds<16, true, false>));
     // one LDS table for all jobs when their dielectric tables are bit-equal
     {
         bool same = true;
@@ -1495,6 +1508,10 @@ int fg_create(int device, fg_ctx **out)
     if (hipDeviceGetAttribute(&c->numCUs, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
         c->numCUs <= 0)
         c->numCUs = 256;
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) == hipSuccess &&
+        lds > 0)
+        c->ldsPerCU = size_t(lds);
     if (int rc = scratchSetUp(device)) {
         fg_destroy(c);
         return rc;

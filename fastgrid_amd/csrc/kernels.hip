@@ -832,7 +832,7 @@ __device__ __forceinline__ void loadEps(float *eps, const float *src, int n, int
 // shape, and the compensated form folds its FP32 partial into the two-float
 // sum after every kFold atoms, so every shape gives the same bits.
 template <int PX, int PZ, bool kComp, bool kClamp>
-__device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float *eps)
+__device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float *eps, int zStride = PZ)
 {
     constexpr int NP = PX * PZ;
     const int tid = threadIdx.x;
@@ -848,7 +848,7 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
         // at the end of each z-slab
         const int c0 = (b % a.nbx) * kElecThreads, c = c0 + tid;
         const int cl = min(c0 + kElecThreads, a.g.nx * a.g.ny) - 1;  // last column of the brick
-        iz0 = (b / a.nbx) * PZ;
+        iz0 = (b / a.nbx) * zStride;
         ix = c % a.g.nx;
         iy = c / a.g.nx;  // >= ny past the plane: the lane stores nothing
         cx[0] = min(c, cl) % a.g.nx;
@@ -863,7 +863,7 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
         const int bx = b % a.nbx, by = (b / a.nbx) % a.nby, bz = b / (a.nbx * a.nby);
         ix0 = bx * bxl * PX;
         iy0 = by * byl;
-        iz0 = bz * PZ;
+        iz0 = bz * zStride;
         ix = ix0 + PX * (tid & (bxl - 1));
         iy = iy0 + (tid >> sh);
         bx1 = min(ix0 + bxl * PX - 1, a.g.nx - 1);
@@ -1203,6 +1203,22 @@ __global__ __launch_bounds__(256) void k_fill_i32(int32_t *p, int n, int32_t v)
 // workgroup that starts late, its slot held by a kernel running
 // concurrently, takes fewer bricks instead of stretching the launch by a
 // whole brick's time; which workgroup computes a brick changes no bits.
+// Brick b at the kernel's shape, except the last z-brick of a column with
+// at most PZ / 2 planes left: it runs with PZ / 2 points per lane, on the
+// same z origin (same bits: a pair's arithmetic does not depend on the shape).
+template <int PX, int PZ, bool kComp, bool kClamp>
+__device__ __forceinline__ void elecItem(const ElecArgs &a, int b, const float *eps)
+{
+    if (PZ > 4) {
+        const int bz = a.bxShift == 0 ? b / a.nbx : b / (a.nbx * a.nby);
+        if (a.g.nz - bz * PZ <= PZ / 2) {
+            elecBrick<PX, (PZ > 4 ? PZ / 2 : PZ), kComp, kClamp>(a, b, eps, PZ);
+            return;
+        }
+    }
+    elecBrick<PX, PZ, kComp, kClamp>(a, b, eps);
+}
+
 template <int PX, int PZ, bool kComp, bool kClamp>
 __global__ __launch_bounds__(kElecThreads) void k_elec(ElecArgs a, int *counter)
 {
@@ -1213,7 +1229,7 @@ __global__ __launch_bounds__(kElecThreads) void k_elec(ElecArgs a, int *counter)
     for (;;) {
         const int item = nextItem(counter, &slot);  // its barrier also orders the table load
         if (item >= total) break;
-        elecBrick<PX, PZ, kComp, kClamp>(a, item, eps);
+        elecItem<PX, PZ, kComp, kClamp>(a, item, eps);
     }
 }
 
@@ -1239,7 +1255,7 @@ __global__ __launch_bounds__(kElecThreads) void k_elec_batch(const ElecArgs *job
             loaded = job;
         }
         __syncthreads();  // table loaded
-        elecBrick<PX, PZ, kComp, kClamp>(a, item - base[job], eps);
+        elecItem<PX, PZ, kComp, kClamp>(a, item - base[job], eps);
     }
 }
 
