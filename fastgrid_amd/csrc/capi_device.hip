@@ -164,6 +164,7 @@ struct fg_job {
     fgk::CellGrid cg{};
     int nbx = 0, nby = 0, nbz = 0, nb = 0;
     int32_t *dCounts = nullptr, *dStart = nullptr, *dHStart = nullptr, *dList = nullptr, *dHList = nullptr;
+    int64_t listTotal = 0;  // candidates over all bricks (the count pass's total)
     // type-sorted candidate lists for the typed cutoff kernel (pair-table rows in LDS)
     bool typed = false;
     int32_t addOnlyClasses = 0;  // bit h: H-bond class h only adds (see CutoffArgs)
@@ -180,6 +181,7 @@ struct fg_job {
     bool shaped = false;  // the shape came from fg_autotune or fg_job_set_shape
     int plainDiel = FG_PLAIN_DIEL_TABLE;  // dielectric variant of the plain e-kernel
     float farCut2 = 0, lastCut2 = 0, tableCut2 = 0, spacing100 = 0;
+    double elecEd = 0;  // bound on the FP32 coordinates' error per axis (scaled units)
     // e-map band of the cutoff kernel (A^2) and the candidate lists' reach (A)
     double e2Lo = 0, e2Hi = 0, listReach = 8.0 + 1e-4;
     // plain form (FP32 throughout)
@@ -359,15 +361,18 @@ void launchCutoffM(fg_job *j, const fgk::CutoffArgs &a, dim3 grid, dim3 block)
     else launchCutoffKH<MM, false, TY>(j, a, grid, block);
 }
 
-// Candidate lists of the job, made once when it is created (they depend only
-// on the atoms and the grid): the fill pass at the count pass's offsets, and
-// the type-sorted copy the typed cutoff kernel reads.  Runs and batches read
-// them; the unsorted lists serve the floating grid and the untyped kernels.
-int fillLists(fg_job *j)
+// Candidate lists of the job: the fill pass at the offsets of the count pass
+// (made when the job is created, to size the lists), and the type-sorted copy
+// the typed cutoff kernel reads; the unsorted lists serve the floating grid
+// and the untyped kernels.  Every run makes them again before its cutoff
+// phase, as the reference builds its cutoff grid in every calculateGridmaps
+// (FG/CalculateGridmaps.cpp:332-415); job creation makes them once too, for
+// the queries that read them outside a run (fg_job_cutoff_pairs).
+int fillLists(fg_job *j, bool sorted = true)
 {
     fgk::ListArgs la = listArgs(j);
     hipLaunchKernelGGL(fgk::k_brick_lists<true>, dim3(j->nb), dim3(fgk::kListThreads), 0, j->ctx->stream, la);
-    if (j->typed)
+    if (sorted && j->typed)
         hipLaunchKernelGGL(fgk::k_type_sort, dim3(j->nb), dim3(fgk::kListThreads), 0, j->ctx->stream, la);
     FG_HIP(hipGetLastError());
     return FG_OK;
@@ -416,14 +421,23 @@ fgk::ElecArgs elecArgs(const fg_job *j, int px, int pz, int tile, int bxShift)
     a.tile = tile;
     a.bxShift = bxShift;
     if (bxShift == 0) {
-        a.nbx = int((int64_t(j->g.nx) * j->g.ny + fgk::kElecThreads - 1) / fgk::kElecThreads);
-        a.nby = 1;
+        // linear items: units of px points of a row times the pz planes of a
+        // slab, 256 to an item; region 0 the full slabs, region 1 the rest
+        // (job creation bounds the grid so that these counts fit in 32 bits)
+        const int64_t T = fgk::kElecThreads;
+        a.ux = (j->g.nx + px - 1) / px;
+        a.units = a.ux * j->g.ny;
+        a.slabs = j->g.nz / pz;
+        a.itemsA = int32_t((int64_t(a.slabs) * a.units + T - 1) / T);
+        a.items = a.itemsA + (j->g.nz > a.slabs * pz ? int32_t((a.units + T - 1) / T) : 0);
+        a.nbx = a.nby = a.nbz = 0;
     } else {
         const int bw = (1 << bxShift) * px, byl = fgk::kElecThreads >> bxShift;
         a.nbx = (j->g.nx + bw - 1) / bw;
         a.nby = (j->g.ny + byl - 1) / byl;
+        a.nbz = (j->g.nz + pz - 1) / pz;
+        a.items = a.nbx * a.nby * a.nbz;
     }
-    a.nbz = (j->g.nz + pz - 1) / pz;
     a.clampMax = j->tableLen - 1;
     a.floorBitsNeg4 = int32_t(0u - 4u * fgk::kFloorBits);
     a.P = j->P;
@@ -439,6 +453,21 @@ fgk::ElecArgs elecArgs(const fg_job *j, int px, int pz, int tile, int bxShift)
     return a;
 }
 
+// Workgroup slots per CU that the persistent e-kernel leaves to the cutoff
+// phase running beside it, from an estimate of the two phases' VALU
+// wave-instructions: about 45 per candidate and wave of the cutoff kernel (4
+// waves visit every candidate of their brick) and 7.5 per 64 pairs of the
+// e-kernel.  The cutoff kernels wait on memory far more than the e-kernel,
+// so they need more slots than their share of the instructions: on the
+// 256-pocket batch (cutoff ~21 % of the instructions) two slots gave 65.2 ms
+// per batch against 66.7 ms with one and 89 ms with three; on the 50,000-atom
+// job (~12 %) one slot lets both phases end together.
+int cutoffReserve(double candidates, double elecPairs)
+{
+    const double cut = 180.0 * candidates, elec = 7.5 / 64.0 * elecPairs;
+    return cut > 0.15 * (cut + elec) ? 2 : 1;
+}
+
 // The persistent e-kernel: the workgroups its occupancy admits per CU (found
 // when the job is made, so that nothing here queries the device), less
 // `reserve` per CU left to the kernels running concurrently (a persistent
@@ -449,8 +478,7 @@ int launchElecK(fg_job *j, const fgk::ElecArgs &a, size_t lds, int reserve, hipS
 {
     auto k = fgk::k_elec<PX, PZ, kComp, kClamp>;
     const int perCU = j->elecPerCU[PX - 1][PZ == 8][kComp];
-    const int bricks = a.nbx * a.nby * a.nbz;
-    const int wgs = std::max(1, std::min(bricks, std::max(perCU - reserve, 1) * j->ctx->numCUs));
+    const int wgs = std::max(1, std::min(a.items, std::max(perCU - reserve, 1) * j->ctx->numCUs));
     hipLaunchKernelGGL(fgk::k_fill_i32, dim3(1), dim3(256), 0, s, j->dCounter, 1, 0);
     hipLaunchKernelGGL(k, dim3(unsigned(wgs)), dim3(fgk::kElecThreads), lds, s, a, j->dCounter);
     return FG_OK;
@@ -466,8 +494,7 @@ int launchElecV(fg_job *j, const fgk::ElecArgs &a, bool comp, size_t lds, int re
                     : launchElecK<PX, PZ, false, false>(j, a, lds, reserve, s);
 }
 
-// k_elec of shape (px, pz, tile, bxShift) on stream s; px 2 needs a 2D brick
-// (bxShift > 0).
+// k_elec of shape (px, pz, tile, bxShift) on stream s.
 int launchElecShape(fg_job *j, int px, int pz, int tile, int bxShift, bool comp, int reserve, hipStream_t s)
 {
     const fgk::ElecArgs a = elecArgs(j, px, pz, tile, bxShift);
@@ -515,6 +542,15 @@ int internEps(fg_ctx *ctx, const float *t)
     if (ctx->epsTables.size() >= kMaxEpsTables) return -1;
     ctx->epsTables.emplace_back(t, t + fgk::kTableLen);
     return int(ctx->epsTables.size() - 1);
+}
+
+// Squared reach (scaled units) within which a sub-tile's farthest corner
+// never indexes past a table of tableLen entries (0: no clamp, no limit),
+// for FP32 coordinates within Ed of the FP64 ones.
+float tableCut2Of(int tableLen, double Ed)
+{
+    const double tableCut = tableLen > 0 ? (tableLen - 3.0) - 4 * Ed : 1e18;
+    return float(tableCut * tableCut);
 }
 
 int buildElecTiles(fg_ctx *ctx, fg_job *j, const fg_job_desc &d, double gridMaxAbs)
@@ -597,6 +633,7 @@ int buildElecTiles(fg_ctx *ctx, fg_job *j, const fg_job_desc &d, double gridMaxA
     // most Ed per axis (< 1e-5 A for atoms within 100 A of the origin), far
     // inside the 0.02 A margin.  A pair beyond the guard band gets the same
     // FP32 arithmetic in the near and far loops, so the cut moves no bits.
+    j->elecEd = Ed;
     const double farCut = 802.0 + 4 * Ed;
     j->farCut2 = float(farCut * farCut);
     j->spacing100 = float(s * 100.0);
@@ -632,8 +669,7 @@ int buildElecTiles(fg_ctx *ctx, fg_job *j, const fg_job_desc &d, double gridMaxA
     // past the table (the clamped variant skips its clamp for them).
     const double lastCut = (fgk::kTableLen + 2.0) + 4 * Ed;
     j->lastCut2 = float(lastCut * lastCut);
-    const double tableCut = j->clamp ? (j->tableLen - 3.0) - 4 * Ed : 1e18;
-    j->tableCut2 = float(tableCut * tableCut);
+    j->tableCut2 = tableCut2Of(j->clamp ? j->tableLen : 0, Ed);
     eps32.resize(size_t(j->tableLen));
     for (int k = 0; k < j->tableLen; ++k) {
         double v = d.dist_dep_diel ? eps[std::min(k, fgk::kTableLen - 1)] : 1.0;
@@ -687,7 +723,8 @@ int buildCutoff(fg_job *j, const fg_job_desc &d)
                     j->pointAng = true;
                 }
             }
-            c.flags = dis | (kind << 1);
+            // the kernel reads the atom's uniform fields with one scalar move
+            c.flags = dis | (kind << 1) | (c.hbond << 3) | (c.type << 8);
         }
         atoms[size_t(i)] = c;
         xyz[size_t(3 * i)] = c.x;
@@ -989,9 +1026,8 @@ int elecOccupancy(fg_job *j)
 }
 
 // Share of the device's workgroup slots one launch of shape (px, pz, tile,
-// bxShift) keeps busy on average: bricks / (slots x rounds).  A launch of
-// fewer bricks than slots leaves slots idle, so its time measures one
-// brick's latency rather than throughput.
+// bxShift) keeps busy.  A launch of fewer items than slots leaves slots
+// idle, so its time measures one item's latency rather than throughput.
 int elecFill(const fg_job *j, int px, int pz, int bxShift, bool comp, double &fill)
 {
     const fgk::ElecArgs a = elecArgs(j, px, pz, 32, bxShift);
@@ -1000,11 +1036,12 @@ int elecFill(const fg_job *j, int px, int pz, int bxShift, bool comp, double &fi
     int perCU = 0;
     FG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k, fgk::kElecThreads,
                                                          size_t(j->tableLen) * sizeof(float)));
-    // the persistent grid a batch launches: one slot per CU is left to its
-    // concurrent cutoff phase (launchElecBatch)
+    // the persistent grid of the timed launch (timeShape: one slot per CU
+    // left free, as beside a cutoff phase); its workgroups take items until
+    // none are left, so a launch of fewer items than workgroups keeps only
+    // that share of them busy, and a larger one all of them
     const double slots = double(std::max(perCU - 1, 1)) * double(j->ctx->numCUs);
-    const double bricks = double(a.nbx) * a.nby * a.nbz;
-    fill = bricks / (slots * std::ceil(bricks / slots));
+    fill = std::min(1.0, double(a.items) / slots);
     return FG_OK;
 }
 
@@ -1064,8 +1101,6 @@ int tuneJobs(fg_ctx *ctx, fg_job *const *jobs, int n, bool comp)
             for (int b = 0; b < 3; ++b)
                 for (int d = 0; d < 4; ++d) {
                     c.t[v][b][d] = kNone;
-                    // two points along x need a 2D brick
-                    if (kVar[v][0] == 2 && kShifts[d] == 0) continue;
                     if (int rc = timeShape(rep, kVar[v][0], kVar[v][1], kTiles[b], kShifts[d], comp, c.t[v][b][d]))
                         return rc;
                     if (n > 1) {
@@ -1202,6 +1237,7 @@ struct fg_batch {
     int cutItems = 0, elecItems = 0, elecWGs = 0, farItems = 0;
     size_t cutLds = 0;  // LDS of one workgroup of the batch's cutoff kernel
     fgk::CutoffArgs *dCut = nullptr;
+    fgk::ListArgs *dList = nullptr;  // the candidate-list passes of every job
     fgk::ElecArgs *dElec = nullptr;
     int32_t *dCutBase = nullptr, *dElecBase = nullptr, *dFarBase = nullptr, *dCounters = nullptr;
     hipGraph_t graph = nullptr;
@@ -1219,7 +1255,7 @@ void destroyBatch(fg_batch *b)
     }
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    for (void *p : {static_cast<void *>(b->dCut), static_cast<void *>(b->dElec),
+    for (void *p : {static_cast<void *>(b->dCut), static_cast<void *>(b->dList), static_cast<void *>(b->dElec),
                     static_cast<void *>(b->dCutBase), static_cast<void *>(b->dElecBase),
                     static_cast<void *>(b->dFarBase), static_cast<void *>(b->dCounters)})
         if (p) (void)hipFree(p);
@@ -1281,11 +1317,17 @@ int launchElecBatch(fg_batch *b, bool launch, hipStream_t s)
     auto k = fgk::k_elec_batch<PX, PZ, kComp, kClamp>;
     const size_t lds = size_t(b->sharedTableLen > 0 ? b->sharedTableLen : b->maxTableLen) * sizeof(float);
     // a persistent grid would hold every slot until the queue is empty: keep
-    // one per CU, and the LDS of one cutoff workgroup, for the concurrent
-    // cutoff kernel
+    // one or two per CU (cutoffReserve), and the LDS of one cutoff
+    // workgroup, for the concurrent cutoff kernel
     if (!b->elecWGs) {
         const bool shared = b->aff || b->near || b->fmap;
-        if (int rc = persistentWGs(b, k, fgk::kElecThreads, lds, b->elecItems, b->elecWGs, shared ? 1 : 0))
+        double cand = 0, pairs = 0;
+        for (const fg_job *j : b->jobs) {
+            cand += double(j->listTotal);
+            pairs += double(j->N) * double(j->P);
+        }
+        if (int rc = persistentWGs(b, k, fgk::kElecThreads, lds, b->elecItems, b->elecWGs,
+                                   shared ? cutoffReserve(cand, pairs) : 0))
             return rc;
         if (shared && b->cutLds > 0 && b->ctx->ldsPerCU > b->cutLds) {
             const int fit = int((b->ctx->ldsPerCU - b->cutLds) / (lds + 256));
@@ -1335,7 +1377,15 @@ int enqueueBatch(fg_batch *b, hipEvent_t mid, bool launch = true)
         if (launch) FG_HIP(hipEventRecord(ctx->e1, side));
     }
     if (b->aff || b->near || b->fmap) {
-        // every job's lists (and type-sorted lists) were made at its creation
+        // every job's candidate lists, then their type-sorted copies (for the
+        // typed kernel), before the cutoff phase, as a single run makes them
+        if (launch) {
+            hipLaunchKernelGGL(fgk::k_brick_lists_batch<true>, dim3(b->cutItems), dim3(fgk::kListThreads), 0, s,
+                               b->dList, b->dCutBase, n);
+            if (b->typed && b->aff)
+                hipLaunchKernelGGL(fgk::k_type_sort_batch, dim3(b->cutItems), dim3(fgk::kListThreads), 0, s, b->dList,
+                                   b->dCutBase, n);
+        }
         if (b->fmap && launch)
             hipLaunchKernelGGL(fgk::k_fmap_batch, dim3(b->cutItems), dim3(fgk::kCutThreads), 0, s, b->dCut, b->dCutBase,
                                n);
@@ -1411,6 +1461,16 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
         b->px = v >= 2 ? 2 : 1;
         b->pz = v & 1 ? 8 : 4;
     }
+    // Tables longer than kTableLen entries (jobs whose distances reach past
+    // 81.9 A) would cost the persistent e-kernel workgroups per CU, and the
+    // longest table sets the LDS of every workgroup: the batch runs the
+    // clamped variant on the first kTableLen entries instead (entries past
+    // 8191 repeat entry 8191, so the clamped index reads the same value).
+    const bool clampAll = b->elec && b->maxTableLen > fgk::kTableLen;
+    if (clampAll) {
+        b->clamp = true;
+        b->maxTableLen = fgk::kTableLen;
+    }
     b->MM = maxM <= 4 ? 4 : b->KH > 4 ? 16 : maxM <= 6 ? 6 : maxM <= 8 ? 8 : 16;
     if (b->aff && maxM == 0) b->aff = false;
     b->cutLds = !b->aff ? sizeof(fgk::CutoffLds<1, false, false>)
@@ -1425,20 +1485,25 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
         b->sharedTableLen = same ? b->maxTableLen : 0;
     }
     std::vector<fgk::CutoffArgs> ca;
+    std::vector<fgk::ListArgs> la;
     std::vector<fgk::ElecArgs> ea;
     std::vector<int32_t> cutBase(1, 0), elecBase(1, 0), farBase(1, 0);
     for (fg_job *j : b->jobs) {
         ca.push_back(cutoffArgs(j, b->near ? 1 : 0));
         if (!(b->typed && b->aff)) ca.back().brickList = j->dList;
-        // two points per lane along x need a 2D brick (the shape does not
-        // change the results)
+        la.push_back(listArgs(j));
+        if (!(b->typed && b->aff)) la.back().sorted = nullptr;  // no type sort
         int jpx, jpz, jtile, jshift;
         shapeOf(j, jpx, jpz, jtile, jshift);
-        const int shift = b->px == 2 && jshift == 0 ? 4 : jshift;
-        fgk::ElecArgs e = elecArgs(j, b->px, b->pz, jtile, shift);
+        fgk::ElecArgs e = elecArgs(j, b->px, b->pz, jtile, jshift);
+        if (clampAll) {
+            e.tableLen = std::min(e.tableLen, int32_t(fgk::kTableLen));
+            e.clampMax = e.tableLen - 1;
+            e.tableCut2 = tableCut2Of(e.tableLen, j->elecEd);
+        }
         if (b->sharedTableLen > 0) e.clampMax = b->sharedTableLen - 1;
         ea.push_back(e);
-        const int64_t c = int64_t(cutBase.back()) + j->nb, el = int64_t(elecBase.back()) + int64_t(e.nbx) * e.nby * e.nbz;
+        const int64_t c = int64_t(cutBase.back()) + j->nb, el = int64_t(elecBase.back()) + int64_t(e.items);
         if (c > INT32_MAX || el > INT32_MAX) return fgc::fail(FG_E_ARG, "fg_batch_create: too many work items");
         cutBase.push_back(int32_t(c));
         elecBase.push_back(int32_t(el));
@@ -1450,6 +1515,7 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
     b->elecItems = elecBase.back();
     b->farItems = farBase.back();
     if (int rc = batchUpload(b->dCut, ca)) return rc;
+    if (int rc = batchUpload(b->dList, la)) return rc;
     if (int rc = batchUpload(b->dElec, ea)) return rc;
     if (int rc = batchUpload(b->dCutBase, cutBase)) return rc;
     if (int rc = batchUpload(b->dElecBase, elecBase)) return rc;
@@ -1563,8 +1629,10 @@ int jobBuildHost(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
     j->N = d->atoms.n;
     j->M = d->n_maps;
     j->T = d->tables.n_types;
-    // the kernels index the pair table with 32-bit offsets
-    if (j->T < 0 || int64_t(j->T) * fgk::kPairBins * std::max(j->M, 1) >= (int64_t(1) << 31)) {
+    // the kernels index the pair table with 32-bit offsets, formed by a
+    // 24-bit multiply of (type x 801 + bin) by the map count
+    if (j->T < 0 || int64_t(j->T) * fgk::kPairBins * std::max(j->M, 1) >= (int64_t(1) << 31) ||
+        int64_t(j->T) * fgk::kPairBins >= (int64_t(1) << 24)) {
         delete j;
         return fgc::fail(FG_E_ARG, "fg_job_create: pair table too large (n_types x 801 x n_maps >= 2^31)");
     }
@@ -1801,6 +1869,7 @@ int jobsFinishDevice(fg_ctx *ctx, fg_job *const *jobs, int n, int threads)
         off[size_t(i)] = total2;
         const int32_t t0 = tot[size_t(2 * i)], t1 = tot[size_t(2 * i + 1)];
         if (t0 < 0 || t1 < 0) return fail(fgc::fail(FG_E_DEVICE, "fg_job_create: candidate count overflow"));
+        j->listTotal = t0;
         devAlloc(j, j->dList, size_t(t0));
         devAlloc(j, j->dHList, size_t(t1));
         if (j->typed) {
@@ -1920,19 +1989,27 @@ static int enqueueRun(fg_ctx *ctx, fg_job *j, unsigned what, int precision, RunM
         FG_HIP(rec(ctx->e0, ctx->side));
         if (precision == FG_PREC_PLAIN) {
             if (int rc = runPlainElec(j, ctx->side)) return rc;
-        } else if (int rc = launchElec(j, precision, (aff || near || fmap) ? 1 : 0, ctx->side)) {
+        } else if (int rc = launchElec(j, precision,
+                                       (aff || near || fmap)
+                                           ? cutoffReserve(double(j->listTotal), double(j->N) * double(j->P))
+                                           : 0,
+                                       ctx->side)) {
             return rc;
         }
         FG_HIP(rec(ctx->e1, ctx->side));
         FG_HIP(hipEventRecord(ctx->join, ctx->side));
     }
     if (precision == FG_PREC_PLAIN) {
+        // the plain cutoff kernel reads the unsorted lists
+        if ((aff && j->hasGeom) || fmap)
+            if (int rc = fillLists(j, false)) return rc;
         if (aff && j->hasGeom)
             if (int rc = runPlainAffinity(j)) return rc;
         if (fmap)
             if (int rc = launchFmap(j)) return rc;
     } else if (aff || near || fmap) {
         const bool affMaps = aff && j->hasGeom;
+        if (int rc = fillLists(j)) return rc;
         if (aff || near)
             if (int rc = launchCutoff(j, affMaps, near ? 1 : 0)) return rc;
         if (fmap)
@@ -2288,10 +2365,8 @@ int fg_job_set_shape(fg_job *job, const fg_shape *sh)
     if (!job || !sh) return fgc::fail(FG_E_ARG, "fg_job_set_shape: NULL argument");
     const int shift = sh->bx == 8 ? 3 : sh->bx == 16 ? 4 : sh->bx == 32 ? 5 : sh->bx == 256 ? 0 : -1;
     if ((sh->px != 1 && sh->px != 2) || (sh->pz != 4 && sh->pz != 8) ||
-        (sh->tile != 32 && sh->tile != 64 && sh->tile != 128) || shift < 0 || (sh->px == 2 && shift == 0))
-        return fgc::fail(FG_E_ARG,
-                         "fg_job_set_shape: px must be 1 or 2, pz 4 or 8, tile 32/64/128, bx 8/16/32/256 (256 only "
-                         "with px 1)");
+        (sh->tile != 32 && sh->tile != 64 && sh->tile != 128) || shift < 0)
+        return fgc::fail(FG_E_ARG, "fg_job_set_shape: px must be 1 or 2, pz 4 or 8, tile 32/64/128, bx 8/16/32/256");
     job->px = sh->px;
     job->pz = sh->pz;
     job->tile = sh->tile;

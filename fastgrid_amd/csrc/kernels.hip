@@ -154,27 +154,60 @@ __device__ __forceinline__ void brickList(const ListArgs &a, int b, ListLds &S)
         if (c1[k] > a.cg.nc[k] - 1) c1[k] = a.cg.nc[k] - 1;
     }
     const int ncx = c1[0] - c0[0] + 1, ncy = c1[1] - c0[1] + 1, ncz = c1[2] - c0[2] + 1;
+    // at most 8 cells per axis (the cell edge is at least 4 A and the brick's)
     const int ncell = (ncx > 0 && ncy > 0 && ncz > 0) ? ncx * ncy * ncz : 0;
-    // cell table and prefix of atom counts (one thread; at most kMaxCellsPerBrick cells)
+    const double reach2 = reach * reach, reachH2 = reachH * reachH;
+    // cell table and prefix of atom counts, cells in z, y, x order: thread
+    // t takes cells 4t .. 4t+3; a cell whose box lies wholly beyond the reach
+    // of the brick box counts no atoms (atoms lie inside their cell's box;
+    // 1e-6 A of slack covers the rounding of the cell index)
+    constexpr int kCellsPerThread = kMaxCellsPerBrick / kListThreads;
+    int cnt4[kCellsPerThread], mine = 0;
+#pragma unroll
+    for (int q = 0; q < kCellsPerThread; ++q) {
+        const int m = kCellsPerThread * tid + q;
+        int n = 0;
+        if (m < ncell) {
+            const int ci[3] = {c0[0] + m % ncx, c0[1] + (m / ncx) % ncy, c0[2] + m / (ncx * ncy)};
+            const int c = (ci[2] * a.cg.nc[1] + ci[1]) * a.cg.nc[0] + ci[0];
+            double dd = 0;
+            for (int k = 0; k < 3; ++k) {
+                const double clo = a.cg.origin[k] + ci[k] * a.cg.size, chi = clo + a.cg.size;
+                const double t = fmax(fmax(clo - hi[k], lo[k] - chi), 0.0);
+                dd += t * t;
+            }
+            const double r = reach + 1e-6;
+            if (dd <= r * r) n = a.cellStart[c + 1] - a.cellStart[c];
+            cellId[m] = c;
+        }
+        cnt4[q] = n;
+        mine += n;
+    }
+    scanBuf[tid] = mine;
     if (tid == 0) {
-        int off = 0, m = 0;
-        for (int cz = 0; cz < ncz; ++cz)
-            for (int cy = 0; cy < ncy; ++cy)
-                for (int cx = 0; cx < ncx && m < kMaxCellsPerBrick; ++cx) {
-                    int c = ((c0[2] + cz) * a.cg.nc[1] + (c0[1] + cy)) * a.cg.nc[0] + (c0[0] + cx);
-                    cellId[m] = c;
-                    cellOff[m] = off;
-                    off += a.cellStart[c + 1] - a.cellStart[c];
-                    ++m;
-                }
-        cellOff[m] = off;
         base = kFill ? a.start[b] : 0;
         baseH = kFill ? a.hStart[b] : 0;
     }
     __syncthreads();
-    const int m = ncell < kMaxCellsPerBrick ? ncell : kMaxCellsPerBrick;
+    for (int s = 1; s < kListThreads; s <<= 1) {  // inclusive scan of the per-thread counts
+        const int x = tid >= s ? scanBuf[tid - s] : 0;
+        __syncthreads();
+        scanBuf[tid] += x;
+        __syncthreads();
+    }
+    {
+        int off = scanBuf[tid] - mine;
+#pragma unroll
+        for (int q = 0; q < kCellsPerThread; ++q) {
+            const int m = kCellsPerThread * tid + q;
+            if (m < ncell) cellOff[m] = off;
+            off += cnt4[q];
+        }
+        if (tid == kListThreads - 1) cellOff[ncell] = scanBuf[tid];
+    }
+    __syncthreads();
+    const int m = ncell;
     const int total = cellOff[m];
-    const double reach2 = reach * reach, reachH2 = reachH * reachH;
     int cnt = 0, cntH = 0;
     for (int v0 = 0; v0 < total; v0 += kListThreads) {
         const int v = v0 + tid;
@@ -392,9 +425,8 @@ __device__ __forceinline__ double rsqrtFast(double x)
 __device__ __forceinline__ int binGuarded(double r, bool &edge)
 {
     const double t = r * 100.0;
-    const int k = int(t);
-    edge = __builtin_fabs((t - double(k)) - 0.5) > 0.5 - 1e-7;
-    return k;
+    edge = __builtin_fabs(__builtin_amdgcn_fract(t) - 0.5) > 0.5 - 1e-7;
+    return int(t);
 }
 
 // Initial value of a covalent map at (gx, gy, gz) (FG/main.cpp:39-46).
@@ -573,7 +605,8 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                             bool edgeE;
                             int iE = binGuarded(rSq * y, edgeE);
                             if (edgeE) iE = binOf(sqrt(dx2 + (dy2 + dz2)));
-                            enear += A.w * __builtin_fmin(y, 2.0) * (kTyped ? L.eps[iE] : a.eps[iE]);
+                            enear = __builtin_fma(A.w * __builtin_fmin(y, 2.0), kTyped ? L.eps[iE] : a.eps[iE],
+                                                  enear);
                         }
                         continue;
                     }
@@ -595,11 +628,12 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                     idxE = binOf(sqrt(dx2 + (dy2 + dz2)));
                 }
                 // r <= 8 A: both bins are <= 800
-                if (a.doElecNear) enear += eW * __builtin_fmin(yinv, 2.0) * (kTyped ? L.eps[idxE] : a.eps[idxE]);
+                if (a.doElecNear)
+                    enear = __builtin_fma(eW * __builtin_fmin(yinv, 2.0), kTyped ? L.eps[idxE] : a.eps[idxE], enear);
                 if (!kAffinity) continue;
                 // atom fields are the same in every lane: keep them in scalar registers
-                const int type = uniform(A.type), ahb = uniform(A.hbond), fl = uniform(A.flags);
-                const int dis = fl & kFlagDisorder, kind = fl >> 1;
+                const int fl = uniform(A.flags);
+                const int dis = fl & kFlagDisorder, kind = (fl >> 1) & 3, ahb = (fl >> 3) & 7, type = fl >> 8;
                 if (type == a.hydrogen && dis) continue;
                 double racc = 1, rdon = 1, Hramp = 1;
                 if (kind == kAngConst) {
@@ -617,7 +651,7 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                 }
                 const double S = a.desolv[idx];
                 // 32-bit offsets (job creation bounds the table below 2^31 entries)
-                const unsigned rowOff = unsigned((type * kPairBins + idx) * M);
+                const unsigned rowOff = __umul24(unsigned(type * kPairBins + idx), unsigned(M));
                 const double *grow = a.pair + rowOff;
                 const double *lrow = kTyped ? &L.row[idx * ldsMaps<MM>()] : grow;
                 // map m's entry: LDS for the first MM/2 maps of the typed kernel
@@ -638,7 +672,7 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                     for (int m = 0; m < MM; ++m) e[m] += FG_ROW(m);
                     sv = __builtin_fma(vol, S, sv);
                     sq = __builtin_fma(solQ, S, sq);
-                    dmap += A.qvol * S;
+                    dmap = __builtin_fma(A.qvol, S, dmap);
                     continue;
                 }
 #pragma unroll
@@ -670,7 +704,7 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                 }
                 sv = __builtin_fma(vol, S, sv);
                 sq = __builtin_fma(solQ, S, sq);
-                dmap += A.qvol * S;
+                dmap = __builtin_fma(A.qvol, S, dmap);
 #undef FG_ROW
             }
         }
@@ -826,52 +860,70 @@ __device__ __forceinline__ void loadEps(float *eps, const float *src, int n, int
     for (int i = threadIdx.x; i < n; i += kElecThreads) eps[i] = i < zeroBelow ? 0.f : src[i];
 }
 
-// One brick of the electrostatic map: 256 lanes in x, y; each lane owns PX
-// consecutive points along x times PZ points along z.  `eps` is the table in
-// LDS.  The FP32 arithmetic of a (point, atom) pair does not depend on the
-// shape, and the compensated form folds its FP32 partial into the two-float
-// sum after every kFold atoms, so every shape gives the same bits.
+// One work item of the electrostatic map: 256 lanes, each owning PX points
+// along x times PZ consecutive points along z.  2D bricks (bxShift > 0):
+// 1 << bxShift lanes along x, the rest along y, a lane's x points 1 <<
+// bxShift apart.  Linear items (bxShift 0): 256 consecutive units of PX
+// adjacent points (ElecArgs); `region` 1 is the partial
+// slab at the top of the grid, whose units stand zStride planes above the
+// last full slab.  `eps` is the table in LDS.  The FP32 arithmetic of a
+// (point, atom) pair does not depend on the shape, and the compensated form
+// folds its FP32 partial into the two-float sum after every kFold atoms, so
+// every shape gives the same bits.
 template <int PX, int PZ, bool kComp, bool kClamp>
-__device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float *eps, int zStride = PZ)
+__device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float *eps, int zStride = PZ,
+                                          int region = 0)
 {
     constexpr int NP = PX * PZ;
     const int tid = threadIdx.x;
     const int sh = a.bxShift;
-    int ix, iy, ix0, iy0, bx1, by1, iz0;
-    // (ix, iy): the lane's column, for the stores; (cx, cy, cz): the points
-    // it computes.  Padding lanes past the grid compute a copy of the brick's
-    // last point on that axis (and store nothing), so every lane's points lie
-    // in the brick box and a far test against the box holds for all of them.
+    // (ix, iy, iz0): the lane's first point, for the stores; (cx, cy, cz):
+    // the points it computes.  A padding lane computes a copy of a real
+    // point (and stores nothing), so every lane's points lie in its wave's
+    // box of real points and the far tests against that box hold for all.
+    int ix, iy, iz0, xs = 1;  // xs: x distance between the lane's points
+    bool real = true;
     int cx[PX], cy, cz[PZ];
-    if (PX == 1 && sh == 0) {
-        // 256 consecutive columns of the x-y plane (x fastest): padding only
-        // at the end of each z-slab
-        const int c0 = (b % a.nbx) * kElecThreads, c = c0 + tid;
-        const int cl = min(c0 + kElecThreads, a.g.nx * a.g.ny) - 1;  // last column of the brick
-        iz0 = (b / a.nbx) * zStride;
-        ix = c % a.g.nx;
-        iy = c / a.g.nx;  // >= ny past the plane: the lane stores nothing
-        cx[0] = min(c, cl) % a.g.nx;
-        cy = min(c, cl) / a.g.nx;
-        iy0 = c0 / a.g.nx;
-        by1 = cl / a.g.nx;
-        ix0 = iy0 == by1 ? c0 % a.g.nx : 0;
-        bx1 = iy0 == by1 ? cl % a.g.nx : a.g.nx - 1;
+    if (sh == 0) {
+        int u, nU, slab0;
+        if (region == 0) {
+            u = b * kElecThreads + tid;
+            nU = a.slabs * a.units;
+            slab0 = 0;
+        } else {
+            u = (b - a.itemsA) * kElecThreads + tid;
+            nU = a.units;
+            slab0 = a.slabs;
+        }
+        real = u < nU;
+        if (!real) u = nU - 1;
+        const int s = slab0 + u / a.units, w = u - (u / a.units) * a.units;
+        iy = w / a.ux;
+        ix = PX * (w - iy * a.ux);
+        iz0 = s * zStride;
+#pragma unroll
+        for (int i = 0; i < PX; ++i) cx[i] = min(ix + i, a.g.nx - 1);
+        cy = iy;
     } else {
-        // bxl lanes along x, each owning PX consecutive points
+        // a lane's PX points are bxl apart along x, so that each table read
+        // of a wave (one point per lane) covers a compact bxl x (64 / bxl)
+        // patch: nearby points read nearby bins, fewer LDS bank conflicts
         const int bxl = 1 << sh, byl = kElecThreads >> sh;
         const int bx = b % a.nbx, by = (b / a.nbx) % a.nby, bz = b / (a.nbx * a.nby);
-        ix0 = bx * bxl * PX;
-        iy0 = by * byl;
+        const int ix0 = bx * bxl * PX, iy0 = by * byl;
         iz0 = bz * zStride;
-        ix = ix0 + PX * (tid & (bxl - 1));
+        ix = ix0 + (tid & (bxl - 1));
         iy = iy0 + (tid >> sh);
-        bx1 = min(ix0 + bxl * PX - 1, a.g.nx - 1);
-        by1 = min(iy0 + byl - 1, a.g.ny - 1);
+        xs = bxl;
+        real = iy < a.g.ny && ix < a.g.nx;
+        // past the grid: the brick's last point on that axis
+        const int bx1 = min(ix0 + bxl * PX - 1, a.g.nx - 1), by1 = min(iy0 + byl - 1, a.g.ny - 1);
 #pragma unroll
-        for (int i = 0; i < PX; ++i) cx[i] = min(ix + i, bx1);
+        for (int i = 0; i < PX; ++i) cx[i] = min(ix + i * bxl, bx1);
         cy = min(iy, by1);
     }
+    // a wave of padding lanes only has nothing to do (no barrier follows)
+    if (!__ballot(real)) return;
     const int bz1 = min(iz0 + PZ - 1, a.g.nz - 1);
 #pragma unroll
     for (int k = 0; k < PZ; ++k) cz[k] = min(iz0 + k, bz1);
@@ -882,28 +934,25 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
     for (int i = 0; i < PX; ++i) gx[i] = float(cx[i] - a.g.dx2) * s100;
 #pragma unroll
     for (int k = 0; k < PZ; ++k) gz[k] = float(cz[k] - a.g.dz2) * s100;
-    // brick box (scaled), clamped to the grid
-    const float blo[3] = {float(ix0 - a.g.dx2) * s100, float(iy0 - a.g.dy2) * s100, float(iz0 - a.g.dz2) * s100};
-    const float bhi[3] = {float(bx1 - a.g.dx2) * s100, float(by1 - a.g.dy2) * s100, float(bz1 - a.g.dz2) * s100};
-    // the wave's own box (its 64 lanes' points): near tiles are tested again
-    // per wave and sub-tile, so a wave far from a sub-tile skips the
-    // per-pair exclusion of the near loop
+    // the wave's box (its 64 lanes' points): tiles and sub-tiles are tested
+    // against it, so a wave far from a sub-tile skips the per-pair
+    // exclusion of the near loop
     float wlo[3], whi[3];
     {
-        float xl = gx[0], xh = gx[PX - 1], yl = gy, yh = gy;
+        float v[6] = {gx[0], gy, gz[0], gx[PX - 1], gy, gz[PZ - 1]};
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
-            xl = fminf(xl, __shfl_xor(xl, o));
-            xh = fmaxf(xh, __shfl_xor(xh, o));
-            yl = fminf(yl, __shfl_xor(yl, o));
-            yh = fmaxf(yh, __shfl_xor(yh, o));
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                v[k] = fminf(v[k], __shfl_xor(v[k], o));
+                v[3 + k] = fmaxf(v[3 + k], __shfl_xor(v[3 + k], o));
+            }
         }
-        wlo[0] = __int_as_float(uniform(__float_as_int(xl)));
-        whi[0] = __int_as_float(uniform(__float_as_int(xh)));
-        wlo[1] = __int_as_float(uniform(__float_as_int(yl)));
-        whi[1] = __int_as_float(uniform(__float_as_int(yh)));
-        wlo[2] = gz[0];
-        whi[2] = gz[PZ - 1];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            wlo[k] = __int_as_float(uniform(__float_as_int(v[k])));
+            whi[k] = __int_as_float(uniform(__float_as_int(v[3 + k])));
+        }
     }
     const unsigned clampMax = unsigned(a.clampMax);
     const unsigned clampBits = kFloorBits + clampMax;
@@ -920,6 +969,21 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
 #pragma unroll
     for (int q = 0; q < NP; ++q) hi[q] = lo[q] = part[q] = 0.f;
 
+    // Atoms come kGroup at a time in one scalar load: scalar loads may
+    // return out of order, so a wave that uses one waits for every LDS read
+    // it has in flight, once per group.  Groups of 2 (8 points per lane) or
+    // 4 (16 points, whose per-atom work is twice as long); ranges are
+    // multiples of kSubTile atoms.
+    constexpr int kGroup = NP >= 16 ? 4 : 2;
+    auto forAtoms = [&](int c0, int c1, auto &&fn) {
+        for (int j = c0; j < c1; j += kGroup) {
+            float v[4 * kGroup];
+#pragma unroll
+            for (int i = 0; i < 4 * kGroup; ++i) v[i] = tiles[4 * j + i];
+#pragma unroll
+            for (int t = 0; t < kGroup; ++t) fn(make_float4(v[4 * t], v[4 * t + 1], v[4 * t + 2], v[4 * t + 3]));
+        }
+    };
     // far field: every lane's pair is beyond 8 A (box tests at 8.02 A).
     // Clamp: bins past the table's end are clamped to its last entry (the
     // clamped kernel variant, for tiles that may reach beyond the table).
@@ -931,9 +995,7 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
     auto farAtoms = [&](int c0, int c1, auto clampTag, auto nearTag) {
         constexpr bool kClampHere = kClamp && decltype(clampTag)::value;
         constexpr bool kNear = decltype(nearTag)::value;
-#pragma unroll 2
-        for (int j = c0; j < c1; ++j) {
-            const float4 at = make_float4(tiles[4 * j], tiles[4 * j + 1], tiles[4 * j + 2], tiles[4 * j + 3]);
+        forAtoms(c0, c1, [&](const float4 at) {
             const float dy = at.y - gy;
             const float dy2 = dy * dy;
             float dxy2[PX], dz[PZ];
@@ -976,16 +1038,14 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
                 const float rc = kComp ? ri[q] : fminf(ri[q], 0.02f);
                 part[q] = __builtin_fmaf(at.w * rc, ev[q], part[q]);
             }
-        }
+        });
     };
     // beyond the last table bin: every lane's pair has 100 r > 8192, where
     // the table holds its last entry, so the bin, its address and the read
     // drop out; the rest is the far loop's arithmetic, so the bits are the same
     const float tLast = eps[kTableLen - 1];
     auto lastBinAtoms = [&](int c0, int c1) {
-#pragma unroll 2
-        for (int j = c0; j < c1; ++j) {
-            const float4 at = make_float4(tiles[4 * j], tiles[4 * j + 1], tiles[4 * j + 2], tiles[4 * j + 3]);
+        forAtoms(c0, c1, [&](const float4 at) {
             const float dy = at.y - gy;
             const float dy2 = dy * dy;
             float dxy2[PX], dz[PZ];
@@ -1001,7 +1061,7 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
                 const float r2 = __builtin_fmaf(dz[q % PZ], dz[q % PZ], dxy2[q / PZ]);
                 part[q] = __builtin_fmaf(at.w * __builtin_amdgcn_rsqf(r2), tLast, part[q]);
             }
-        }
+        });
     };
     // squared shortest and longest distances between sub-tile q0's box and
     // the box (lo, hi)
@@ -1026,7 +1086,7 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
 
     for (int j0 = 0; j0 < a.nAtoms; j0 += a.tile) {
         const int j1 = min(j0 + a.tile, a.nAtoms);
-        // the tile's class against the brick, from its sub-tile boxes: near
+        // the tile's class against the wave, from its sub-tile boxes: near
         // (0), far (1), far but possibly past the table's end (2, clamped
         // variant only), past the last table bin (3)
         int kind = 1;
@@ -1034,7 +1094,7 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
             bool far = true, last = true, inTable = true;
             for (int q = j0; q < j1; q += kSubTile) {
                 float dmin2, dmax2;
-                boxDist2(q, blo, bhi, dmin2, dmax2);
+                boxDist2(q, wlo, whi, dmin2, dmax2);
                 far = far && dmin2 > a.farCut2;
                 last = last && dmin2 > a.lastCut2;
                 inTable = inTable && dmax2 < a.tableCut2;
@@ -1052,8 +1112,8 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
             } else if (kind == 2) {
                 farAtoms(c0, c1, kYes(), kNo());
             } else {
-                // near tile: decide again per wave and sub-tile (a fold chunk
-                // holds whole sub-tiles)
+                // near tile: decide per sub-tile (a fold chunk holds whole
+                // sub-tiles)
                 for (int q = c0; q < c1; q += kSubTile) {
                     const int q1 = min(q + kSubTile, c1);
                     if (uniform(farFrom(q, wlo, whi))) farAtoms(q, q1, kYes(), kNo());
@@ -1074,16 +1134,16 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
             }
         }
     }
-    if (iy >= a.g.ny) return;
+    if (!real) return;
 #pragma unroll
     for (int i = 0; i < PX; ++i) {
-        if (ix + i >= a.g.nx) break;
+        if (ix + i * xs >= a.g.nx) break;
 #pragma unroll
         for (int k = 0; k < PZ; ++k) {
             const int iz = iz0 + k;
             if (iz >= a.g.nz) break;
             const int q = i * PZ + k;
-            const int64_t p = (int64_t(iz) * a.g.ny + iy) * a.g.nx + ix + i;
+            const int64_t p = (int64_t(iz) * a.g.ny + iy) * a.g.nx + ix + i * xs;
             if (kComp) a.far[p] = double(hi[q]) + double(lo[q]);
             else a.e[p] = double(part[q]);
         }
@@ -1203,29 +1263,43 @@ __global__ __launch_bounds__(256) void k_fill_i32(int32_t *p, int n, int32_t v)
 // workgroup that starts late, its slot held by a kernel running
 // concurrently, takes fewer bricks instead of stretching the launch by a
 // whole brick's time; which workgroup computes a brick changes no bits.
-// Brick b at the kernel's shape, except the last z-brick of a column with
-// at most PZ / 2 planes left: it runs with PZ / 2 points per lane, on the
-// same z origin (same bits: a pair's arithmetic does not depend on the shape).
+// Item b at the kernel's shape, except items whose slab has at most PZ / 2
+// planes left (the top z-brick of a 2D column, region 1 of linear items):
+// they run with PZ / 2 points per lane on the same z origin (same bits: a
+// pair's arithmetic does not depend on the shape).
 template <int PX, int PZ, bool kComp, bool kClamp>
 __device__ __forceinline__ void elecItem(const ElecArgs &a, int b, const float *eps)
 {
-    if (PZ > 4) {
-        const int bz = a.bxShift == 0 ? b / a.nbx : b / (a.nbx * a.nby);
-        if (a.g.nz - bz * PZ <= PZ / 2) {
-            elecBrick<PX, (PZ > 4 ? PZ / 2 : PZ), kComp, kClamp>(a, b, eps, PZ);
-            return;
+    constexpr int PH = PZ > 4 ? PZ / 2 : PZ;
+    if (a.bxShift == 0) {
+        if (b < a.itemsA) {
+            elecBrick<PX, PZ, kComp, kClamp>(a, b, eps, PZ, 0);
+        } else if (PZ > 4 && a.g.nz - a.slabs * PZ <= PZ / 2) {
+            elecBrick<PX, PH, kComp, kClamp>(a, b, eps, PZ, 1);
+        } else {
+            elecBrick<PX, PZ, kComp, kClamp>(a, b, eps, PZ, 1);
         }
+        return;
+    }
+    if (PZ > 4 && a.g.nz - (b / (a.nbx * a.nby)) * PZ <= PZ / 2) {
+        elecBrick<PX, PH, kComp, kClamp>(a, b, eps, PZ);
+        return;
     }
     elecBrick<PX, PZ, kComp, kClamp>(a, b, eps);
 }
 
+// Register budget of the e-kernels: 4 waves per SIMD (128 VGPRs), the
+// workgroups per CU that a table of up to 10,240 entries leaves in LDS.
+constexpr int kElecWaves = 4;
+
 template <int PX, int PZ, bool kComp, bool kClamp>
-__global__ __launch_bounds__(kElecThreads) void k_elec(ElecArgs a, int *counter)
+__global__ __launch_bounds__(kElecThreads) __attribute__((amdgpu_waves_per_eu(kElecWaves))) void k_elec(ElecArgs a,
+                                                                                                         int *counter)
 {
     extern __shared__ float eps[];
     __shared__ int slot;
     loadEps(eps, a.eps32, a.tableLen, kComp ? kNearBins : 0);
-    const int total = a.nbx * a.nby * a.nbz;
+    const int total = a.items;
     for (;;) {
         const int item = nextItem(counter, &slot);  // its barrier also orders the table load
         if (item >= total) break;
@@ -1237,8 +1311,8 @@ __global__ __launch_bounds__(kElecThreads) void k_elec(ElecArgs a, int *counter)
 // repeat entry 8191), loaded once from jobs[0] with that length; else each
 // job's table is loaded when the workgroup moves to it.
 template <int PX, int PZ, bool kComp, bool kClamp>
-__global__ __launch_bounds__(kElecThreads) void k_elec_batch(const ElecArgs *jobs, const int32_t *base, int nJobs,
-                                                             int *counter, int sharedTableLen)
+__global__ __launch_bounds__(kElecThreads) __attribute__((amdgpu_waves_per_eu(kElecWaves))) void k_elec_batch(
+    const ElecArgs *jobs, const int32_t *base, int nJobs, int *counter, int sharedTableLen)
 {
     extern __shared__ float eps[];
     __shared__ int slot;
@@ -1271,6 +1345,26 @@ __attribute__((amdgpu_waves_per_eu(cutWaves<MM, KH, kAffinity, kTyped>()))) void
     const int item = int(blockIdx.x);
     const int job = uniform(jobOfItem(base, nJobs, item));
     cutoffBrick<MM, KH, kAffinity, kPointAng, kTyped>(loadConst(jobs + job), item - base[job], L);
+}
+
+// Candidate lists (kFill) and their type-sorted copies of every job: one
+// workgroup per (job, brick) item, the items of the cutoff kernels.
+template <bool kFill>
+__global__ __launch_bounds__(kListThreads) void k_brick_lists_batch(const ListArgs *jobs, const int32_t *base, int nJobs)
+{
+    __shared__ ListLds S;
+    const int item = int(blockIdx.x);
+    const int job = uniform(jobOfItem(base, nJobs, item));
+    brickList<kFill>(loadConst(jobs + job), item - base[job], S);
+}
+
+__global__ __launch_bounds__(kListThreads) void k_type_sort_batch(const ListArgs *jobs, const int32_t *base, int nJobs)
+{
+    __shared__ int32_t cnt[kMaxTypes], runStart[kMaxTypes], wcnt[4 * kMaxTypes];
+    const int item = int(blockIdx.x);
+    const int job = uniform(jobOfItem(base, nJobs, item));
+    const ListArgs a = loadConst(jobs + job);
+    if (a.sorted) typeSortBrick(a, item - base[job], cnt, runStart, wcnt);
 }
 
 // Floating grids of every job that has one: one workgroup per (job, brick) item.

@@ -49,7 +49,7 @@ struct alignas(16) CutAtom {
     double rv[3];        // bond vector
     double rv2[3];       // second bond vector
     double ang[3];       // racc, rdon, Hramp when they do not depend on the point
-    int32_t type, hbond, rexp, flags;  // flags: kFlagDisorder | angular kind << 1
+    int32_t type, hbond, rexp, flags;  // flags: kFlagDisorder | angular kind << 1 | hbond << 3 | type << 8
 };
 static_assert(sizeof(CutAtom) == 128, "CutAtom must stay 128 bytes");
 constexpr int kFlagDisorder = 1;
@@ -128,9 +128,19 @@ struct ElecArgs {
     int32_t nAtoms;          // atoms padded to a multiple of kSubTile
     int32_t tableLen;        // entries of eps32
     int32_t tile;            // atoms per tile (a multiple of kSubTile)
-    int32_t bxShift;         // lanes along x = 1 << bxShift (8, 16 or 32); 0: the brick is 256
-                             // consecutive columns of the x-y plane (x fastest)
-    int32_t nbx, nby, nbz;   // bricks per axis (bxShift 0: column blocks per plane, 1, z-bricks)
+    int32_t bxShift;         // lanes along x = 1 << bxShift (8, 16 or 32); 0: linear items (below)
+    int32_t nbx, nby, nbz;   // 2D bricks per axis (bxShift > 0)
+    // Linear items (bxShift 0): a unit is PX x-adjacent points of one row
+    // times the PZ planes of one z-slab; units are numbered slab after slab,
+    // row after row, and an item is 256 consecutive units, so items cross
+    // rows and slabs and only the last item of a region has padding lanes.
+    // Region 0 holds the `slabs` full slabs (itemsA items), region 1 the
+    // remaining nz - slabs * PZ planes, if any.
+    int32_t ux;              // units per row: ceil(nx / PX)
+    int32_t units;           // units per slab: ux * ny
+    int32_t slabs;           // full slabs: nz / PZ
+    int32_t itemsA;          // items of region 0
+    int32_t items;           // work items of a launch (bricks or linear items)
     int32_t clampMax;        // last usable table index
     int32_t floorBitsNeg4;   // -4 * bit pattern of 2^23 (mod 2^32): the table's byte offset for biased bins
     int32_t pad;

@@ -64,8 +64,17 @@ def _full_grid(ctx, a, zs):
 
 def test_L_full_grid_and_sampled_planes(ctx, tmp_path):
     import bench
+    import fastgrid_amd as fg
     a = _workload(str(tmp_path), bench.WORKLOAD_L)
-    job, _ = _full_grid(ctx, a, (0, 63, 126))
+    job, comp = _full_grid(ctx, a, (0, 63, 126))
+    # other kernel shapes give the same bits at full size: 2D bricks with
+    # 16 points per lane, and linear items that cross rows and z-slabs
+    for shape in [(8, 64, 16, 2), (8, 128, 256, 2), (4, 32, 256, 1)]:
+        job.shape = shape
+        job.run(fg.RUN_ALL, fg.PREC_COMPENSATED)
+        got = job.fetch()
+        for m in range(a.n_maps + 2):
+            assert np.array_equal(got[m], comp[m]), (shape, m)
     job.close()
 
 
