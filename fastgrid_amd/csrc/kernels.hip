@@ -464,7 +464,7 @@ struct CutoffLds {
 
 // Waves per SIMD a cutoff kernel is register-allocated for: the workgroups
 // (one wave per SIMD each) that its LDS admits in a CU's 160 KB, at most 5
-// (96 VGPRs) for up to 8 maps and 2 min/max maps, else at most 4 (128 VGPRs;
+// (96 VGPRs) for up to 4 maps and 2 min/max maps, else at most 4 (128 VGPRs;
 // the larger loops spill at 96).  Left to itself the compiler schedules the
 // unrolled map loop for fewer waves than the LDS allows (up to 163 VGPRs
 // with 8 maps).
@@ -472,7 +472,7 @@ template <int MM, int KH, bool kAffinity, bool kTyped>
 constexpr int cutWaves()
 {
     constexpr int lds = int((160 * 1024) / sizeof(CutoffLds<MM, kAffinity, kTyped>));
-    constexpr int cap = (MM <= 8 && KH <= 2) ? 5 : 4;
+    constexpr int cap = (MM <= 4 && KH <= 2) ? 5 : 4;
     return lds < cap ? lds : cap;
 }
 
@@ -663,34 +663,33 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                         : row;
                 const double vol = sVS[2 * c], solQ = sVS[2 * c + 1];
                 const int shift = 2 * ahb;
+                // one addend per map and a single update of every e[m] after
+                // the two paths join, so the accumulators never sit in
+                // divergent branches (a map that keeps a min/max, or a
+                // covalent map, adds 0: e + 0 == e)
+                double add[MM];
                 if (((a.addOnlyClasses >> ahb) & 1) && kTyped) {
                     // every map adds the table value for this atom (no H-bond
                     // term, no covalent map): no per-map control flow.  Maps
                     // m >= M (the kernel is compiled for 4, 6, 8 or 16 maps)
                     // accumulate values that are never stored.
 #pragma unroll
-                    for (int m = 0; m < MM; ++m) e[m] += FG_ROW(m);
-                    sv = __builtin_fma(vol, S, sv);
-                    sq = __builtin_fma(solQ, S, sq);
-                    dmap = __builtin_fma(A.qvol, S, dmap);
-                    continue;
-                }
+                    for (int m = 0; m < MM; ++m) add[m] = FG_ROW(m);
+                } else {
 #pragma unroll
                 for (int m = 0; m < MM; ++m) {
+                    add[m] = 0.0;
                     if (m >= M || a.mp[m].isCovalent) continue;
                     const int act = (a.mp[m].actions >> shift) & 3;
                     const double E = FG_ROW(m);
-                    // one addend per map and a single update of e[m], so the
-                    // accumulators never sit in divergent branches (a map that
-                    // keeps a min/max adds nothing here: e + 0 == e)
-                    double add = E;
+                    add[m] = E;
                     if (act != kActAdd) {
                         // min(max(E / 100, 0), 1), formed on the host per table entry
                         const double rsph = a.rsph[rowOff + m];
                         if (act == kActDonor) {
-                            add = rowH[m] * Hramp * (racc + (1 - racc) * rsph);
+                            add[m] = rowH[m] * Hramp * (racc + (1 - racc) * rsph);
                         } else {
-                            add = 0.0;
+                            add[m] = 0.0;
                             if (m < KH) {
                                 const double f = act == kActMinRacc ? racc : rdon;
                                 const double v = E * (f + (1 - f) * rsph);
@@ -700,8 +699,10 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                             }
                         }
                     }
-                    e[m] += add;
                 }
+                }
+#pragma unroll
+                for (int m = 0; m < MM; ++m) e[m] += add[m];
                 sv = __builtin_fma(vol, S, sv);
                 sq = __builtin_fma(solQ, S, sq);
                 dmap = __builtin_fma(A.qvol, S, dmap);
