@@ -180,7 +180,7 @@ struct fg_job {
     int px = 1, pz = 4, tile = 64, bxShift = 4;
     bool shaped = false;  // the shape came from fg_autotune or fg_job_set_shape
     int plainDiel = FG_PLAIN_DIEL_TABLE;  // dielectric variant of the plain e-kernel
-    float farCut2 = 0, lastCut2 = 0, tableCut2 = 0, spacing100 = 0;
+    float lastCut2 = 0, tableCut2 = 0, spacing100 = 0;
     double elecEd = 0;  // bound on the FP32 coordinates' error per axis (scaled units)
     // e-map band of the cutoff kernel (A^2) and the candidate lists' reach (A)
     double e2Lo = 0, e2Hi = 0, listReach = 8.0 + 1e-4;
@@ -421,11 +421,15 @@ fgk::ElecArgs elecArgs(const fg_job *j, int px, int pz, int tile, int bxShift)
     a.tile = tile;
     a.bxShift = bxShift;
     if (bxShift == 0) {
-        // linear items: units of px points of a row times the pz planes of a
-        // slab, 256 to an item; region 0 the full slabs, region 1 the rest
-        // (job creation bounds the grid so that these counts fit in 32 bits)
+        // linear items: units of px points 8 apart in a row times the pz
+        // planes of a slab, in 8 px x 8-column tiles, 256 units to an item;
+        // region 0 the full slabs, region 1 the rest (job creation bounds the
+        // grid so that these counts fit in 32 bits)
         const int64_t T = fgk::kElecThreads;
-        a.ux = (j->g.nx + px - 1) / px;
+        a.tilesX = (j->g.nx + 8 * px - 1) / (8 * px);
+        const int lastWidth = j->g.nx - 8 * px * (a.tilesX - 1);
+        a.lastLanes = px == 2 ? (lastWidth + 1) / 2 : lastWidth;
+        a.ux = 8 * (a.tilesX - 1) + a.lastLanes;
         a.units = a.ux * j->g.ny;
         a.slabs = j->g.nz / pz;
         a.itemsA = int32_t((int64_t(a.slabs) * a.units + T - 1) / T);
@@ -446,7 +450,6 @@ fgk::ElecArgs elecArgs(const fg_job *j, int px, int pz, int tile, int bxShift)
     a.eps32 = j->dEps32;
     a.e = j->dMaps + int64_t(j->M) * j->P;
     a.far = j->dFar;
-    a.farCut2 = j->farCut2;
     a.lastCut2 = j->lastCut2;
     a.tableCut2 = j->tableCut2;
     a.spacing100 = j->spacing100;
@@ -628,14 +631,10 @@ int buildElecTiles(fg_ctx *ctx, fg_job *j, const fg_job_desc &d, double gridMaxA
     if (j->e2Hi >= 64.08)  // bins up to 800 only in the cutoff kernel's tables
         return fgc::fail(FG_E_ARG, "fg_job_create: coordinates too far from the origin (FP32 near-field band)");
     j->listReach = std::sqrt(j->e2Hi) + 1e-4;
-    // Boxes farther apart than ~8.02 A hold only far pairs: the FP32
-    // coordinates and outward-rounded boxes differ from the FP64 ones by at
-    // most Ed per axis (< 1e-5 A for atoms within 100 A of the origin), far
-    // inside the 0.02 A margin.  A pair beyond the guard band gets the same
-    // FP32 arithmetic in the near and far loops, so the cut moves no bits.
+    // The FP32 coordinates and outward-rounded sub-tile boxes differ from
+    // the FP64 ones by at most Ed per axis (< 1e-5 A for atoms within 100 A
+    // of the origin): the margin of the table-range tests below.
     j->elecEd = Ed;
-    const double farCut = 802.0 + 4 * Ed;
-    j->farCut2 = float(farCut * farCut);
     j->spacing100 = float(s * 100.0);
     // dielectric table for k_elec: extended past 8191 (no clamp) when it fits
     double span2 = 0;
@@ -663,9 +662,9 @@ int buildElecTiles(fg_ctx *ctx, fg_job *j, const fg_job_desc &d, double gridMaxA
         j->tableLen = fgk::kTableLen;
         j->clamp = true;
     }
-    // Sub-tiles whose boxes lie farther than lastCut from the brick only read
-    // the last table bin (100 r > 8191 with the same 2 + 4 Ed margin as the far
-    // cut); sub-tiles whose farthest corner lies within tableCut never index
+    // Sub-tiles whose boxes lie farther than lastCut from a wave's box only
+    // read the last table bin (100 r > 8191, with a margin of 2 bins plus 4
+    // Ed); sub-tiles whose farthest corner lies within tableCut never index
     // past the table (the clamped variant skips its clamp for them).
     const double lastCut = (fgk::kTableLen + 2.0) + 4 * Ed;
     j->lastCut2 = float(lastCut * lastCut);

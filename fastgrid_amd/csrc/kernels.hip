@@ -52,12 +52,18 @@ constexpr unsigned kFloorBits = 0x4B000000u;   // bit pattern of 2^23
 // reading zeros from table bins 0..kNearBins-1.
 constexpr int kNearBins = 800;
 
+// Added to dy^2 in the far-field arithmetic: every nonzero square of these
+// scaled differences (>= ~1e-6) absorbs it exactly, so only an atom sitting
+// exactly on a grid point sees it, as a finite 1/r (1e15) times a zero table
+// entry instead of inf * 0.
+constexpr float kTinyR2 = 1e-30f;
+
 // The far-field kernel's FP32 squared distance (scaled by 1e4) between atom
 // (x, y, z, scaled by 100) and a grid point: the operations in its order.
 __device__ __forceinline__ float farR2(float ax, float ay, float az, float gx, float gy, float gz)
 {
     const float dy = ay - gy;
-    const float dy2 = dy * dy;
+    const float dy2 = __builtin_fmaf(dy, dy, kTinyR2);
     const float dx = ax - gx;
     const float dxy2 = __builtin_fmaf(dx, dx, dy2);
     const float dz = az - gz;
@@ -865,7 +871,8 @@ __device__ __forceinline__ void loadEps(float *eps, const float *src, int n, int
 // along x times PZ consecutive points along z.  2D bricks (bxShift > 0):
 // 1 << bxShift lanes along x, the rest along y, a lane's x points 1 <<
 // bxShift apart.  Linear items (bxShift 0): 256 consecutive units of PX
-// adjacent points (ElecArgs); `region` 1 is the partial
+// points 8 apart along x, adjacent in the last tile (ElecArgs); `region` 1
+// is the partial
 // slab at the top of the grid, whose units stand zStride planes above the
 // last full slab.  `eps` is the table in LDS.  The FP32 arithmetic of a
 // (point, atom) pair does not depend on the shape, and the compensated form
@@ -899,11 +906,21 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
         real = u < nU;
         if (!real) u = nU - 1;
         const int s = slab0 + u / a.units, w = u - (u / a.units) * a.units;
-        iy = w / a.ux;
-        ix = PX * (w - iy * a.ux);
+        // within the slab: tile row ty (8 rows, fewer in the last), tile tx
+        // in it (8 x h units, fewer lane columns in the last), lane (lx, ly)
+        const int ty = w / (8 * a.ux), r = w - ty * 8 * a.ux, h = min(8, a.g.ny - 8 * ty);
+        const int tx = r / (8 * h), s2 = r - tx * 8 * h;
+        const bool lastTile = tx == a.tilesX - 1;
+        const int wt = lastTile ? a.lastLanes : 8;
+        const int ly = s2 / wt, lx = s2 - ly * wt;
+        // two points per lane: 8 apart, or adjacent in the last tile, which
+        // may be narrower than 16 points
+        xs = PX == 2 && lastTile ? 1 : 8;
+        ix = tx * 8 * PX + (xs == 1 ? 2 * lx : lx);
+        iy = 8 * ty + ly;
         iz0 = s * zStride;
 #pragma unroll
-        for (int i = 0; i < PX; ++i) cx[i] = min(ix + i, a.g.nx - 1);
+        for (int i = 0; i < PX; ++i) cx[i] = min(ix + xs * i, a.g.nx - 1);
         cy = iy;
     } else {
         // a lane's PX points are bxl apart along x, so that each table read
@@ -985,20 +1002,17 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
             for (int t = 0; t < kGroup; ++t) fn(make_float4(v[4 * t], v[4 * t + 1], v[4 * t + 2], v[4 * t + 3]));
         }
     };
-    // far field: every lane's pair is beyond 8 A (box tests at 8.02 A).
-    // Clamp: bins past the table's end are clamped to its last entry (the
-    // clamped kernel variant, for tiles that may reach beyond the table).
-    // Near (sub-tiles within 8.02 A of the wave): some pairs have FP32 bins
-    // below kNearBins; the compensated form's table reads 0 there, so they
-    // add nothing (they are the cutoff kernel's).  r^2 is raised to 1
-    // (r = 0.01 A, bin 0) so that an atom on a grid point gives 1/r = 1, not
-    // inf; every pair the far field keeps has r^2 > 6.4e5, unchanged.
-    auto farAtoms = [&](int c0, int c1, auto clampTag, auto nearTag) {
+    // Every pair of a tile: the compensated form's table reads 0 at FP32
+    // bins below kNearBins, so the pairs within 8 A add nothing (they are the
+    // cutoff kernel's), and an atom on a grid point has a finite 1/r
+    // (kTinyR2).  Clamp: bins past the table's end are clamped to its last
+    // entry (the clamped kernel variant, for tiles that may reach beyond the
+    // table).
+    auto farAtoms = [&](int c0, int c1, auto clampTag) {
         constexpr bool kClampHere = kClamp && decltype(clampTag)::value;
-        constexpr bool kNear = decltype(nearTag)::value;
         forAtoms(c0, c1, [&](const float4 at) {
             const float dy = at.y - gy;
-            const float dy2 = dy * dy;
+            const float dy2 = __builtin_fmaf(dy, dy, kTinyR2);
             float dxy2[PX], dz[PZ];
 #pragma unroll
             for (int i = 0; i < PX; ++i) {
@@ -1011,8 +1025,7 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
             unsigned bin[NP];
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
-                float r2 = __builtin_fmaf(dz[q % PZ], dz[q % PZ], dxy2[q / PZ]);
-                if (kNear) r2 = __builtin_fmaxf(r2, 1.0f);
+                const float r2 = __builtin_fmaf(dz[q % PZ], dz[q % PZ], dxy2[q / PZ]);
                 ri[q] = __builtin_amdgcn_rsqf(r2);
                 if (kComp) {
                     // floor(100 r) in one operation: r2 * ri + (2^23 - 1/2)
@@ -1048,7 +1061,7 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
     auto lastBinAtoms = [&](int c0, int c1) {
         forAtoms(c0, c1, [&](const float4 at) {
             const float dy = at.y - gy;
-            const float dy2 = dy * dy;
+            const float dy2 = __builtin_fmaf(dy, dy, kTinyR2);
             float dxy2[PX], dz[PZ];
 #pragma unroll
             for (int i = 0; i < PX; ++i) {
@@ -1077,30 +1090,24 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
             dmax2 = __builtin_fmaf(e, e, dmax2);
         }
     };
-    auto farFrom = [&](int q0, const float *lo, const float *hi) {
-        float dmin2, dmax2;
-        boxDist2(q0, lo, hi, dmin2, dmax2);
-        return dmin2 > a.farCut2;
-    };
     using kYes = std::integral_constant<bool, true>;
     using kNo = std::integral_constant<bool, false>;
 
     for (int j0 = 0; j0 < a.nAtoms; j0 += a.tile) {
         const int j1 = min(j0 + a.tile, a.nAtoms);
-        // the tile's class against the wave, from its sub-tile boxes: near
-        // (0), far (1), far but possibly past the table's end (2, clamped
-        // variant only), past the last table bin (3)
+        // the tile's class against the wave, from its sub-tile boxes: within
+        // the table (1), possibly past the table's end (2, clamped variant
+        // only), past the last table bin (3)
         int kind = 1;
         if (kComp) {
-            bool far = true, last = true, inTable = true;
+            bool last = true, inTable = true;
             for (int q = j0; q < j1; q += kSubTile) {
                 float dmin2, dmax2;
                 boxDist2(q, wlo, whi, dmin2, dmax2);
-                far = far && dmin2 > a.farCut2;
                 last = last && dmin2 > a.lastCut2;
                 inTable = inTable && dmax2 < a.tableCut2;
             }
-            kind = uniform(last ? 3 : far ? (kClamp && !inTable ? 2 : 1) : 0);
+            kind = uniform(last ? 3 : (kClamp && !inTable ? 2 : 1));
         }
         for (int c0 = j0; c0 < j1; c0 += kFold) {
             const int c1 = min(c0 + kFold, j1);
@@ -1108,18 +1115,10 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
                 lastBinAtoms(c0, c1);
             } else if (kind == 1) {
                 // the FP32 form (kComp false) clamps as its variant says
-                if (kComp) farAtoms(c0, c1, kNo(), kNo());
-                else farAtoms(c0, c1, kYes(), kNo());
-            } else if (kind == 2) {
-                farAtoms(c0, c1, kYes(), kNo());
+                if (kComp) farAtoms(c0, c1, kNo());
+                else farAtoms(c0, c1, kYes());
             } else {
-                // near tile: decide per sub-tile (a fold chunk holds whole
-                // sub-tiles)
-                for (int q = c0; q < c1; q += kSubTile) {
-                    const int q1 = min(q + kSubTile, c1);
-                    if (uniform(farFrom(q, wlo, whi))) farAtoms(q, q1, kYes(), kNo());
-                    else farAtoms(q, q1, kYes(), kYes());
-                }
+                farAtoms(c0, c1, kYes());
             }
             if (kComp && ((c1 & (kFold - 1)) == 0 || c1 == a.nAtoms)) {
                 // fold the FP32 partial into the two-float accumulator (TwoSum)

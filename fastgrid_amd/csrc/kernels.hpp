@@ -130,13 +130,20 @@ struct ElecArgs {
     int32_t tile;            // atoms per tile (a multiple of kSubTile)
     int32_t bxShift;         // lanes along x = 1 << bxShift (8, 16 or 32); 0: linear items (below)
     int32_t nbx, nby, nbz;   // 2D bricks per axis (bxShift > 0)
-    // Linear items (bxShift 0): a unit is PX x-adjacent points of one row
-    // times the PZ planes of one z-slab; units are numbered slab after slab,
-    // row after row, and an item is 256 consecutive units, so items cross
-    // rows and slabs and only the last item of a region has padding lanes.
-    // Region 0 holds the `slabs` full slabs (itemsA items), region 1 the
-    // remaining nz - slabs * PZ planes, if any.
-    int32_t ux;              // units per row: ceil(nx / PX)
+    // Linear items (bxShift 0): a unit is PX points 8 apart along x (two
+    // adjacent ones in the last tile column) times the PZ planes of one
+    // z-slab.  A slab's columns form tiles of 8 PX x 8 columns (narrower in
+    // the last tile column, shorter in the last tile row); a tile holds 8 x 8
+    // units (lane columns x rows), fewer in the last ones, so that one
+    // wave's reads cover a compact patch.  Units are
+    // numbered slab after slab, tile row after tile row, tile after tile,
+    // row after row in a tile; an item is 256 consecutive units, so items
+    // cross tiles and slabs and only the last item of a region has padding
+    // lanes.  Region 0 holds the `slabs` full slabs (itemsA items), region 1
+    // the remaining nz - slabs * PZ planes, if any.
+    int32_t ux;              // lane columns per tile row: 8 (tilesX - 1) + lastLanes
+    int32_t tilesX;          // tiles per tile row: ceil(nx / (8 PX))
+    int32_t lastLanes;       // lane columns of the last tile: its width w (PX 1), ceil(w / 2) (PX 2)
     int32_t units;           // units per slab: ux * ny
     int32_t slabs;           // full slabs: nz / PZ
     int32_t itemsA;          // items of region 0
@@ -151,7 +158,6 @@ struct ElecArgs {
     double *e;               // [P] electrostatic map (the FP32 form writes it)
     double *far;             // [P] far-field sums of the compensated form, added to the near part
                              // k_cutoff wrote in e by k_add_far
-    float farCut2;           // (802)^2: sub-tiles farther than 8.02 A hold only far pairs
     float lastCut2;          // sub-tiles farther than this (~81.94 A) read only the last table bin
     float tableCut2;         // sub-tiles closer than this at their farthest stay inside the table
     float spacing100;        // 100 * spacing

@@ -295,9 +295,9 @@ int fg_job_summary(fg_ctx *ctx, fg_job *job, fg_summary *out);
 /* Shape of a job's electrostatic kernel: grid points per lane along z (4 or
  * 8), receptor atoms per tile (32, 64 or 128), lanes of the 256-lane brick
  * along x (8, 16 or 32; 256: linear items, 256 consecutive lane units taken
- * row after row and z-slab after z-slab, which pad only at the end of the
- * grid) and consecutive grid points per lane along x (1 or 2).  The shape
- * changes only the speed: every shape gives the same bits. */
+ * tile of 8 x 8 lane columns after tile and z-slab after z-slab, which pad
+ * only at the end of the grid) and grid points per lane along x (1 or 2).
+ * The shape changes only the speed: every shape gives the same bits. */
 typedef struct {
     int32_t pz, tile, bx, px;
 } fg_shape;
