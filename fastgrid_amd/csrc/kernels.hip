@@ -554,38 +554,66 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
     const double cut2 = a.doElecNear ? a.e2Hi : 64.0;  // e2Hi > 64
 
     // candidates: one run over the brick's list, or one run per receptor
-    // type over the type-sorted list with that type's rows in LDS
+    // type over the type-sorted list with that type's rows in LDS, in chunks
+    // of kChunk atoms staged in LDS.  Each chunk's list entries and atom
+    // records are loaded into registers while the chunk before it is
+    // computed (one 16-byte piece of a record per lane, lane 8c also atom
+    // c's desolvation factors), so their latency hides behind that work.
     const int nRuns = kTyped ? a.T : 1;
-    for (int run = 0; run < nRuns; ++run) {
-        int c0, c1;
-        if (kTyped) {
-            c0 = constView(a.typeStart)[b * (a.T + 1) + run];
-            c1 = constView(a.typeStart)[b * (a.T + 1) + run + 1];
-            if (c0 == c1) continue;
-            __syncthreads();  // every lane is done with the previous type's rows
+    auto runStart = [&](int r) {
+        return kTyped ? constView(a.typeStart)[b * (a.T + 1) + r] : constView(a.brickStart)[b + r];
+    };
+    // the chunk after (run, cb) of a run ending at c1; run == nRuns: none
+    int run = -1, cb = 0, c1 = 0;
+    auto advance = [&]() {
+        if (run >= 0 && cb + kChunk < c1) {
+            cb += kChunk;
+            return;
+        }
+        for (++run; run < nRuns; ++run) {
+            cb = runStart(run);
+            c1 = runStart(run + 1);
+            if (cb < c1) return;
+        }
+    };
+    float4 pRec = make_float4(0.f, 0.f, 0.f, 0.f);
+    double2 pVS = make_double2(0.0, 0.0);
+    int pIdx = -1;
+    auto prefetch = [&]() {
+        const int nc = min(kChunk, c1 - cb), at = tid >> 3;
+        pIdx = at < nc ? a.brickList[cb + at] : -1;
+        if (pIdx >= 0) {
+            pRec = reinterpret_cast<const float4 *>(a.atoms + pIdx)[tid & 7];
+            if (kAffinity && (tid & 7) == 0) pVS = reinterpret_cast<const double2 *>(a.vs)[pIdx];
+        }
+    };
+    advance();
+    if (run < nRuns) prefetch();
+    int rowsOf = -1;  // the receptor type whose rows are in LDS
+    while (run < nRuns) {
+        const int nc = min(kChunk, c1 - cb);
+        __syncthreads();  // every lane is done with the previous chunk (and type's rows)
+        if (kTyped && run != rowsOf) {
             const double *src = a.pair + unsigned(run * kPairBins * M);
             constexpr int ML = ldsMaps<MM>();
             for (int q = tid; q < kRowBins * ML; q += kCutThreads) {
                 const int bin = q / ML, m = q - bin * ML;
                 L.row[q] = m < M ? src[bin * M + m] : 0.0;
             }
-        } else {
-            c0 = constView(a.brickStart)[b];
-            c1 = constView(a.brickStart)[b + 1];
+            rowsOf = run;
         }
-        for (int cb = c0; cb < c1; cb += kChunk) {
-            const int nc = min(kChunk, c1 - cb);
-            __syncthreads();  // every lane is done with the previous chunk
-            // stage nc atom records (16-byte pieces) and their desolvation factors
-            for (int q = tid; q < nc * 8; q += kCutThreads) {
-                const int ia = a.brickList[cb + (q >> 3)];
-                if ((q & 7) == 0) sIdx[q >> 3] = ia;
-                reinterpret_cast<float4 *>(sAtom)[q] = reinterpret_cast<const float4 *>(a.atoms + ia)[q & 7];
+        // stage this chunk, request the next one
+        if (pIdx >= 0) {
+            reinterpret_cast<float4 *>(sAtom)[tid] = pRec;
+            if ((tid & 7) == 0) {
+                sIdx[tid >> 3] = pIdx;
+                if (kAffinity) reinterpret_cast<double2 *>(sVS)[tid >> 3] = pVS;
             }
-            if (kAffinity && tid < nc)
-                reinterpret_cast<double2 *>(sVS)[tid] =
-                    reinterpret_cast<const double2 *>(a.vs)[a.brickList[cb + tid]];
-            __syncthreads();
+        }
+        advance();
+        if (run < nRuns) prefetch();
+        __syncthreads();
+        {
             for (int c = 0; c < nc; ++c) {
                 const CutAtom &A = sAtom[c];
                 const double dx = A.x - gx, dy = A.y - gy, dz = A.z - gz;
