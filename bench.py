@@ -171,9 +171,9 @@ def copy_calibration(nbytes=1 << 30):
 
 
 # the autotuner's candidate shapes (pz, tile, bx, px): variant (px, pz) x tile
-# x brick width, the 256-column brick for px 1 only
+# x brick width (256: linear items)
 ELEC_SHAPES = [(pz, tile, bx, px) for px, pz in ((1, 4), (1, 8), (2, 4), (2, 8)) for tile in (32, 64, 128)
-               for bx in ((8, 16, 32, 256) if px == 1 else (8, 16, 32))]
+               for bx in (8, 16, 32, 256)]
 
 
 def elec_shapes(fg, shapes, w=WORKLOAD_L):
@@ -657,7 +657,7 @@ def main():
     ap.add_argument("--no-small", action="store_true", help="skip the small and medium configurations")
     ap.add_argument("--elec-shapes", default="",
                     help="only launch the electrostatic kernel on L at these shapes, e.g. '4,32,16,1;4,32,8,2' "
-                         "(pz,tile,bx,px), or 'all' for the autotuner's 42 shapes (for rocprofv3 counter runs)")
+                         "(pz,tile,bx,px), or 'all' for the autotuner's 48 shapes (for rocprofv3 counter runs)")
     ap.add_argument("--elec-workload", default="L", choices=["L", "X"],
                     help="workload of --elec-shapes: L (50k atoms, 127^3) or X (the 250k-atom / 161^3 stress size)")
     ap.add_argument("--batch-graph-only", action="store_true",

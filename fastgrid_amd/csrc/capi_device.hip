@@ -302,6 +302,22 @@ fgk::ListArgs listArgs(const fg_job *j)
     return la;
 }
 
+// Whether the cutoff phase's waves raise their issue priority above the
+// persistent e-kernel's beside them (CutoffArgs::prio), from an estimate of
+// the two phases' VALU wave-instructions: about 45 per candidate and wave of
+// the cutoff kernel (4 waves visit every candidate of their brick) and 7.5
+// per 64 pairs of the e-kernel.  The cutoff kernels wait on memory far more
+// than the e-kernel, which holds 3 of the 4 slots per CU, so with a large
+// share of the work they end long after it: on the 256-pocket batch (cutoff
+// ~21 % of the instructions) the raised priority took a replay from 62.4 ms
+// to 60.0 ms; on the 50,000-atom job (~12 %) the two phases end together
+// without it (22.45 ms per step, 22.7 ms with it).
+int cutoffPriority(double candidates, double elecPairs)
+{
+    const double cut = 180.0 * candidates, elec = 7.5 / 64.0 * elecPairs;
+    return cut > 0.15 * (cut + elec) ? 1 : 0;
+}
+
 fgk::CutoffArgs cutoffArgs(const fg_job *j, int doElecNear)
 {
     fgk::CutoffArgs a{};
@@ -338,6 +354,7 @@ fgk::CutoffArgs cutoffArgs(const fg_job *j, int doElecNear)
     a.e2Lo = j->e2Lo;
     a.e2Hi = j->e2Hi;
     a.spacing100 = j->spacing100;
+    a.prio = cutoffPriority(double(j->listTotal), double(j->N) * double(j->P));
     return a;
 }
 
@@ -456,20 +473,6 @@ fgk::ElecArgs elecArgs(const fg_job *j, int px, int pz, int tile, int bxShift)
     return a;
 }
 
-// Workgroup slots per CU that the persistent e-kernel leaves to the cutoff
-// phase running beside it, from an estimate of the two phases' VALU
-// wave-instructions: about 45 per candidate and wave of the cutoff kernel (4
-// waves visit every candidate of their brick) and 7.5 per 64 pairs of the
-// e-kernel.  The cutoff kernels wait on memory far more than the e-kernel,
-// so they need more slots than their share of the instructions: on the
-// 256-pocket batch (cutoff ~21 % of the instructions) two slots gave 65.2 ms
-// per batch against 66.7 ms with one and 89 ms with three; on the 50,000-atom
-// job (~12 %) one slot lets both phases end together.
-int cutoffReserve(double candidates, double elecPairs)
-{
-    const double cut = 180.0 * candidates, elec = 7.5 / 64.0 * elecPairs;
-    return cut > 0.15 * (cut + elec) ? 2 : 1;
-}
 
 // The persistent e-kernel: the workgroups its occupancy admits per CU (found
 // when the job is made, so that nothing here queries the device), less
@@ -1234,6 +1237,7 @@ struct fg_batch {
     bool pointAng = false, clamp = false, typed = true;  // typed: every job has type-sorted lists
     int sharedTableLen = 0, maxTableLen = 0;
     int cutItems = 0, elecItems = 0, elecWGs = 0, farItems = 0;
+    int cutPrio = 0;  // CutoffArgs::prio of every job of the batch
     size_t cutLds = 0;  // LDS of one workgroup of the batch's cutoff kernel
     fgk::CutoffArgs *dCut = nullptr;
     fgk::ListArgs *dList = nullptr;  // the candidate-list passes of every job
@@ -1316,17 +1320,11 @@ int launchElecBatch(fg_batch *b, bool launch, hipStream_t s)
     auto k = fgk::k_elec_batch<PX, PZ, kComp, kClamp>;
     const size_t lds = size_t(b->sharedTableLen > 0 ? b->sharedTableLen : b->maxTableLen) * sizeof(float);
     // a persistent grid would hold every slot until the queue is empty: keep
-    // one or two per CU (cutoffReserve), and the LDS of one cutoff
-    // workgroup, for the concurrent cutoff kernel
+    // one per CU, and the LDS of one cutoff workgroup, for the concurrent
+    // cutoff kernel
     if (!b->elecWGs) {
         const bool shared = b->aff || b->near || b->fmap;
-        double cand = 0, pairs = 0;
-        for (const fg_job *j : b->jobs) {
-            cand += double(j->listTotal);
-            pairs += double(j->N) * double(j->P);
-        }
-        if (int rc = persistentWGs(b, k, fgk::kElecThreads, lds, b->elecItems, b->elecWGs,
-                                   shared ? cutoffReserve(cand, pairs) : 0))
+        if (int rc = persistentWGs(b, k, fgk::kElecThreads, lds, b->elecItems, b->elecWGs, shared ? 1 : 0))
             return rc;
         if (shared && b->cutLds > 0 && b->ctx->ldsPerCU > b->cutLds) {
             const int fit = int((b->ctx->ldsPerCU - b->cutLds) / (lds + 256));
@@ -1483,12 +1481,21 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
         for (const fg_job *j : b->jobs) same = same && j->epsId >= 0 && j->epsId == b->jobs[0]->epsId;
         b->sharedTableLen = same ? b->maxTableLen : 0;
     }
+    {
+        double cand = 0, pairs = 0;
+        for (const fg_job *j : b->jobs) {
+            cand += double(j->listTotal);
+            pairs += double(j->N) * double(j->P);
+        }
+        b->cutPrio = cutoffPriority(cand, pairs);
+    }
     std::vector<fgk::CutoffArgs> ca;
     std::vector<fgk::ListArgs> la;
     std::vector<fgk::ElecArgs> ea;
     std::vector<int32_t> cutBase(1, 0), elecBase(1, 0), farBase(1, 0);
     for (fg_job *j : b->jobs) {
         ca.push_back(cutoffArgs(j, b->near ? 1 : 0));
+        ca.back().prio = b->cutPrio;
         if (!(b->typed && b->aff)) ca.back().brickList = j->dList;
         la.push_back(listArgs(j));
         if (!(b->typed && b->aff)) la.back().sorted = nullptr;  // no type sort
@@ -1988,11 +1995,7 @@ static int enqueueRun(fg_ctx *ctx, fg_job *j, unsigned what, int precision, RunM
         FG_HIP(rec(ctx->e0, ctx->side));
         if (precision == FG_PREC_PLAIN) {
             if (int rc = runPlainElec(j, ctx->side)) return rc;
-        } else if (int rc = launchElec(j, precision,
-                                       (aff || near || fmap)
-                                           ? cutoffReserve(double(j->listTotal), double(j->N) * double(j->P))
-                                           : 0,
-                                       ctx->side)) {
+        } else if (int rc = launchElec(j, precision, (aff || near || fmap) ? 1 : 0, ctx->side)) {
             return rc;
         }
         FG_HIP(rec(ctx->e1, ctx->side));

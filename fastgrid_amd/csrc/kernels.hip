@@ -489,6 +489,9 @@ constexpr int cutWaves()
 template <int MM, int KH, bool kAffinity, bool kPointAng, bool kTyped>
 __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLds<MM, kAffinity, kTyped> &L)
 {
+    // the waves of a cutoff phase that carries a large share of the work
+    // issue before the e-kernel's when both are ready (see CutoffArgs::prio)
+    if (a.prio) __builtin_amdgcn_s_setprio(1);
     CutAtom *sAtom = L.atom;
     double *sVS = L.vs;
     int32_t *sIdx = L.idx;

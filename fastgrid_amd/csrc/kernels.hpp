@@ -94,7 +94,11 @@ struct CellGrid {
 struct CutoffArgs {
     GridGeom g;
     int32_t nbx, nby, nbz;  // bricks per axis
-    int32_t M, hydrogen, doElecNear, pad;
+    int32_t M, hydrogen, doElecNear;
+    // 1: the cutoff waves raise their issue priority above the e-kernel's
+    // running beside them (their memory waits leave the VALU to it, and when
+    // ready they issue first), for cutoff phases with a large share of the work
+    int32_t prio;
     int64_t P;
     const CutAtom *atoms;
     const double *vs;                         // [N][2] vol and solpar + solparQ |q| per atom
