@@ -5,24 +5,25 @@
 //    electrostatic map.  One workgroup per brick of 8x8x4 grid points, one
 //    point per lane.  Candidate atoms (those within 8 A of the brick) are
 //    streamed in a uniform order, so every receptor-atom branch (H-bond
-//    class, disorder, per-map probe class) is wave-uniform and atom records
-//    are fetched with scalar loads.  All arithmetic is FP64 in the
-//    reference's operation order (no contraction), so results match the
+//    class, disorder, per-map probe class) is wave-uniform.  All arithmetic
+//    is FP64 in the reference's operation order, so results match the
 //    reference CPU path (FG/CalculateGridmaps.cpp:84-316).
-//  * k_elec: far-field electrostatics in FP32.  One workgroup per brick of
-//    256 lanes (16x16, 8x32 or 32x8 in x, y) x PZ points along z.  Receptor
-//    atoms are spatially sorted into tiles (a tuned multiple of kSubTile
-//    atoms); each tile's contribution is
-//    summed in FP32 and folded into a compensated two-float accumulator, and
-//    the final per-point value is formed in FP64.  The inverse dielectric
-//    table lives in LDS.  Pairs whose exact FP64 distance is <= 8 A belong
-//    to k_cutoff and are masked here; the mask is decided in FP32 with an
-//    FP64 replay in a narrow ambiguity band, so the two kernels partition the
-//    (point, atom) pairs exactly.
+//  * k_elec: far-field electrostatics in FP32, persistent: workgroups of 256
+//    lanes take work items (2D bricks of 8, 16 or 32 lanes along x, or
+//    linear items of 256 lane units) from a counter, each lane PX points
+//    along x times PZ along z.  Receptor atoms are spatially sorted into
+//    tiles (a tuned multiple of kSubTile atoms); partial sums are FP32 over
+//    64 atoms, folded into a compensated two-float accumulator, and the final
+//    per-point value is formed in FP64.  The inverse dielectric table lives
+//    in LDS.  In the compensated form its bins below kNearBins hold 0: those
+//    pairs belong to k_cutoff, which takes exactly the pairs the far field's
+//    own FP32 bin drops, so the two kernels partition the (point, atom)
+//    pairs exactly.
 //  * k_elec_fp64: the whole electrostatic sum in the reference's FP64
 //    arithmetic (FG/electrostatics/Cpu.cpp:27-88), used as the on-device
 //    FP64 oracle for large workloads.
-// plus the candidate-list builders k_brick_count / k_brick_fill and a scan.
+// plus the candidate-list builders (k_brick_lists: count at job creation,
+// fill in every run; k_type_sort) and a scan.
 #pragma once
 
 #include <hip/hip_runtime.h>
