@@ -338,6 +338,8 @@ fgk::CutoffArgs cutoffArgs(const fg_job *j, int doElecNear)
     a.typeStart = j->dTypeStart;
     a.T = j->T;
     a.addOnlyClasses = j->addOnlyClasses;
+    a.covMask = 0;
+    for (int m = 0; m < j->M && m < 16; ++m) a.covMask |= j->mp[size_t(m)].isCovalent ? 1 << m : 0;
     a.pair = j->dPair;
     a.rsph = j->dRsph;
     a.desolv = j->dDesolv;
@@ -665,18 +667,24 @@ int buildElecTiles(fg_ctx *ctx, fg_job *j, const fg_job_desc &d, double gridMaxA
         j->tableLen = fgk::kTableLen;
         j->clamp = true;
     }
-    // Sub-tiles whose boxes lie farther than lastCut from a wave's box only
-    // read the last table bin (100 r > 8191, with a margin of 2 bins plus 4
-    // Ed); sub-tiles whose farthest corner lies within tableCut never index
-    // past the table (the clamped variant skips its clamp for them).
-    const double lastCut = (fgk::kTableLen + 2.0) + 4 * Ed;
-    j->lastCut2 = float(lastCut * lastCut);
-    j->tableCut2 = tableCut2Of(j->clamp ? j->tableLen : 0, Ed);
     eps32.resize(size_t(j->tableLen));
     for (int k = 0; k < j->tableLen; ++k) {
         double v = d.dist_dep_diel ? eps[std::min(k, fgk::kTableLen - 1)] : 1.0;
         eps32[size_t(k)] = float(v);
     }
+    // The FP32 table is constant from bin `tail` on (332/eps(r) stops changing
+    // in FP32 near 58.4 A; a constant dielectric is constant everywhere), at
+    // or past the compensated form's first far bin.  Sub-tiles whose boxes lie
+    // farther than lastCut from a wave's box (a margin of 2 bins plus 4 Ed)
+    // read only that value, as the last table bin: the bin, its address and
+    // the read drop out and every pair keeps its bits.  Sub-tiles whose
+    // farthest corner lies within tableCut never index past the table (the
+    // clamped variant skips its clamp for them).
+    int tail = fgk::kTableLen - 1;
+    while (tail > fgk::kNearBins && eps32[size_t(tail - 1)] == eps32[size_t(fgk::kTableLen - 1)]) --tail;
+    const double lastCut = (tail + 2.0) + 4 * Ed;
+    j->lastCut2 = float(lastCut * lastCut);
+    j->tableCut2 = tableCut2Of(j->clamp ? j->tableLen : 0, Ed);
     j->epsId = internEps(ctx, eps32.data());
     if (int rc = devUpload(j, j->dTiles, tiles.data(), tiles.size())) return rc;
     if (int rc = devUpload(j, j->dSubBox, box.data(), box.size())) return rc;
@@ -705,9 +713,11 @@ int buildCutoff(fg_job *j, const fg_job_desc &d)
                 c.rv2[k] = d.geom.rvector2[3 * i + k];
                 zeroVectors = zeroVectors && c.rv[k] == 0 && c.rv2[k] == 0;
             }
-            c.type = A.type[i];
             c.hbond = A.hbond[i];
             c.rexp = d.geom.rexp[i];
+            c.acts = 0;
+            for (int k = 0; k < j->M && k < 16; ++k)
+                c.acts |= int32_t(uint32_t((j->mp[size_t(k)].actions >> (2 * c.hbond)) & 3) << (2 * k));
             const int dis = d.geom.disorder[i] ? 1 : 0;
             int kind = fgk::kAngNone;
             c.ang[0] = c.ang[1] = c.ang[2] = 1.0;
@@ -726,7 +736,7 @@ int buildCutoff(fg_job *j, const fg_job_desc &d)
                 }
             }
             // the kernel reads the atom's uniform fields with one scalar move
-            c.flags = dis | (kind << 1) | (c.hbond << 3) | (c.type << 8);
+            c.flags = dis | (kind << 1) | (c.hbond << 3) | (A.type[i] << 8);
         }
         atoms[size_t(i)] = c;
         xyz[size_t(3 * i)] = c.x;

@@ -699,7 +699,6 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                         ? a.pair + unsigned((a.hydrogen * kPairBins + (idx - 110 > 0 ? idx - 110 : 0)) * M)
                         : row;
                 const double vol = sVS[2 * c], solQ = sVS[2 * c + 1];
-                const int shift = 2 * ahb;
                 // one addend per map and a single update of every e[m] after
                 // the two paths join, so the accumulators never sit in
                 // divergent branches (a map that keeps a min/max, or a
@@ -713,11 +712,12 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
 #pragma unroll
                     for (int m = 0; m < MM; ++m) add[m] = FG_ROW(m);
                 } else {
+                const unsigned acts = unsigned(uniform(A.acts));
 #pragma unroll
                 for (int m = 0; m < MM; ++m) {
                     add[m] = 0.0;
-                    if (m >= M || a.mp[m].isCovalent) continue;
-                    const int act = (a.mp[m].actions >> shift) & 3;
+                    if (m >= M || ((a.covMask >> m) & 1)) continue;
+                    const int act = int(acts >> (2 * m)) & 3;
                     const double E = FG_ROW(m);
                     add[m] = E;
                     if (act != kActAdd) {
@@ -1085,8 +1085,9 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
             }
         });
     };
-    // beyond the last table bin: every lane's pair has 100 r > 8192, where
-    // the table holds its last entry, so the bin, its address and the read
+    // in the table's constant tail: every lane's pair has a bin from which on
+    // the FP32 table holds its last entry (100 r past ~5842 for the
+    // distance-dependent dielectric), so the bin, its address and the read
     // drop out; the rest is the far loop's arithmetic, so the bits are the same
     const float tLast = eps[kTableLen - 1];
     auto lastBinAtoms = [&](int c0, int c1) {
@@ -1128,7 +1129,7 @@ __device__ __forceinline__ void elecBrick(const ElecArgs &a, int b, const float 
         const int j1 = min(j0 + a.tile, a.nAtoms);
         // the tile's class against the wave, from its sub-tile boxes: within
         // the table (1), possibly past the table's end (2, clamped variant
-        // only), past the last table bin (3)
+        // only), in the table's constant tail (3)
         int kind = 1;
         if (kComp) {
             bool last = true, inTable = true;

@@ -50,7 +50,10 @@ struct alignas(16) CutAtom {
     double rv[3];        // bond vector
     double rv2[3];       // second bond vector
     double ang[3];       // racc, rdon, Hramp when they do not depend on the point
-    int32_t type, hbond, rexp, flags;  // flags: kFlagDisorder | angular kind << 1 | hbond << 3 | type << 8
+    // acts: bits 2k..2k+1 = what map k (kernel order) does with this atom's
+    // H-bond class (kAct* below), so the pair loop decodes the actions of
+    // every map from one uniform word instead of holding them per map
+    int32_t acts, hbond, rexp, flags;  // flags: kFlagDisorder | angular kind << 1 | hbond << 3 | type << 8
 };
 static_assert(sizeof(CutAtom) == 128, "CutAtom must stay 128 bytes");
 constexpr int kFlagDisorder = 1;
@@ -109,6 +112,7 @@ struct CutoffArgs {
     int32_t T;                                // receptor types (rows of the pair table)
     int32_t addOnlyClasses;                   // bit h: atoms of H-bond class h add E to every map
                                               // and no map is covalent
+    int32_t covMask;                          // bit m: map m (kernel order) is a covalent map
     const double *pair;    // [T][kPairBins][M], maps in kernel order
     const double *rsph;    // [T][kPairBins][M]: min(max(pair / 100, 0), 1) (FG/CalculateGridmaps.cpp:210)
     const double *desolv;  // [8192]
@@ -163,7 +167,7 @@ struct ElecArgs {
     double *e;               // [P] electrostatic map (the FP32 form writes it)
     double *far;             // [P] far-field sums of the compensated form, added to the near part
                              // k_cutoff wrote in e by k_add_far
-    float lastCut2;          // sub-tiles farther than this (~81.94 A) read only the last table bin
+    float lastCut2;          // sub-tiles farther than this (~58.4 A) lie in the table's constant tail
     float tableCut2;         // sub-tiles closer than this at their farthest stay inside the table
     float spacing100;        // 100 * spacing
 };
