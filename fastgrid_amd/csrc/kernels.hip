@@ -411,27 +411,22 @@ __device__ __noinline__ AngTerms pointAngular(const CutAtom &A, bool isClosest, 
 }
 
 // FP64 1/sqrt(x) for the cutoff kernel's pairs (1e-24 <= x <= e2Hi, a hair
-// above 64; the same binade parity as the measured range): the
-// hardware estimate plus one Newton step.  v_rsq_f64 measured at most
-// 2^-24.2 relative on 4 M inputs in [1e-6, 64] on MI355X; from any estimate
-// within 2^-22 one step leaves at most 1.5 e^2 plus rounding, below 4e-13,
-// which is all the continuous uses need; table-bin decisions made from it
-// are guarded (see binGuarded).
-__device__ __forceinline__ double rsqrtFast(double x)
-{
-    const double y = __builtin_amdgcn_rsq(x);
-    return y * __builtin_fma(-0.5 * x * y, y, 1.5);
-}
+// above 64): the hardware estimate, v_rsq_f64, measured at most 2^-24.2
+// relative on 4 M inputs in [1e-6, 64] on MI355X.  Its continuous use (the
+// near-field e term, w * min(1/r, 2) * eps) needs no more (a relative
+// 2^-24 per term against the compensated form's 1e-3 kcal/mol bound);
+// table-bin decisions made from it are guarded (see binGuarded).
+__device__ __forceinline__ double rsqrtFast(double x) { return __builtin_amdgcn_rsq(x); }
 
 // floor(100 r) for 0 <= r <= 8 (bins 0..800: no clamp) from an approximation
-// r_a within 1e-12 relative of the exact r: equal to the reference's
-// angstromToIndex unless 100 r_a lies within 1e-7 of an integer, in which
-// case `edge` is set and the caller recomputes r in the reference's exact
-// operation order.
+// r_a within 2^-23 relative of the exact r (100 r_a within 1e-4 of 100 r):
+// equal to the reference's angstromToIndex unless 100 r_a lies within 2e-4
+// of an integer, in which case `edge` is set and the caller recomputes r in
+// the reference's exact operation order.
 __device__ __forceinline__ int binGuarded(double r, bool &edge)
 {
     const double t = r * 100.0;
-    edge = __builtin_fabs(__builtin_amdgcn_fract(t) - 0.5) > 0.5 - 1e-7;
+    edge = __builtin_fabs(__builtin_amdgcn_fract(t) - 0.5) > 0.5 - 2e-4;
     return int(t);
 }
 
