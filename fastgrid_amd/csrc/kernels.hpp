@@ -5,9 +5,11 @@
 //    electrostatic map.  One workgroup per brick of 8x8x4 grid points, one
 //    point per lane.  Candidate atoms (those within 8 A of the brick) are
 //    streamed in a uniform order, so every receptor-atom branch (H-bond
-//    class, disorder, per-map probe class) is wave-uniform.  All arithmetic
-//    is FP64 in the reference's operation order, so results match the
-//    reference CPU path (FG/CalculateGridmaps.cpp:84-316).
+//    class, disorder, per-map probe class) is wave-uniform.  The affinity
+//    and desolvation sums are FP64 in the reference's operation order with
+//    exactly its table bins, so they match the reference CPU path
+//    (FG/CalculateGridmaps.cpp:84-316); the near-field e term takes 1/r from
+//    the hardware FP64 reciprocal square root (2^-24 relative).
 //  * k_elec: far-field electrostatics in FP32, persistent: workgroups of 256
 //    lanes take work items (2D bricks of 8, 16 or 32 lanes along x, or
 //    linear items of 256 lane units) from a counter, each lane PX points
