@@ -449,10 +449,11 @@ constexpr int kMaxTypes = 32;  // receptor types the type-sorted lists support
 // desolvation value are read through the vector L1.  The per-lane table
 // reads of the pair loop (one bin per lane) are gathers either way; splitting
 // them keeps the LDS and the L1 busy at the same time.
+constexpr int ldsMapsOf(int MM) { return MM / 2 > 4 ? 4 : MM / 2 > 0 ? MM / 2 : 1; }
 template <int MM>
 constexpr int ldsMaps()
 {
-    return MM / 2 > 4 ? 4 : MM / 2 > 0 ? MM / 2 : 1;
+    return ldsMapsOf(MM);
 }
 template <int MM, bool kAffinity, bool kTyped>
 struct CutoffLds {
@@ -683,11 +684,16 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                 }
                 const double S = a.desolv[idx];
                 // 32-bit offsets (job creation bounds the table below 2^31 entries)
-                const unsigned rowOff = __umul24(unsigned(type * kPairBins + idx), unsigned(M));
+                const unsigned rowIdx = unsigned(type * kPairBins + idx);
+                const unsigned rowOff = __umul24(rowIdx, unsigned(M));
                 const double *grow = a.pair + rowOff;
                 const double *lrow = kTyped ? &L.row[idx * ldsMaps<MM>()] : grow;
+                const double *ghi = kTyped ? a.pairHi + __umul24(rowIdx, unsigned(a.hiStride)) : grow;
                 // map m's entry: LDS for the first MM/2 maps of the typed kernel
-#define FG_ROW(m) ((kTyped && (m) < ldsMaps<MM>()) ? lrow[(m)] : grow[(m)])
+                // (at most 4), the compact table for the others
+#define FG_ROW(m)                                                                        \
+    ((kTyped && (m) < ldsMaps<MM>()) ? lrow[(m)]                                          \
+                                     : (kTyped ? ghi[(m) - ldsMaps<MM>()] : grow[(m)]))
                 const double *row = grow;
                 const double *rowH =
                     (dis && a.hydrogen >= 0)
@@ -807,6 +813,19 @@ __device__ __forceinline__ void fmapBrick(const CutoffArgs &a, int b)
             if (d2 < m2) m2 = d2;
         }
     if (valid) fmap[(int64_t(iz) * a.g.ny + iy) * a.g.nx + ix] = float(sqrt(m2));
+}
+
+// The compact table of the maps from `first` on (CutoffArgs::pairHi) from
+// the full table of `rows` rows, one entry per lane, then `pad` zeros.
+__global__ __launch_bounds__(256) void k_pair_hi(const double *pair, int M, int first, int stride, int64_t rows,
+                                                 int pad, double *out)
+{
+    const int64_t q = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    const int64_t n = rows * stride;
+    if (q >= n + pad) return;
+    const int64_t row = q / stride;
+    const int k = int(q - row * stride);
+    out[q] = q < n && first + k < M ? pair[row * M + first + k] : 0.0;
 }
 
 // Covalent map values alone (fg_covalent_map), one point per lane.
