@@ -159,10 +159,10 @@ struct fg_job {
     unsigned runWhat = 0;
     int runKey[5] = {-1, 0, 0, 0, 0};  // precision, px, pz, tile, bxShift
     double *dPair = nullptr, *dRsph = nullptr, *dDesolv = nullptr, *dEps = nullptr, *dCovalent = nullptr;
-    // compact table of the maps the job's typed cutoff kernel does not stage
-    // in LDS (CutoffArgs::pairHi): maps hiFirst.. in rows of hiStride doubles
+    // compact table of the maps the job's typed cutoff kernel (compiled for
+    // hiMM maps) does not stage in LDS (CutoffArgs::pairHi)
     double *dPairHi = nullptr;
-    int hiFirst = 0, hiStride = 2, pairRows = 1;
+    int hiMM = 4;
     double *dXYZ = nullptr;
     int32_t *dHbond = nullptr, *dCellStart = nullptr, *dCellAtoms = nullptr;
     fgk::CellGrid cg{};
@@ -328,9 +328,6 @@ int cutKernelMM(const fg_job *j)
     return j->M <= 4 ? 4 : (j->M <= 6 && j->KH <= 4) ? 6 : (j->M <= 8 && j->KH <= 4) ? 8 : 16;
 }
 
-// Doubles per row of the compact table of the maps from `first` on.
-int hiStrideOf(int M, int first) { return std::max(2, (M - first + 1) & ~1); }
-
 fgk::CutoffArgs cutoffArgs(const fg_job *j, int doElecNear)
 {
     fgk::CutoffArgs a{};
@@ -355,7 +352,6 @@ fgk::CutoffArgs cutoffArgs(const fg_job *j, int doElecNear)
     for (int m = 0; m < j->M && m < 16; ++m) a.covMask |= j->mp[size_t(m)].isCovalent ? 1 << m : 0;
     a.pair = j->dPair;
     a.pairHi = j->dPairHi;
-    a.hiStride = j->hiStride;
     a.rsph = j->dRsph;
     a.desolv = j->dDesolv;
     a.eps = j->dEps;
@@ -848,16 +844,15 @@ int buildCutoff(fg_job *j, const fg_job_desc &d)
     }
     if (int rc = devUpload(j, j->dPair, perm.data(), perm.size())) return rc;
     if (int rc = devUpload(j, j->dRsph, rsph.data(), rsph.size())) return rc;
-    // the compact table for the job's own typed kernel; 16 zero doubles at
-    // its end, where a kernel compiled for more maps than the job has reads
-    // the unused ones of the last row
-    j->pairRows = std::max(d.tables.n_types, 1) * fgk::kPairBins;
-    j->hiFirst = fgk::ldsMapsOf(cutKernelMM(j));
-    j->hiStride = hiStrideOf(j->M, j->hiFirst);
-    std::vector<double> compact(size_t(j->pairRows) * size_t(j->hiStride) + 16, 0.0);
-    for (size_t r = 0; r < size_t(j->pairRows); ++r)
-        for (int k = 0; j->hiFirst + k < j->M; ++k)
-            compact[r * size_t(j->hiStride) + size_t(k)] = perm[r * size_t(Mx) + size_t(j->hiFirst + k)];
+    // the compact table for the job's own typed kernel (its rows cover every
+    // map the kernel is compiled for; those the job lacks hold 0)
+    const size_t rows = size_t(std::max(d.tables.n_types, 1)) * fgk::kPairBins;
+    j->hiMM = cutKernelMM(j);
+    const int first = fgk::ldsMapsOf(j->hiMM), stride = fgk::hiStrideOf(j->hiMM);
+    std::vector<double> compact(rows * size_t(stride), 0.0);
+    for (size_t r = 0; r < rows; ++r)
+        for (int k = 0; first + k < j->M && k < stride; ++k)
+            compact[r * size_t(stride) + size_t(k)] = perm[r * size_t(Mx) + size_t(first + k)];
     if (int rc = devUpload(j, j->dPairHi, compact.data(), compact.size())) return rc;
     std::vector<double> zt(fgk::kTableLen, 0.0);
     if (int rc = devUpload(j, j->dDesolv, (j->hasGeom && d.tables.desolv) ? d.tables.desolv : zt.data(), zt.size()))
@@ -1279,9 +1274,6 @@ struct fg_batch {
     fgk::ListArgs *dList = nullptr;  // the candidate-list passes of every job
     fgk::ElecArgs *dElec = nullptr;
     int32_t *dCutBase = nullptr, *dElecBase = nullptr, *dFarBase = nullptr, *dCounters = nullptr;
-    // compact tables for jobs whose own cutoff kernel stages a different
-    // number of maps in LDS than the batch's
-    std::vector<double *> dPairHi;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -1301,7 +1293,6 @@ void destroyBatch(fg_batch *b)
                     static_cast<void *>(b->dCutBase), static_cast<void *>(b->dElecBase),
                     static_cast<void *>(b->dFarBase), static_cast<void *>(b->dCounters)})
         if (p) (void)hipFree(p);
-    for (double *p : b->dPairHi) (void)hipFree(p);
     delete b;
 }
 
@@ -1536,20 +1527,6 @@ int buildBatch(fg_ctx *ctx, fg_batch *b, fg_job *const *jobs, int n, unsigned wh
     for (fg_job *j : b->jobs) {
         ca.push_back(cutoffArgs(j, b->near ? 1 : 0));
         ca.back().prio = b->cutPrio;
-        const int first = fgk::ldsMapsOf(b->MM);
-        if (b->aff && b->typed && first != j->hiFirst && j->M > 0) {
-            // the batch's kernel stages `first` maps in LDS: a compact table from there
-            const int stride = hiStrideOf(j->M, first);
-            const int64_t n = int64_t(j->pairRows) * stride + 16;
-            double *t = nullptr;
-            FG_HIP(hipMalloc(&t, size_t(n) * sizeof(double)));
-            b->dPairHi.push_back(t);
-            hipLaunchKernelGGL(fgk::k_pair_hi, dim3(unsigned((n + 255) / 256)), dim3(256), 0, b->ctx->stream, j->dPair,
-                               j->M, first, stride, int64_t(j->pairRows), 16, t);
-            FG_HIP(hipGetLastError());
-            ca.back().pairHi = t;
-            ca.back().hiStride = stride;
-        }
         if (!(b->typed && b->aff)) ca.back().brickList = j->dList;
         la.push_back(listArgs(j));
         if (!(b->typed && b->aff)) la.back().sorted = nullptr;  // no type sort

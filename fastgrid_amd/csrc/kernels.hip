@@ -450,6 +450,9 @@ constexpr int kMaxTypes = 32;  // receptor types the type-sorted lists support
 // reads of the pair loop (one bin per lane) are gathers either way; splitting
 // them keeps the LDS and the L1 busy at the same time.
 constexpr int ldsMapsOf(int MM) { return MM / 2 > 4 ? 4 : MM / 2 > 0 ? MM / 2 : 1; }
+// Doubles per row of the compact table (CutoffArgs::pairHi) of a typed kernel
+// compiled for MM maps: its maps past the LDS ones, padded to an even count.
+constexpr int hiStrideOf(int MM) { return (MM - ldsMapsOf(MM) + 1) / 2 * 2; }
 template <int MM>
 constexpr int ldsMaps()
 {
@@ -482,7 +485,13 @@ constexpr int cutWaves()
 // kPointAng: the job has atoms whose directional terms depend on the point
 // (bonded donors/acceptors); without them the FP64 transcendental code and its
 // registers are left out.
-template <int MM, int KH, bool kAffinity, bool kPointAng, bool kTyped>
+// kCompact (typed kernel): the maps not in LDS come from the compact table
+// (CutoffArgs::pairHi).  Its gathers touch half as many cache lines, which
+// pays where the kernel waits on them (a single job's cutoff phase); in a
+// batch, where the cutoff kernel shares the VALU with the e-kernel for most
+// of its run, its extra address arithmetic costs more than that saves, and the
+// rows of all maps are read instead.
+template <int MM, int KH, bool kAffinity, bool kPointAng, bool kTyped, bool kCompact = false>
 __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLds<MM, kAffinity, kTyped> &L)
 {
     // the waves of a cutoff phase that carries a large share of the work
@@ -688,12 +697,12 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
                 const unsigned rowOff = __umul24(rowIdx, unsigned(M));
                 const double *grow = a.pair + rowOff;
                 const double *lrow = kTyped ? &L.row[idx * ldsMaps<MM>()] : grow;
-                const double *ghi = kTyped ? a.pairHi + __umul24(rowIdx, unsigned(a.hiStride)) : grow;
+                const double *ghi = kCompact ? a.pairHi + rowIdx * unsigned(hiStrideOf(MM)) : grow;
                 // map m's entry: LDS for the first MM/2 maps of the typed kernel
-                // (at most 4), the compact table for the others
+                // (at most 4), the compact table (or the full row) for the others
 #define FG_ROW(m)                                                                        \
     ((kTyped && (m) < ldsMaps<MM>()) ? lrow[(m)]                                          \
-                                     : (kTyped ? ghi[(m) - ldsMaps<MM>()] : grow[(m)]))
+                                     : (kCompact ? ghi[(m) - ldsMaps<MM>()] : grow[(m)]))
                 const double *row = grow;
                 const double *rowH =
                     (dis && a.hydrogen >= 0)
@@ -776,7 +785,7 @@ __attribute__((amdgpu_waves_per_eu(cutWaves<MM, KH, kAffinity, kTyped>()))) void
     const int nb = a.nbx * a.nby * a.nbz;
     const int full = (nb / 8) * 8, g = int(blockIdx.x);
     const int b = g < full ? (g % 8) * (full / 8) + g / 8 : g;
-    cutoffBrick<MM, KH, kAffinity, kPointAng, kTyped>(a, b, L);
+    cutoffBrick<MM, KH, kAffinity, kPointAng, kTyped, kTyped>(a, b, L);
 }
 
 // ------------------------------------------------------------------------
@@ -813,19 +822,6 @@ __device__ __forceinline__ void fmapBrick(const CutoffArgs &a, int b)
             if (d2 < m2) m2 = d2;
         }
     if (valid) fmap[(int64_t(iz) * a.g.ny + iy) * a.g.nx + ix] = float(sqrt(m2));
-}
-
-// The compact table of the maps from `first` on (CutoffArgs::pairHi) from
-// the full table of `rows` rows, one entry per lane, then `pad` zeros.
-__global__ __launch_bounds__(256) void k_pair_hi(const double *pair, int M, int first, int stride, int64_t rows,
-                                                 int pad, double *out)
-{
-    const int64_t q = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    const int64_t n = rows * stride;
-    if (q >= n + pad) return;
-    const int64_t row = q / stride;
-    const int k = int(q - row * stride);
-    out[q] = q < n && first + k < M ? pair[row * M + first + k] : 0.0;
 }
 
 // Covalent map values alone (fg_covalent_map), one point per lane.

@@ -116,11 +116,11 @@ struct CutoffArgs {
                                               // and no map is covalent
     int32_t covMask;                          // bit m: map m (kernel order) is a covalent map
     const double *pair;    // [T][kPairBins][M], maps in kernel order
-    // the maps a typed kernel does not stage in LDS (from its LDS count on),
-    // [T][kPairBins][hiStride], each row zero-padded to an even count: two
-    // maps per 16-byte load from half as many cache lines as rows of all maps
+    // the maps the typed single-job kernel does not stage in LDS (from its
+    // LDS count on), [T][kPairBins][hiStrideOf(MM)] for the kernel's MM, each
+    // row zero-padded: two maps per 16-byte load from half as many cache lines
+    // as rows of all maps (unused by the batch kernel)
     const double *pairHi;
-    int32_t hiStride;
     const double *rsph;    // [T][kPairBins][M]: min(max(pair / 100, 0), 1) (FG/CalculateGridmaps.cpp:210)
     const double *desolv;  // [8192]
     const double *eps;     // [8192]
