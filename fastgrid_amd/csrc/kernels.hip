@@ -513,6 +513,9 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
     const double gz = (iz - a.g.dz2) * a.g.spacing;
     const int64_t p = valid ? (int64_t(iz) * a.g.ny + iy) * a.g.nx + ix : 0;
     const int M = a.M;
+    // a wave with no point in the grid (half of a partial brick column) only
+    // takes part in the staging and its barriers
+    const bool live = __ballot(valid) != 0;
 
     if (kTyped) {
         // rows of the brick-independent tables for r <= 8 A (the barrier of
@@ -529,7 +532,7 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
         // uniform reads of read-only inputs: scalar loads
         const FG_CONST int32_t *hStart = constView(a.hStart), *hList = constView(a.hList);
         const FG_CONST CutAtom *atoms = constView(a.atoms);
-        const int h0 = hStart[b], h1 = hStart[b + 1];
+        const int h0 = hStart[b], h1 = live ? hStart[b + 1] : h0;
         for (int c = h0; c < h1; ++c) {
             const int ia = uniform(hList[c]);
             const double ax = atoms[ia].x, ay = atoms[ia].y, az = atoms[ia].z;
@@ -621,7 +624,7 @@ __device__ __forceinline__ void cutoffBrick(const CutoffArgs &a, int b, CutoffLd
         advance();
         if (run < nRuns) prefetch();
         __syncthreads();
-        {
+        if (live) {
             for (int c = 0; c < nc; ++c) {
                 const CutAtom &A = sAtom[c];
                 const double dx = A.x - gx, dy = A.y - gy, dz = A.z - gz;
