@@ -150,6 +150,7 @@ struct fg_job {
     double *dFar = nullptr;  // [P] far-field e of the compensated form
     double *dCharge = nullptr;   // [N] partial charges (when given), for the summary
     double *dSummary = nullptr;  // [M + 2][2] min/max of the rounded maps, then the total charge
+    double *dSummaryPart = nullptr;  // [M + 2][kSumChunks][2] the maps' min/max per chunk
     int32_t *dMilli = nullptr;   // [M + 2][P] rounded thousandths (fg_job_fetch_milli), then an overflow flag
     int32_t *dCounter = nullptr;  // work counter of the persistent e-kernel
     int elecPerCU[2][2][2] = {};  // e-kernel workgroups per CU by [px - 1][pz == 8][compensated]
@@ -1743,11 +1744,21 @@ int jobBuildHost(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
     const int n[3] = {g.nx, g.ny, g.nz}, dv[3] = {g.div2x, g.div2y, g.div2z};
     for (int k = 0; k < 3; ++k)
         gridMaxAbs = std::max({gridMaxAbs, std::fabs(double(0 - dv[k]) * g.spacing), std::fabs(double(n[k] - 1 - dv[k]) * g.spacing)});
-    // points per lane along z: 8 halves the per-atom overhead but needs enough
-    // z-bricks to fill the GPU
+    // default shape.  Small grids: 2D bricks of 16 lanes along x, one point
+    // per lane along x and 4 along z.  Grids of at least 1024 such bricks of
+    // 8 planes: linear items of 2 x 4 points per lane, tile 128 (what the
+    // autotuner picks on 127^3 and 161^3 grids), with 8 planes per lane when
+    // that still leaves about 1.25 items per workgroup of the persistent
+    // launch (3 per CU).
     {
         const long bricks8 = long((g.nx + 15) / 16) * ((g.ny + 15) / 16) * ((g.nz + 7) / 8);
-        j->pz = bricks8 >= 1024 ? 8 : 4;
+        if (bricks8 >= 1024) {
+            const long items8 = long(g.nx) * g.ny * g.nz / (2 * 8 * fgk::kElecThreads);
+            j->px = 2;
+            j->bxShift = 0;
+            j->tile = 128;
+            j->pz = items8 >= 960 ? 8 : 4;
+        }
     }
     // inputs and outputs staged for one arena
     int rc = buildCutoff(j, *d);
@@ -1757,6 +1768,7 @@ int jobBuildHost(fg_ctx *ctx, const fg_job_desc *d, fg_job **out)
     if (!rc) rc = devAlloc(j, j->dFar, size_t(j->P));
     if (!rc) rc = devAlloc(j, j->dCounter, size_t(1));
     if (!rc) rc = devAlloc(j, j->dSummary, size_t(2) * size_t(j->M + 2) + 1);
+    if (!rc) rc = devAlloc(j, j->dSummaryPart, size_t(2) * size_t(j->M + 2) * fgk::kSumChunks);
     if (!rc) rc = devAlloc(j, j->dMilli, size_t(j->M + 2) * size_t(j->P) + 1);
     if (!rc && A.charge && j->N > 0) rc = devUploadDirect(j, j->dCharge, A.charge, size_t(j->N));
     if (!rc && j->floating) rc = devAlloc(j, j->dFmap, size_t(j->P));
@@ -2363,8 +2375,13 @@ int fg_job_summary(fg_ctx *ctx, fg_job *j, fg_summary *out)
     if (j->ctx != ctx) return fgc::fail(FG_E_ARG, "fg_job_summary: job does not belong to this context");
     FG_HIP(hipSetDevice(ctx->device));
     const int nm = j->M + 2;
-    hipLaunchKernelGGL(fgk::k_job_summary, dim3(unsigned(nm + 1)), dim3(256), 0, ctx->stream, j->dMaps, j->P, nm,
-                       j->dCharge, j->dCharge ? j->N : 0, j->dSummary);
+    if (nm > 256) return fgc::fail(FG_E_ARG, "fg_job_summary: more than 256 maps");
+    hipLaunchKernelGGL(fgk::k_job_summary, dim3(unsigned(fgk::kSumChunks), unsigned(nm + 1)), dim3(256), 0,
+                       ctx->stream, j->dMaps, j->P, nm, j->dCharge, j->dCharge ? j->N : 0, j->dSummaryPart,
+                       j->dSummary);
+    FG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(fgk::k_job_summary_final, dim3(1), dim3(256), 0, ctx->stream, j->dSummaryPart, nm,
+                       j->dSummary);
     FG_HIP(hipGetLastError());
     std::vector<double> h(size_t(2 * nm + 1));
     FG_HIP(hipMemcpyAsync(h.data(), j->dSummary, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

@@ -192,20 +192,21 @@ int main(int argc, char **argv)
     if (fg_job_create(ctx, &desc, &job) != FG_OK || fg_job_set_plain_dielectric(job, o.plainDiel) != FG_OK)
         return fatal(0xbad, fg_last_error());
     const auto t2 = std::chrono::steady_clock::now();
-    if (fg_job_run(ctx, job, FG_RUN_ALL | (o.benchmark ? FG_RUN_PHASE_TIMING : 0u), o.precision) != FG_OK ||
-        fg_sync(ctx) != FG_OK)
+    if (fg_job_run(ctx, job, FG_RUN_ALL | (o.benchmark ? FG_RUN_PHASE_TIMING : 0u), o.precision) != FG_OK)
         return fatal(0xbad, fg_last_error());
+    // the maps as the files print them: integer thousandths rounded on the
+    // device (half the bytes of FP64); a job with values past 32-bit
+    // thousandths comes back in FP64.  The host buffers are made while the
+    // GPU computes.
+    std::vector<std::vector<int32_t>> milli(static_cast<size_t>(M + 2), std::vector<int32_t>(static_cast<size_t>(P)));
+    std::vector<int32_t *> mptrs(static_cast<size_t>(M + 2));
+    for (size_t m = 0; m < milli.size(); ++m) mptrs[m] = milli[m].data();
+    if (fg_sync(ctx) != FG_OK) return fatal(0xbad, fg_last_error());
     fg_timing tm{};
     fg_last_timing(ctx, &tm);
     // the log's map extremes and total charge, computed on the GPU
     fg_summary sm{};
     if (fg_job_summary(ctx, job, &sm) != FG_OK) return fatal(0xbad, fg_last_error());
-    // the maps as the files print them: integer thousandths rounded on the
-    // device (half the bytes of FP64); a job with values past 32-bit
-    // thousandths comes back in FP64
-    std::vector<std::vector<int32_t>> milli(static_cast<size_t>(M + 2), std::vector<int32_t>(static_cast<size_t>(P)));
-    std::vector<int32_t *> mptrs(static_cast<size_t>(M + 2));
-    for (size_t m = 0; m < milli.size(); ++m) mptrs[m] = milli[m].data();
     std::vector<std::vector<double>> host;
     std::vector<double *> ptrs(static_cast<size_t>(nOut), nullptr);
     const bool asMilli = fg_job_fetch_milli(ctx, job, mptrs.data()) == FG_OK;

@@ -1226,28 +1226,44 @@ __global__ __launch_bounds__(256) void k_round_milli(const double *maps, int64_t
     out[i] = int32_t(k);
 }
 
-// The job's summary: workgroup m < nMaps the minimum and maximum of map m's
-// rounded values (GridMapList::calculateEnergyMinMax over the stored, rounded
-// energies, FG/GridMap.cpp:214-224, starting from +-BIG = 1e12), the last
-// workgroup the receptor's total charge summed in atom order by one lane, the
-// reference's own loop (FG/InputDataLoader.cpp:325).  Min and max are exact,
-// so the reduction order does not matter.  out: [nMaps][2], then the charge.
+// The job's summary: the minimum and maximum of each map's rounded values
+// (GridMapList::calculateEnergyMinMax over the stored, rounded energies,
+// FG/GridMap.cpp:214-224, starting from +-BIG = 1e12) and the receptor's total
+// charge summed in atom order, the reference's own loop
+// (FG/InputDataLoader.cpp:325).  Min and max are exact, so the reduction
+// order does not matter: workgroup (c, m) of k_job_summary scans chunk c of
+// map m into part[m][c], and k_job_summary_final combines the chunks.
+// Workgroup (0, nMaps) sums the charges: 256 at a time staged in LDS and added
+// in order by one lane, the next 256 loading meanwhile.
+constexpr int kSumChunks = 64;  // workgroups per map
+
 __global__ __launch_bounds__(256) void k_job_summary(const double *maps, int64_t P, int nMaps, const double *charge,
-                                                      int N, double *out)
+                                                      int N, double *part, double *out)
 {
-    __shared__ double sLo[4], sHi[4];
-    const int m = int(blockIdx.x);
+    const int m = int(blockIdx.y), c = int(blockIdx.x), tid = threadIdx.x;
     if (m == nMaps) {
-        if (threadIdx.x == 0) {
-            double q = 0;
-            for (int i = 0; i < N; ++i) q += charge[i];
-            out[2 * nMaps] = q;
+        if (c != 0) return;
+        __shared__ double sQ[256];
+        double q = 0, next = tid < N ? charge[tid] : 0.0;
+        for (int i0 = 0; i0 < N; i0 += 256) {
+            __syncthreads();  // lane 0 is done with the previous 256
+            sQ[tid] = next;
+            __syncthreads();
+            if (i0 + 256 + tid < N) next = charge[i0 + 256 + tid];
+            if (tid == 0) {
+                const int n = min(256, N - i0);
+                for (int k = 0; k < n; ++k) q += sQ[k];
+            }
         }
+        if (tid == 0) out[2 * nMaps] = q;
         return;
     }
+    __shared__ double sLo[4], sHi[4];
+    const int64_t len = (P + kSumChunks - 1) / kSumChunks;
+    const int64_t p0 = int64_t(c) * len, p1 = min(P, p0 + len);
     double lo = 1.0e12, hi = -1.0e12;
     const double *v = maps + int64_t(m) * P;
-    for (int64_t p = threadIdx.x; p < P; p += 256) {
+    for (int64_t p = p0 + tid; p < p1; p += 256) {
         const double r = round3(v[p]);
         lo = __builtin_fmin(lo, r);
         hi = __builtin_fmax(hi, r);
@@ -1256,19 +1272,33 @@ __global__ __launch_bounds__(256) void k_job_summary(const double *maps, int64_t
         lo = __builtin_fmin(lo, __shfl_xor(lo, o));
         hi = __builtin_fmax(hi, __shfl_xor(hi, o));
     }
-    if ((threadIdx.x & 63) == 0) {
-        sLo[threadIdx.x >> 6] = lo;
-        sHi[threadIdx.x >> 6] = hi;
+    if ((tid & 63) == 0) {
+        sLo[tid >> 6] = lo;
+        sHi[tid >> 6] = hi;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         for (int w = 1; w < 4; ++w) {
             lo = __builtin_fmin(lo, sLo[w]);
             hi = __builtin_fmax(hi, sHi[w]);
         }
-        out[2 * m] = lo;
-        out[2 * m + 1] = hi;
+        part[2 * (int64_t(m) * kSumChunks + c)] = lo;
+        part[2 * (int64_t(m) * kSumChunks + c) + 1] = hi;
     }
+}
+
+// out[2m], out[2m + 1]: map m's minimum and maximum over its chunks (one lane per map)
+__global__ __launch_bounds__(256) void k_job_summary_final(const double *part, int nMaps, double *out)
+{
+    const int m = int(threadIdx.x);
+    if (m >= nMaps) return;
+    double lo = 1.0e12, hi = -1.0e12;
+    for (int c = 0; c < kSumChunks; ++c) {
+        lo = __builtin_fmin(lo, part[2 * (m * kSumChunks + c)]);
+        hi = __builtin_fmax(hi, part[2 * (m * kSumChunks + c) + 1]);
+    }
+    out[2 * m] = lo;
+    out[2 * m + 1] = hi;
 }
 
 // ------------------------------------------------------------------------
