@@ -75,6 +75,44 @@ void formatRange(MapValues v, int64_t lo, int64_t hi, std::string &buf, double &
     mx = b;
 }
 
+// Thousandths [lo, hi) of a map rounded on the device: the text formatValue
+// gives their values k * 0.001 (0 exactly when k is 0), written from the
+// integers directly; mn and mx as formatRange gives them.
+void formatMilliRange(const int32_t *k, int64_t lo, int64_t hi, std::string &buf, double &mn, double &mx)
+{
+    buf.resize(size_t(hi - lo) * 16);
+    char *p = &buf[0];
+    int32_t a = INT32_MAX, b = INT32_MIN;
+    for (int64_t i = lo; i < hi; ++i) {
+        const int32_t v = k[i];
+        a = std::min(a, v);
+        b = std::max(b, v);
+        if (v == 0) {
+            *p++ = '0';
+            *p++ = '\n';
+            continue;
+        }
+        if (v < 0) *p++ = '-';
+        uint32_t u = v < 0 ? 0u - uint32_t(v) : uint32_t(v);
+        uint32_t ip = u / 1000, fp = u % 1000;
+        char tmp[12];
+        int n = 0;
+        do {
+            tmp[n++] = char('0' + ip % 10);
+            ip /= 10;
+        } while (ip);
+        while (n) *p++ = tmp[--n];
+        *p++ = '.';
+        *p++ = char('0' + fp / 100);
+        *p++ = char('0' + fp / 10 % 10);
+        *p++ = char('0' + fp % 10);
+        *p++ = '\n';
+    }
+    buf.resize(size_t(p - &buf[0]));
+    mn = hi > lo ? double(a) * 0.001 : 1e12;
+    mx = hi > lo ? double(b) * 0.001 : -1e12;
+}
+
 // Floating-grid values [lo, hi) (floats held as doubles), printed as the
 // reference prints them: "%.3f" of the value, no rounding step, no "0" form.
 void formatFloatRange(const double *v, int64_t lo, int64_t hi, std::string &buf)
@@ -138,39 +176,67 @@ int writeOutputs(Prepared &p, double *const *maps, const int32_t *const *milli, 
     const int64_t P = p.P;
     p.mapMin.assign(size_t(nm), 0.0);
     p.mapMax.assign(size_t(nm), 0.0);
-    std::vector<std::string> chunks(static_cast<size_t>(threads));
+    // The maps are formatted in parallel chunks into one of two buffer sets
+    // while a writer thread writes the previous map from the other.
+    std::vector<std::string> chunkSets[2] = {std::vector<std::string>(size_t(threads)),
+                                             std::vector<std::string>(size_t(threads))};
     std::vector<double> mn(static_cast<size_t>(threads)), mx(static_cast<size_t>(threads));
+    std::thread writer;
+    bool writeOk = true;  // set by the writer thread, read after joining it
+    auto joinWriter = [&]() {
+        if (writer.joinable()) writer.join();
+    };
+    auto writeChunks = [&hdr, hl](FILE *mf, const std::vector<std::string> &chunks) {
+        bool ok = std::fwrite(hdr, size_t(hl), 1, mf) == 1;
+        for (const std::string &c : chunks)
+            if (!c.empty()) ok = ok && std::fwrite(c.data(), c.size(), 1, mf) == 1;
+        return (std::fclose(mf) == 0) && ok;
+    };
+    int set = 0;
     for (int m = 0; m < nm; ++m) {
         const MapValues mv{maps ? maps[m] : nullptr, milli ? milli[m] : nullptr};
         if (p.maps[size_t(m)].filename.empty() || (!mv.v && !mv.milli)) continue;
+        std::vector<std::string> &chunks = chunkSets[set];
         std::vector<std::thread> pool;
         for (int t = 0; t < threads; ++t) {
             int64_t lo = P * t / threads, hi = P * (t + 1) / threads;
-            pool.emplace_back(formatRange, mv, lo, hi, std::ref(chunks[size_t(t)]), std::ref(mn[size_t(t)]),
-                              std::ref(mx[size_t(t)]));
+            if (mv.milli)
+                pool.emplace_back(formatMilliRange, mv.milli, lo, hi, std::ref(chunks[size_t(t)]),
+                                  std::ref(mn[size_t(t)]), std::ref(mx[size_t(t)]));
+            else
+                pool.emplace_back(formatRange, mv, lo, hi, std::ref(chunks[size_t(t)]), std::ref(mn[size_t(t)]),
+                                  std::ref(mx[size_t(t)]));
         }
         for (auto &th : pool) th.join();
+        double a = 1e12, b = -1e12;
+        for (int t = 0; t < threads; ++t) {
+            a = std::min(a, mn[size_t(t)]);
+            b = std::max(b, mx[size_t(t)]);
+        }
+        p.mapMin[size_t(m)] = a;
+        p.mapMax[size_t(m)] = b;
         FILE *mf = std::fopen(p.maps[size_t(m)].filename.c_str(), "w");
+        joinWriter();  // the previous map is written; its buffer set is free
         if (!mf) {
             err = "Cannot open grid map \"" + p.maps[size_t(m)].filename + "\" for writing.";
             return -2;
         }
-        bool ok = std::fwrite(hdr, size_t(hl), 1, mf) == 1;
-        double a = 1e12, b = -1e12;
-        for (int t = 0; t < threads; ++t) {
-            if (!chunks[size_t(t)].empty())
-                ok = ok && std::fwrite(chunks[size_t(t)].data(), chunks[size_t(t)].size(), 1, mf) == 1;
-            a = std::min(a, mn[size_t(t)]);
-            b = std::max(b, mx[size_t(t)]);
-        }
-        ok = (std::fclose(mf) == 0) && ok;
-        if (!ok) {
+        if (!writeOk) {
+            std::fclose(mf);
             err = "Not enough disk space.";
             return -2;
         }
-        p.mapMin[size_t(m)] = a;
-        p.mapMax[size_t(m)] = b;
+        writer = std::thread([&writeOk, &writeChunks, mf, &chunks]() {
+            if (!writeChunks(mf, chunks)) writeOk = false;
+        });
+        set ^= 1;
     }
+    joinWriter();
+    if (!writeOk) {
+        err = "Not enough disk space.";
+        return -2;
+    }
+    std::vector<std::string> &chunks = chunkSets[0];
     const double *fv = maps ? maps[nm] : fmapValues;
     if (fmap && fv) {
         std::vector<std::thread> pool;
@@ -184,12 +250,7 @@ int writeOutputs(Prepared &p, double *const *maps, const int32_t *const *milli, 
             err = "can't open grid map \"" + p.fmapPath + "\" for writing.";
             return -2;
         }
-        bool ok = std::fwrite(hdr, size_t(hl), 1, mf) == 1;
-        for (int t = 0; t < threads; ++t)
-            if (!chunks[size_t(t)].empty())
-                ok = ok && std::fwrite(chunks[size_t(t)].data(), chunks[size_t(t)].size(), 1, mf) == 1;
-        ok = (std::fclose(mf) == 0) && ok;
-        if (!ok) {
+        if (!writeChunks(mf, chunks)) {
             err = "Not enough disk space.";
             return -2;
         }

@@ -108,3 +108,47 @@ def test_prepare_many_equals_one_by_one(tmp_path):
         assert "GPF 3" in str(ei.value) and ei.value.exit_code == 911
     finally:
         os.chdir(cwd)
+
+
+def test_milli_writer_equals_fp64_writer(tmp_path):
+    """The writer of integer thousandths (the command line's path, values
+    rounded on the device) prints the same files as the FP64 writer: maps of
+    random values at several scales with exact zeros, values that round to
+    zero from either side, rounding ties and values near the 32-bit limit of
+    thousandths, written from FP64, read back as thousandths and written
+    again."""
+    import fastgrid_amd as fg
+    d = str(tmp_path)
+    fg.synth_write(d, "w", 3, 200, "sphere", 8.0, (23, 21, 25), "A C HD N OA")
+    cwd = os.getcwd()
+    os.chdir(d)
+    try:
+        prep = fg.Prepared("w.gpf")
+        M, P = prep.n_maps, prep.arrays().P
+        rng = np.random.default_rng(5)
+        edge = np.array([0.0, -0.0, 0.0004, -0.0004, 0.0005, -0.0005, 0.0015, -0.0025, 1.0005, -2.9995,
+                         2147483.0, -2147483.0, 2147482.9996, 99999.9994])
+        maps = []
+        for m in range(M + 2):
+            v = rng.standard_normal(P) * 10.0 ** rng.integers(-3, 6, P)
+            v[rng.random(P) < 0.2] = 0.0
+            v[:edge.size] = edge
+            maps.append(v)
+        prep.write_outputs(maps)
+        files = sorted(f for f in os.listdir(".") if f.endswith((".map", ".fld")))
+        a = {f: open(f, "rb").read() for f in files}
+        fld = [f for f in files if f.endswith(".fld")][0]
+        names = [ln.split("file=")[1].split()[0] for ln in a[fld].decode().splitlines() if ln.startswith("variable")]
+        assert len(names) == M + 2
+        milli = []
+        for m in range(M + 2):
+            _, k = fx.read_map(names[m])
+            assert k.size == P and np.abs(k).max() < 2 ** 31
+            milli.append(k.astype(np.int32))
+        for f in files:
+            os.remove(f)
+        prep.write_outputs_milli(milli)
+        b = {f: open(f, "rb").read() for f in files}
+        assert a == b and len(a) == M + 3
+    finally:
+        os.chdir(cwd)
