@@ -181,17 +181,24 @@ int writeOutputs(Prepared &p, double *const *maps, const int32_t *const *milli, 
     std::vector<std::string> chunkSets[2] = {std::vector<std::string>(size_t(threads)),
                                              std::vector<std::string>(size_t(threads))};
     std::vector<double> mn(static_cast<size_t>(threads)), mx(static_cast<size_t>(threads));
-    std::thread writer;
-    bool writeOk = true;  // set by the writer thread, read after joining it
-    auto joinWriter = [&]() {
-        if (writer.joinable()) writer.join();
-    };
     auto writeChunks = [&hdr, hl](FILE *mf, const std::vector<std::string> &chunks) {
         bool ok = std::fwrite(hdr, size_t(hl), 1, mf) == 1;
         for (const std::string &c : chunks)
             if (!c.empty()) ok = ok && std::fwrite(c.data(), c.size(), 1, mf) == 1;
         return (std::fclose(mf) == 0) && ok;
     };
+    std::thread writer;
+    bool writeOk = true;  // set by the writer thread, read after joining it
+    auto joinWriter = [&]() {
+        if (writer.joinable()) writer.join();
+    };
+    struct JoinAtExit {  // every way out of this function joins the writer
+        std::thread &t;
+        ~JoinAtExit()
+        {
+            if (t.joinable()) t.join();
+        }
+    } joinAtExit{writer};
     int set = 0;
     for (int m = 0; m < nm; ++m) {
         const MapValues mv{maps ? maps[m] : nullptr, milli ? milli[m] : nullptr};
