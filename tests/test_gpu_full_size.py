@@ -67,6 +67,9 @@ def test_L_full_grid_and_sampled_planes(ctx, tmp_path):
     import fastgrid_amd as fg
     a = _workload(str(tmp_path), bench.WORKLOAD_L)
     job, comp = _full_grid(ctx, a, (0, 63, 126))
+    # untuned, a grid this large runs linear items of 2 x 4 points per lane
+    # (about 1.3 items per workgroup at 8 planes would leave it underfilled)
+    assert job.shape == (4, 128, 256, 2)
     # other kernel shapes give the same bits at full size: 2D bricks with
     # 16 points per lane, and linear items that cross rows and z-slabs
     for shape in [(8, 64, 16, 2), (8, 128, 256, 2), (4, 32, 256, 1)]:
@@ -84,6 +87,7 @@ def test_X_sampled_planes_and_sign_symmetry(ctx, tmp_path):
     a = _workload(str(tmp_path), bench.WORKLOAD_X)
     assert a.n_atoms == 250000 and tuple(a.n) == (161, 161, 161)
     job, comp = _full_grid(ctx, a, (0, 97))
+    assert job.shape == (8, 128, 256, 2)  # untuned: linear items, 8 planes per lane
     job.close()
     b = copy.copy(a)
     b.w, b.charge = -a.w, -a.charge
