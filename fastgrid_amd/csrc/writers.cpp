@@ -222,8 +222,10 @@ int writeOutputs(Prepared &p, double *const *maps, const int32_t *const *milli, 
         }
         p.mapMin[size_t(m)] = a;
         p.mapMax[size_t(m)] = b;
+        // the previous map is written (its buffer set is free, and a GPF that
+        // names one file twice sees it rewritten, not cut short under the writer)
+        joinWriter();
         FILE *mf = std::fopen(p.maps[size_t(m)].filename.c_str(), "w");
-        joinWriter();  // the previous map is written; its buffer set is free
         if (!mf) {
             err = "Cannot open grid map \"" + p.maps[size_t(m)].filename + "\" for writing.";
             return -2;
